@@ -1,0 +1,94 @@
+/*
+ * zkmsm.h -- extensions of libzkmetal_amd.so beyond the zkMetal MSM ABI.
+ * Same conventions as zkmetal.h (status codes, 64 B Montgomery affine
+ * points, 32 B standard-form scalars, 96 B Jacobian results).
+ *
+ * None of these has a counterpart in the reference FFI; they expose the
+ * MI355X-side structure the reference lacks: a device-resident base table
+ * (the reference re-copies bases on every call, MSMEngine.swift:395-397),
+ * batched MSMs over prefixes of it (KZGEngine.batchCommit,
+ * Sources/zkMetal/KZG/KZGEngine.swift:151-179, is the natural caller), scalars
+ * already in device memory, per-stage timing and a request server.
+ */
+#ifndef ZKMSM_H
+#define ZKMSM_H
+
+#include "zkmetal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device-resident base table (SRS) ---------------------------------- */
+
+/* Upload n affine bases; the engine keeps [P_i, phi(P_i)] in HBM. */
+ZkMetalStatus zkmsm_srs_upload(ZkMetalMSMEngine engine, const uint8_t* points, uint32_t n);
+/* Synthetic table: P_i = k_i * G with k_i = zkmsm_gen_scalars(seed, n, 0)[i],
+ * computed on the device (benchmark / test inputs). */
+ZkMetalStatus zkmsm_srs_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n);
+/* Copy the first n bases back (64 B each). */
+ZkMetalStatus zkmsm_srs_download(ZkMetalMSMEngine engine, uint8_t* points, uint32_t n);
+uint32_t zkmsm_srs_size(ZkMetalMSMEngine engine);
+
+/* MSM j uses bases [0, counts[j]) of the resident table and the next
+ * counts[j] scalars of all_scalars.  counts[j] == 0 -> 96 zero bytes.
+ * All MSMs run in one launch sequence. */
+ZkMetalStatus zkmsm_msm_batch_resident(ZkMetalMSMEngine engine, const uint8_t* all_scalars,
+                                       const uint32_t* counts, uint32_t n_msms,
+                                       uint8_t* results);
+/* Same, with the scalars already in device memory (from zkmsm_device_alloc). */
+ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
+                                              const void* device_scalars,
+                                              const uint32_t* counts, uint32_t n_msms,
+                                              uint8_t* results);
+
+/* ---- device memory for callers that keep scalars in HBM ----------------- */
+ZkMetalStatus zkmsm_device_alloc(uint64_t bytes, void** device_ptr);
+ZkMetalStatus zkmsm_device_free(void* device_ptr);
+ZkMetalStatus zkmsm_copy_to_device(void* device_dst, const void* host_src, uint64_t bytes);
+ZkMetalStatus zkmsm_copy_to_host(void* host_dst, const void* device_src, uint64_t bytes);
+
+/* ---- tuning / introspection -------------------------------------------- */
+
+/* Window width for subsequent calls on this engine; 0 restores the
+ * per-size default.  Valid widths: 2..16. */
+ZkMetalStatus zkmsm_set_window_bits(ZkMetalMSMEngine engine, uint32_t c);
+/* Window width the default policy picks for an n-point MSM. */
+uint32_t zkmsm_default_window_bits(uint64_t n);
+
+/* Timing of the engine's last call.  out[0..5]: device ms of the stages
+ * decompose, sort, accumulate, merge, bucket-reduce, copy-out; out[6]:
+ * device total; out[7]: host tail ms; out[8]: wall ms of the call;
+ * out[9]: window bits of the first MSM; out[10]: entries per accumulation
+ * lane; out[11]: buckets per reduction group; out[12]: sorted entries;
+ * out[13]: accumulation lanes.  Returns the number of values written. */
+int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
+
+/* ---- seeded synthetic scalars (host) ------------------------------------ */
+/* dist 0: uniform in Fr; 1: 40% 0, 30% < 2^16, 10% in {1, r-1}, 20% uniform;
+ * 2: uniform 256-bit.  out: n x 32 B. */
+void zkmsm_gen_scalars(uint64_t seed, uint64_t n, int32_t dist, uint8_t* out);
+
+/* ---- primitive probes for GPU unit tests (host buffers) ------------------ */
+/* op 0 mul, 1 add, 2 sub, 3 inverse(a), 4 negate(a), 5 canonicalise(a);
+ * a, b, out: n x 32 B Montgomery field elements. */
+ZkMetalStatus zkmsm_test_fp(ZkMetalMSMEngine engine, int32_t op, const uint8_t* a,
+                            const uint8_t* b, uint8_t* out, uint32_t n);
+/* op 0: p + q, 1: 2p, 2: p + affine(q.X, q.Y); XYZZ points of 128 B. */
+ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,
+                              const uint8_t* q, uint8_t* out, uint32_t n);
+/* GLV split of n scalars: out n x 48 B = k1 (16 B), k2 (16 B), neg1, neg2 (u32). */
+ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, uint8_t* out,
+                             uint32_t n);
+
+/* Measured device rates at full occupancy: out3[0] = v_mad_u64_u32
+ * (32x32->64 multiply-add) per second, out3[1] = FIPS accumulate steps
+ * (multiply-add + carry add) per second, out3[2] = 256-bit Montgomery field
+ * multiplies per second.  out3[0] is the INT32-multiply roofline peak. */
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKMSM_H */

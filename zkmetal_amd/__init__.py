@@ -1,0 +1,271 @@
+"""zkmetal_amd -- BN254 G1 multi-scalar multiplication on AMD MI355X (gfx950).
+
+Python view of libzkmetal_amd.so (the C ABI in include/zkmetal.h and
+include/zkmsm.h).  `MetalMSM` mirrors the reference engine API
+(Sources/zkMetal/MSM/MSMEngine.swift:17,287: `MetalMSM().msm(points:scalars:)`)
+with the same argument meaning and error behaviour (MSMError.invalidInput for
+an empty or mismatched input, MSMError.noGPU when no device exists,
+MSMError.gpuError otherwise).  Every compute call goes through the HIP
+library; there is no CPU path.  Importing works without a GPU; creating an
+engine does not.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzkmetal_amd.so")
+
+SUCCESS, ERR_NO_GPU, ERR_INVALID_INPUT, ERR_GPU_ERROR, ERR_ALLOC_FAILED = 0, -1, -2, -3, -4
+
+_lib = None
+
+
+class MSMError(Exception):
+    """Mirror of the reference's MSMError (MSMEngine.swift:8-15)."""
+
+    def __init__(self, status, what=""):
+        self.status = status
+        names = {ERR_NO_GPU: "noGPU", ERR_INVALID_INPUT: "invalidInput",
+                 ERR_GPU_ERROR: "gpuError", ERR_ALLOC_FAILED: "allocFailed"}
+        super().__init__(f"{names.get(status, status)} {what}".strip())
+
+
+def lib():
+    """Load libzkmetal_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m zkmetal_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u8p, u32p = ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)
+    i32, u32, u64 = ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64
+    sig = {
+        "zkmetal_msm_engine_create": (i32, [ctypes.POINTER(vp)]),
+        "zkmetal_msm_engine_destroy": (None, [vp]),
+        "zkmetal_bn254_msm": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_auto": (i32, [vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u8": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u16": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u32": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u8_auto": (i32, [vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u16_auto": (i32, [vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_u32_auto": (i32, [vp, vp, u32, vp, vp, vp]),
+        "zkmetal_bn254_msm_batch": (i32, [vp, vp, vp, vp, u32, vp]),
+        "zkmetal_bn254_msm_batch_auto": (i32, [vp, vp, vp, u32, vp]),
+        "zkmetal_set_shader_dir": (None, [u8p]),
+        "zkmetal_gpu_available": (i32, []),
+        "zkmetal_version": (ctypes.c_char_p, []),
+        "zkmsm_srs_upload": (i32, [vp, vp, u32]),
+        "zkmsm_srs_generate": (i32, [vp, u64, u32]),
+        "zkmsm_srs_download": (i32, [vp, vp, u32]),
+        "zkmsm_srs_size": (u32, [vp]),
+        "zkmsm_msm_batch_resident": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_msm_batch_resident_device": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_device_alloc": (i32, [u64, ctypes.POINTER(vp)]),
+        "zkmsm_device_free": (i32, [vp]),
+        "zkmsm_copy_to_device": (i32, [vp, vp, u64]),
+        "zkmsm_copy_to_host": (i32, [vp, vp, u64]),
+        "zkmsm_set_window_bits": (i32, [vp, u32]),
+        "zkmsm_default_window_bits": (u32, [u64]),
+        "zkmsm_last_timing": (i32, [vp, ctypes.POINTER(ctypes.c_double), i32]),
+        "zkmsm_gen_scalars": (None, [u64, u64, i32, vp]),
+        "zkmsm_test_fp": (i32, [vp, i32, vp, vp, vp, u32]),
+        "zkmsm_test_xyzz": (i32, [vp, i32, vp, vp, vp, u32]),
+        "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
+        "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _buf(x):
+    """bytes / bytearray / numpy -> (ctypes pointer, keepalive)."""
+    if isinstance(x, np.ndarray):
+        a = np.ascontiguousarray(x)
+        return ctypes.c_void_p(a.ctypes.data), a
+    b = bytes(x)
+    cb = ctypes.create_string_buffer(b, len(b))
+    return ctypes.cast(cb, ctypes.c_void_p), cb
+
+
+def _check(rc, what=""):
+    if rc != SUCCESS:
+        raise MSMError(rc, what)
+
+
+def gpu_available():
+    return bool(lib().zkmetal_gpu_available())
+
+
+def version():
+    return lib().zkmetal_version().decode()
+
+
+def gen_scalars(seed, n, dist=0):
+    """n seeded scalars (32 B each) as a (n, 32) uint8 array."""
+    out = np.zeros((n, 32), dtype=np.uint8)
+    lib().zkmsm_gen_scalars(seed, n, dist, ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def default_window_bits(n):
+    return int(lib().zkmsm_default_window_bits(n))
+
+
+class DeviceBuffer:
+    """Device memory owned by Python (scalars kept resident in HBM)."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        _check(lib().zkmsm_device_alloc(nbytes, ctypes.byref(self.ptr)), "device_alloc")
+        self.nbytes = nbytes
+
+    def upload(self, host):
+        p, keep = _buf(host)
+        _check(lib().zkmsm_copy_to_device(self.ptr, p, self.nbytes), "copy_to_device")
+        del keep
+
+    def free(self):
+        if self.ptr:
+            lib().zkmsm_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+TIMING_KEYS = ["decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "copyout_ms",
+               "device_ms", "host_tail_ms", "wall_ms", "window_bits", "entries_per_lane",
+               "buckets_per_group", "entries", "lanes"]
+
+
+class MetalMSM:
+    """BN254 G1 MSM engine (one HIP stream + device workspace)."""
+
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        _check(lib().zkmetal_msm_engine_create(ctypes.byref(self.handle)), "engine_create")
+
+    def close(self):
+        if self.handle:
+            lib().zkmetal_msm_engine_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def msm(self, points, scalars, n=None):
+        """points: n*64 B affine Montgomery; scalars: n*32 B standard form.
+        Returns the 96-byte Jacobian (X|Y|Z), Montgomery, normalised."""
+        n = _count(points, 64) if n is None else n
+        if n == 0 or _count(scalars, 32) != n or _count(points, 64) != n:
+            raise MSMError(ERR_INVALID_INPUT, "points/scalars")
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = (ctypes.c_uint8 * 96)()
+        base = ctypes.addressof(out)
+        _check(lib().zkmetal_bn254_msm(self.handle, pp, sp, n, base, base + 32, base + 64), "msm")
+        return bytes(out)
+
+    def msm_small(self, points, scalars, width):
+        n = _count(points, 64)
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = (ctypes.c_uint8 * 96)()
+        base = ctypes.addressof(out)
+        f = {1: lib().zkmetal_bn254_msm_u8, 2: lib().zkmetal_bn254_msm_u16, 4: lib().zkmetal_bn254_msm_u32}[width]
+        _check(f(self.handle, pp, sp, n, base, base + 32, base + 64), "msm_small")
+        return bytes(out)
+
+    def msm_batch(self, points, scalars, counts):
+        """zkmetal_bn254_msm_batch: concatenated inputs -> list of 96-byte results."""
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        _check(lib().zkmetal_bn254_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
+                                             len(counts), ctypes.c_void_p(out.ctypes.data)), "msm_batch")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    # ---- resident base table ----
+    def srs_upload(self, points):
+        n = _count(points, 64)
+        pp, k = _buf(points)
+        _check(lib().zkmsm_srs_upload(self.handle, pp, n), "srs_upload")
+
+    def srs_generate(self, seed, n):
+        _check(lib().zkmsm_srs_generate(self.handle, seed, n), "srs_generate")
+
+    def srs_download(self, n):
+        out = np.zeros(n * 64, dtype=np.uint8)
+        _check(lib().zkmsm_srs_download(self.handle, ctypes.c_void_p(out.ctypes.data), n), "srs_download")
+        return out
+
+    def msm_batch_resident(self, scalars, counts):
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        if isinstance(scalars, DeviceBuffer):
+            rc = lib().zkmsm_msm_batch_resident_device(self.handle, scalars.ptr, ctypes.c_void_p(counts.ctypes.data),
+                                                       len(counts), ctypes.c_void_p(out.ctypes.data))
+        else:
+            sp, k = _buf(scalars)
+            rc = lib().zkmsm_msm_batch_resident(self.handle, sp, ctypes.c_void_p(counts.ctypes.data),
+                                                len(counts), ctypes.c_void_p(out.ctypes.data))
+        _check(rc, "msm_batch_resident")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def set_window_bits(self, c):
+        _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")
+
+    def last_timing(self):
+        v = (ctypes.c_double * 14)()
+        k = lib().zkmsm_last_timing(self.handle, v, 14)
+        return {TIMING_KEYS[i]: v[i] for i in range(k)}
+
+    def probe_rates(self):
+        """Measured {mad64_per_s, fips_step_per_s, fpmul_per_s} at full occupancy."""
+        v = (ctypes.c_double * 3)()
+        _check(lib().zkmsm_probe_rates(self.handle, v), "probe_rates")
+        return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2]}
+
+    # ---- primitive probes (GPU unit tests) ----
+    def test_fp(self, op, a, b):
+        n = _count(a, 32)
+        ap, k1 = _buf(a)
+        bp, k2 = _buf(b)
+        out = np.zeros(n * 32, dtype=np.uint8)
+        _check(lib().zkmsm_test_fp(self.handle, op, ap, bp, ctypes.c_void_p(out.ctypes.data), n), "test_fp")
+        return out
+
+    def test_xyzz(self, op, p, q):
+        n = _count(p, 128)
+        pp, k1 = _buf(p)
+        qp, k2 = _buf(q)
+        out = np.zeros(n * 128, dtype=np.uint8)
+        _check(lib().zkmsm_test_xyzz(self.handle, op, pp, qp, ctypes.c_void_p(out.ctypes.data), n), "test_xyzz")
+        return out
+
+    def test_glv(self, scalars):
+        n = _count(scalars, 32)
+        sp, k = _buf(scalars)
+        out = np.zeros(n * 48, dtype=np.uint8)
+        _check(lib().zkmsm_test_glv(self.handle, sp, ctypes.c_void_p(out.ctypes.data), n), "test_glv")
+        return out
+
+
+def _count(x, size):
+    nb = x.nbytes if isinstance(x, np.ndarray) else len(x)
+    return nb // size

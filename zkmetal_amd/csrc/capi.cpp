@@ -1,0 +1,292 @@
+// extern "C" boundary of libzkmetal_amd.so: the zkMetal MSM ABI
+// (include/zkmetal.h) and this library's extensions (include/zkmsm.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/zkmsm.h"
+#include "engine.hpp"
+#include "gen.hpp"
+
+using zk::Engine;
+
+namespace {
+
+std::mutex g_auto_mu;
+Engine* g_auto = nullptr;
+std::mutex g_dir_mu;
+std::string g_shader_dir;
+
+// lazily created process-wide engine (zkMetal_ffi.swift:23-52)
+int32_t auto_engine(Engine** e) {
+    std::lock_guard<std::mutex> lk(g_auto_mu);
+    if (!g_auto) {
+        int32_t rc = Engine::create(&g_auto);
+        if (rc != zk::kOk) return rc;
+    }
+    *e = g_auto;
+    return zk::kOk;
+}
+
+inline Engine* as_engine(ZkMetalMSMEngine h) { return static_cast<Engine*>(h); }
+
+int32_t split_out(const uint8_t* r96, uint8_t* x, uint8_t* y, uint8_t* z) {
+    memcpy(x, r96, 32);
+    memcpy(y, r96 + 32, 32);
+    memcpy(z, r96 + 64, 32);
+    return zk::kOk;
+}
+
+int32_t msm_impl(Engine* e, const uint8_t* points, const uint8_t* scalars, uint32_t n,
+                 uint8_t* x, uint8_t* y, uint8_t* z) {
+    if (!e) return zk::kInvalidInput;
+    if (n == 0 || !points || !scalars || !x || !y || !z) return zk::kInvalidInput;
+    uint8_t r[96];
+    int32_t rc;
+    {
+        std::lock_guard<std::mutex> lk(e->mutex());
+        rc = e->msm_host(points, scalars, n, r);
+    }
+    if (rc != zk::kOk) return rc;
+    return split_out(r, x, y, z);
+}
+
+// zero-extend 1/2/4-byte little-endian scalars (zkMetal_ffi.swift:1176-1196)
+int32_t msm_small_impl(Engine* e, const uint8_t* points, const uint8_t* scalars, uint32_t n,
+                       int width, uint8_t* x, uint8_t* y, uint8_t* z) {
+    if (n == 0 || !scalars) return zk::kInvalidInput;
+    std::vector<uint8_t> full((size_t)n * 32, 0);
+    for (uint32_t i = 0; i < n; i++) memcpy(&full[(size_t)i * 32], scalars + (size_t)i * width, width);
+    return msm_impl(e, points, full.data(), n, x, y, z);
+}
+
+int32_t batch_impl(Engine* e, const uint8_t* pts, const uint8_t* sc, const uint32_t* counts,
+                   uint32_t n_msms, uint8_t* results) {
+    if (!e) return zk::kInvalidInput;
+    if (n_msms == 0) return zk::kInvalidInput;  // zkMetal_ffi.swift:1250
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_batch_host(pts, sc, counts, n_msms, results);
+}
+
+}  // namespace
+
+extern "C" {
+
+ZkMetalStatus zkmetal_msm_engine_create(ZkMetalMSMEngine* out) {
+    if (!out) return ZKMETAL_ERR_INVALID_INPUT;
+    Engine* e = nullptr;
+    int32_t rc = Engine::create(&e);
+    *out = e;
+    return rc;
+}
+
+void zkmetal_msm_engine_destroy(ZkMetalMSMEngine engine) { delete as_engine(engine); }
+
+ZkMetalStatus zkmetal_bn254_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                const uint8_t* scalars, uint32_t n_points, uint8_t* result_x,
+                                uint8_t* result_y, uint8_t* result_z) {
+    return msm_impl(as_engine(engine), points, scalars, n_points, result_x, result_y, result_z);
+}
+
+ZkMetalStatus zkmetal_bn254_msm_auto(const uint8_t* points, const uint8_t* scalars,
+                                     uint32_t n_points, uint8_t* result_x, uint8_t* result_y,
+                                     uint8_t* result_z) {
+    Engine* e = nullptr;
+    int32_t rc = auto_engine(&e);
+    if (rc != zk::kOk) return rc;
+    return msm_impl(e, points, scalars, n_points, result_x, result_y, result_z);
+}
+
+#define ZK_SMALL(name, width)                                                                   \
+    ZkMetalStatus name(ZkMetalMSMEngine engine, const uint8_t* points, const uint8_t* scalars, \
+                       uint32_t n, uint8_t* x, uint8_t* y, uint8_t* z) {                        \
+        return msm_small_impl(as_engine(engine), points, scalars, n, width, x, y, z);          \
+    }                                                                                           \
+    ZkMetalStatus name##_auto(const uint8_t* points, const uint8_t* scalars, uint32_t n,      \
+                              uint8_t* x, uint8_t* y, uint8_t* z) {                             \
+        Engine* e = nullptr;                                                                    \
+        int32_t rc = auto_engine(&e);                                                           \
+        if (rc != zk::kOk) return rc;                                                           \
+        return msm_small_impl(e, points, scalars, n, width, x, y, z);                          \
+    }
+ZK_SMALL(zkmetal_bn254_msm_u8, 1)
+ZK_SMALL(zkmetal_bn254_msm_u16, 2)
+ZK_SMALL(zkmetal_bn254_msm_u32, 4)
+#undef ZK_SMALL
+
+ZkMetalStatus zkmetal_bn254_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
+                                      const uint8_t* all_scalars, const uint32_t* counts,
+                                      uint32_t n_msms, uint8_t* results) {
+    return batch_impl(as_engine(engine), all_points, all_scalars, counts, n_msms, results);
+}
+
+ZkMetalStatus zkmetal_bn254_msm_batch_auto(const uint8_t* all_points, const uint8_t* all_scalars,
+                                           const uint32_t* counts, uint32_t n_msms,
+                                           uint8_t* results) {
+    Engine* e = nullptr;
+    int32_t rc = auto_engine(&e);
+    if (rc != zk::kOk) return rc;
+    return batch_impl(e, all_points, all_scalars, counts, n_msms, results);
+}
+
+void zkmetal_set_shader_dir(const char* path) {
+    std::lock_guard<std::mutex> lk(g_dir_mu);
+    g_shader_dir = path ? path : "";
+}
+
+int32_t zkmetal_gpu_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return count > 0 ? 1 : 0;
+}
+
+const char* zkmetal_version(void) { return "0.2.0-mi355x"; }
+
+// ---------------------------------------------------------------- extensions
+
+ZkMetalStatus zkmsm_srs_upload(ZkMetalMSMEngine engine, const uint8_t* points, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_upload(points, n);
+}
+
+ZkMetalStatus zkmsm_srs_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_generate(seed, n);
+}
+
+ZkMetalStatus zkmsm_srs_download(ZkMetalMSMEngine engine, uint8_t* points, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_download(points, n);
+}
+
+uint32_t zkmsm_srs_size(ZkMetalMSMEngine engine) {
+    Engine* e = as_engine(engine);
+    return e ? e->srs_size() : 0;
+}
+
+ZkMetalStatus zkmsm_msm_batch_resident(ZkMetalMSMEngine engine, const uint8_t* all_scalars,
+                                       const uint32_t* counts, uint32_t n_msms,
+                                       uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_resident(all_scalars, false, counts, n_msms, results);
+}
+
+ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
+                                              const void* device_scalars,
+                                              const uint32_t* counts, uint32_t n_msms,
+                                              uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_resident(device_scalars, true, counts, n_msms, results);
+}
+
+ZkMetalStatus zkmsm_device_alloc(uint64_t bytes, void** device_ptr) {
+    if (!device_ptr || bytes == 0) return ZKMETAL_ERR_INVALID_INPUT;
+    if (hipMalloc(device_ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *device_ptr = nullptr;
+        return ZKMETAL_ERR_ALLOC_FAILED;
+    }
+    return ZKMETAL_SUCCESS;
+}
+
+ZkMetalStatus zkmsm_device_free(void* device_ptr) {
+    if (!device_ptr) return ZKMETAL_SUCCESS;
+    return hipFree(device_ptr) == hipSuccess ? ZKMETAL_SUCCESS : ZKMETAL_ERR_GPU_ERROR;
+}
+
+ZkMetalStatus zkmsm_copy_to_device(void* dst, const void* src, uint64_t bytes) {
+    if (!dst || !src) return ZKMETAL_ERR_INVALID_INPUT;
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? ZKMETAL_SUCCESS
+                                                                           : ZKMETAL_ERR_GPU_ERROR;
+}
+
+ZkMetalStatus zkmsm_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+    if (!dst || !src) return ZKMETAL_ERR_INVALID_INPUT;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? ZKMETAL_SUCCESS
+                                                                           : ZKMETAL_ERR_GPU_ERROR;
+}
+
+ZkMetalStatus zkmsm_set_window_bits(ZkMetalMSMEngine engine, uint32_t c) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    if (c != 0 && (c < 2 || c > 16)) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    e->set_window_bits(c);
+    return ZKMETAL_SUCCESS;
+}
+
+uint32_t zkmsm_default_window_bits(uint64_t n) { return zk::default_window_bits(n); }
+
+int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values) {
+    Engine* e = as_engine(engine);
+    if (!e || !out) return 0;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    const zk::Timing& t = e->last_timing();
+    double v[14];
+    for (int s = 0; s < zk::kNumStages; s++) v[s] = t.stage_ms[s];
+    v[6] = t.device_ms;
+    v[7] = t.host_tail_ms;
+    v[8] = t.wall_ms;
+    v[9] = t.window_bits_first;
+    v[10] = t.T;
+    v[11] = t.G;
+    v[12] = (double)t.entries;
+    v[13] = (double)t.lanes;
+    int32_t k = max_values < 14 ? max_values : 14;
+    for (int32_t i = 0; i < k; i++) out[i] = v[i];
+    return k;
+}
+
+void zkmsm_gen_scalars(uint64_t seed, uint64_t n, int32_t dist, uint8_t* out) {
+    if (!out) return;
+    zkh::gen_scalars(seed, n, dist, out);
+}
+
+ZkMetalStatus zkmsm_test_fp(ZkMetalMSMEngine engine, int32_t op, const uint8_t* a,
+                            const uint8_t* b, uint8_t* out, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->test_fp(op, a, b, out, n);
+}
+
+ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,
+                              const uint8_t* q, uint8_t* out, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->test_xyzz(op, p, q, out, n);
+}
+
+ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, uint8_t* out,
+                             uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->test_glv(scalars, out, n);
+}
+
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->probe_rates(out3);
+}
+
+}  // extern "C"

@@ -1,0 +1,590 @@
+// MSM engine: planner, launch sequence, host tail.
+//
+// Reference counterpart: MetalMSM.msm (Sources/zkMetal/MSM/MSMEngine.swift:287-909)
+// and the batch loop _msm_batch_impl (Sources/zkMetal-ffi/zkMetal_ffi.swift:1241-1298).
+// Differences by design:
+//   * every n runs on the GPU (no n <= 2048 CPU route, MSMEngine.swift:295-297);
+//   * a batch of MSMs is one launch sequence over all (MSM, window) segments
+//     instead of a sequential loop;
+//   * bases can stay resident on the device (the reference re-copies them on
+//     every call, MSMEngine.swift:395-397).
+#include "engine.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <thread>
+
+#include "gen.hpp"
+#include "kernels.hpp"
+
+namespace zk {
+
+#define ZK_TRY(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return kGpuError;       \
+    } while (0)
+#define ZK_ALLOC(buf, bytes)                          \
+    do {                                              \
+        if ((buf).ensure(bytes) != hipSuccess) return kAllocFailed; \
+    } while (0)
+
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+hipError_t HostBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ------------------------------------------------------------------ planner
+
+static inline uint32_t nwin_for(uint32_t c) { return (128 + c - 1) / c; }
+
+uint32_t default_window_bits(uint64_t n) {
+    // field multiplications: W * (2n * 10 madd + 2^(c-1) * ~40 for bucket reduction)
+    uint32_t best = 16;
+    double best_cost = 1e300;
+    for (uint32_t c = 4; c <= 16; c++) {
+        double W = nwin_for(c);
+        double cost = W * (20.0 * (double)n + 40.0 * (double)(1u << (c - 1)));
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = c;
+        }
+    }
+    return best;
+}
+
+int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
+              Plan& pl) {
+    pl = Plan();
+    pl.msms.resize(nmsm);
+    pl.msm_scalar0.resize(nmsm + 1);
+    for (uint32_t j = 0; j < nmsm; j++) {
+        if (ns[j] == 0) return kInvalidInput;
+        uint32_t c = (cs && cs[j]) ? cs[j] : default_window_bits(ns[j]);
+        if (c < 2 || c > 16) return kInvalidInput;
+        MsmDesc& m = pl.msms[j];
+        m.scalar0 = pl.scalars;
+        m.table0 = table0[j];
+        m.n = ns[j];
+        m.c = c;
+        m.nwin = nwin_for(c);
+        m.seg0 = (uint32_t)pl.segs.size();
+        pl.msm_scalar0[j] = pl.scalars;
+        pl.scalars += ns[j];
+        for (uint32_t w = 0; w < m.nwin; w++) {
+            SegDesc s{};
+            s.ent0 = pl.entries;
+            s.bkt0 = pl.buckets;
+            s.len = 2 * ns[j];
+            s.nbk = 1u << (c - 1);
+            s.tile0 = pl.tiles;
+            s.ntiles = (s.len + kSortTile - 1) / kSortTile;
+            s.msm = j;
+            s.win = w;
+            pl.entries += s.len;
+            pl.buckets += s.nbk;
+            pl.tiles += s.ntiles;
+            pl.segs.push_back(s);
+        }
+    }
+    pl.msm_scalar0[nmsm] = pl.scalars;
+    // 32-bit positions / bucket ids on the device
+    if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
+    const uint32_t nseg = (uint32_t)pl.segs.size();
+    // accumulation: ~2^18 lanes in flight, at least 4 entries per lane
+    pl.T = (uint32_t)std::max<uint64_t>(4, (pl.entries + (1u << 18) - 1) >> 18);
+    // bucket reduction: ~2^16 groups
+    uint32_t G = 2;
+    while (G < 64 && (pl.buckets + G - 1) / G > (1u << 16)) G *= 2;
+    pl.G = G;
+    pl.seg_lane0.resize(nseg + 1);
+    pl.seg_grp0.resize(nseg + 1);
+    for (uint32_t s = 0; s < nseg; s++) {
+        SegDesc& sg = pl.segs[s];
+        sg.lane0 = pl.lanes;
+        sg.grp0 = pl.groups;
+        pl.seg_lane0[s] = pl.lanes;
+        pl.seg_grp0[s] = pl.groups;
+        pl.lanes += (sg.len + pl.T - 1) / pl.T;
+        pl.groups += (sg.nbk + G - 1) / G;
+        for (uint32_t t = 0; t < sg.ntiles; t++) pl.tile_seg.push_back(s);
+    }
+    pl.seg_lane0[nseg] = pl.lanes;
+    pl.seg_grp0[nseg] = pl.groups;
+    return kOk;
+}
+
+// ------------------------------------------------------------------ engine
+
+int32_t Engine::create(Engine** out) {
+    if (!out) return kInvalidInput;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return kNoGpu;
+    }
+    Engine* e = new Engine();
+    if (hipGetDevice(&e->dev_) != hipSuccess || hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return kGpuError;
+    }
+    for (auto& ev : e->ev_) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            delete e;
+            return kGpuError;
+        }
+    }
+    *out = e;
+    return kOk;
+}
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& ev : ev_)
+        if (ev) (void)hipEventDestroy(ev);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// pack the plan's descriptor arrays into one blob (16-byte aligned sections)
+struct DescLayout {
+    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, tile_seg, total;
+};
+
+static DescLayout layout_for(const Plan& pl) {
+    DescLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 15) & ~size_t(15);
+        return o;
+    };
+    L.msms = take(pl.msms.size() * sizeof(MsmDesc));
+    L.msm_scalar0 = take(pl.msm_scalar0.size() * 8);
+    L.segs = take(pl.segs.size() * sizeof(SegDesc));
+    L.seg_lane0 = take(pl.seg_lane0.size() * 8);
+    L.seg_grp0 = take(pl.seg_grp0.size() * 8);
+    L.tile_seg = take(pl.tile_seg.size() * 4);
+    L.total = off;
+    return L;
+}
+
+static constexpr uint32_t kMergeFanIn = 8;
+
+int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96) {
+    auto t_wall0 = std::chrono::steady_clock::now();
+    const uint32_t nmsm = (uint32_t)pl.msms.size();
+    const uint32_t nseg = (uint32_t)pl.segs.size();
+    const DescLayout L = layout_for(pl);
+
+    const uint64_t rec_cap = std::max<uint64_t>(2 * pl.lanes, pl.groups) + 2;
+    ZK_ALLOC(desc_, L.total);
+    ZK_ALLOC(hdesc_, L.total);
+    ZK_ALLOC(digits_, pl.entries * 4);
+    ZK_ALLOC(keysA_, pl.entries * 2);
+    ZK_ALLOC(valsA_, pl.entries * 4);
+    ZK_ALLOC(keysB_, pl.entries * 2);
+    ZK_ALLOC(valsB_, pl.entries * 4);
+    ZK_ALLOC(hist_, (uint64_t)pl.tiles * 256 * 4);
+    ZK_ALLOC(segvalid_, (uint64_t)nseg * 4);
+    ZK_ALLOC(buckets_, pl.buckets * 128);
+    ZK_ALLOC(recAk_, rec_cap * 4);
+    ZK_ALLOC(recAp_, rec_cap * 128);
+    ZK_ALLOC(recBk_, rec_cap * 4);
+    ZK_ALLOC(recBp_, rec_cap * 128);
+    ZK_ALLOC(winsum_, (uint64_t)nseg * 128);
+    ZK_ALLOC(err_, 4);
+    ZK_ALLOC(hwin_, (uint64_t)nseg * 128 + 16);
+
+    uint8_t* hb = (uint8_t*)hdesc_.p;
+    memcpy(hb + L.msms, pl.msms.data(), pl.msms.size() * sizeof(MsmDesc));
+    memcpy(hb + L.msm_scalar0, pl.msm_scalar0.data(), pl.msm_scalar0.size() * 8);
+    memcpy(hb + L.segs, pl.segs.data(), pl.segs.size() * sizeof(SegDesc));
+    memcpy(hb + L.seg_lane0, pl.seg_lane0.data(), pl.seg_lane0.size() * 8);
+    memcpy(hb + L.seg_grp0, pl.seg_grp0.data(), pl.seg_grp0.size() * 8);
+    memcpy(hb + L.tile_seg, pl.tile_seg.data(), pl.tile_seg.size() * 4);
+    uint8_t* db = (uint8_t*)desc_.p;
+    const MsmDesc* d_msms = (const MsmDesc*)(db + L.msms);
+    const uint64_t* d_msm_scalar0 = (const uint64_t*)(db + L.msm_scalar0);
+    const SegDesc* d_segs = (const SegDesc*)(db + L.segs);
+    const uint64_t* d_seg_lane0 = (const uint64_t*)(db + L.seg_lane0);
+    const uint64_t* d_seg_grp0 = (const uint64_t*)(db + L.seg_grp0);
+    const uint32_t* d_tile_seg = (const uint32_t*)(db + L.tile_seg);
+    uint32_t* d_err = (uint32_t*)err_.p;
+    hipStream_t st = stream_;
+
+    ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
+    launch_fill_zero(err_.p, 16, st);
+    ZK_TRY(hipEventRecord(ev_[0], st));
+    launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, pl.scalars, digits_.p, d_err, st);
+    ZK_TRY(hipEventRecord(ev_[1], st));
+    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, digits_.p, nullptr, 0, (uint32_t*)hist_.p,
+                     (uint32_t*)segvalid_.p, (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
+    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, keysA_.p, (const uint32_t*)valsA_.p, 1,
+                     (uint32_t*)hist_.p, (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p,
+                     (uint32_t*)valsB_.p, st);
+    ZK_TRY(hipEventRecord(ev_[2], st));
+    launch_fill_zero(buckets_.p, pl.buckets * 128, st);
+    launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
+                      (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
+                      buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
+    ZK_TRY(hipEventRecord(ev_[3], st));
+    // merge bucket partials until one lane holds every remaining run
+    uint32_t* ck = (uint32_t*)recAk_.p;
+    void* cp = recAp_.p;
+    uint32_t* ok = (uint32_t*)recBk_.p;
+    void* op = recBp_.p;
+    uint64_t nrec = 2 * pl.lanes;
+    for (;;) {
+        uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
+        launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, buckets_.p, st);
+        if (lanes <= 1) break;
+        nrec = 2 * lanes;
+        std::swap(ck, ok);
+        std::swap(cp, op);
+    }
+    ZK_TRY(hipEventRecord(ev_[4], st));
+    launch_fill_zero(winsum_.p, (uint64_t)nseg * 128, st);
+    launch_bucket_reduce(d_segs, d_seg_grp0, nseg, pl.G, pl.groups, buckets_.p,
+                         (uint32_t*)recAk_.p, recAp_.p, st);
+    ck = (uint32_t*)recAk_.p;
+    cp = recAp_.p;
+    ok = (uint32_t*)recBk_.p;
+    op = recBp_.p;
+    nrec = pl.groups;
+    for (;;) {
+        uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
+        launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, winsum_.p, st);
+        if (lanes <= 1) break;
+        nrec = 2 * lanes;
+        std::swap(ck, ok);
+        std::swap(cp, op);
+    }
+    ZK_TRY(hipEventRecord(ev_[5], st));
+    uint8_t* hw = (uint8_t*)hwin_.p;
+    ZK_TRY(hipMemcpyAsync(hw, winsum_.p, (uint64_t)nseg * 128, hipMemcpyDeviceToHost, st));
+    ZK_TRY(hipMemcpyAsync(hw + (uint64_t)nseg * 128, err_.p, 4, hipMemcpyDeviceToHost, st));
+    ZK_TRY(hipEventRecord(ev_[6], st));
+    ZK_TRY(hipStreamSynchronize(st));
+    ZK_TRY(hipGetLastError());
+    uint32_t err;
+    memcpy(&err, hw + (uint64_t)nseg * 128, 4);
+    if (err) return kGpuError;
+
+    for (int s = 0; s < kNumStages; s++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
+        timing_.stage_ms[s] = ms;
+    }
+    (void)hipEventElapsedTime(&timing_.device_ms, ev_[0], ev_[kNumStages]);
+
+    // host tail: Horner over windows, then affine normalisation
+    auto t_tail0 = std::chrono::steady_clock::now();
+    std::vector<zkh::Xyzz> res(nmsm);
+    auto tail = [&](uint32_t jb, uint32_t je) {
+        for (uint32_t j = jb; j < je; j++) {
+            const MsmDesc& m = pl.msms[j];
+            zkh::Xyzz acc = zkh::identity();
+            for (int w = (int)m.nwin - 1; w >= 0; w--) {
+                if (w != (int)m.nwin - 1)
+                    for (uint32_t b = 0; b < m.c; b++) acc = zkh::xdbl(acc);
+                zkh::Xyzz s;
+                memcpy(&s, hw + (uint64_t)(m.seg0 + w) * 128, 128);
+                acc = zkh::xadd(acc, s);
+            }
+            res[j] = acc;
+        }
+    };
+    uint32_t nt = nmsm >= 16 ? std::min<uint32_t>(8, nmsm / 8) : 1;
+    if (nt > 1) {
+        std::vector<std::thread> th;
+        uint32_t chunk = (nmsm + nt - 1) / nt;
+        for (uint32_t t = 1; t < nt; t++)
+            th.emplace_back(tail, std::min(nmsm, t * chunk), std::min(nmsm, (t + 1) * chunk));
+        tail(0, std::min(nmsm, chunk));
+        for (auto& x : th) x.join();
+    } else {
+        tail(0, nmsm);
+    }
+    std::vector<zkh::Fp> xs, ys;
+    zkh::batch_to_affine(res, xs, ys);
+    for (uint32_t j = 0; j < nmsm; j++) {
+        uint8_t* o = out96 + (uint64_t)j * 96;
+        if (zkh::is_id(res[j])) {
+            zkh::Fp one = zkh::one(), zero = zkh::zero();
+            memcpy(o, one.v, 32);
+            memcpy(o + 32, one.v, 32);
+            memcpy(o + 64, zero.v, 32);
+        } else {
+            zkh::Fp one = zkh::one();
+            memcpy(o, xs[j].v, 32);
+            memcpy(o + 32, ys[j].v, 32);
+            memcpy(o + 64, one.v, 32);
+        }
+    }
+    auto t_end = std::chrono::steady_clock::now();
+    timing_.host_tail_ms = std::chrono::duration<float, std::milli>(t_end - t_tail0).count();
+    timing_.wall_ms = std::chrono::duration<float, std::milli>(t_end - t_wall0).count();
+    timing_.nmsm = nmsm;
+    timing_.window_bits_first = nmsm ? pl.msms[0].c : 0;
+    timing_.T = pl.T;
+    timing_.G = pl.G;
+    timing_.entries = pl.entries;
+    timing_.buckets = pl.buckets;
+    timing_.lanes = pl.lanes;
+    return kOk;
+}
+
+// ------------------------------------------------------------ entry points
+
+int32_t Engine::msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t n,
+                         uint8_t* out96) {
+    uint32_t counts[1] = {n};
+    return msm_batch_host(points, scalars, counts, 1, out96);
+}
+
+int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
+                               const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
+    if (!points || !scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
+    std::vector<uint32_t> ns;
+    std::vector<uint64_t> t0;
+    std::vector<uint32_t> idx;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        if (counts[j] == 0) {
+            memset(out96 + (uint64_t)j * 96, 0, 96);  // zkMetal_ffi.swift:1260-1264
+            continue;
+        }
+        ns.push_back(counts[j]);
+        t0.push_back(2 * total);
+        idx.push_back(j);
+        total += counts[j];
+    }
+    if (ns.empty()) return kOk;
+    if (total >= (1ull << 30)) return kInvalidInput;
+    Plan pl;
+    std::vector<uint32_t> cs(ns.size(), c_override_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    if (rc != kOk) return rc;
+    ZK_ALLOC(pts_in_, total * 64);
+    ZK_ALLOC(ptable_, total * 128);
+    ZK_ALLOC(scalars_, total * 32);
+    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * 64, hipMemcpyHostToDevice, stream_));
+    ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
+    launch_build_table(pts_in_.p, ptable_.p, total, stream_);
+    std::vector<uint8_t> res(ns.size() * 96);
+    rc = run(pl, scalars_.p, ptable_.p, res.data());
+    if (rc != kOk) return rc;
+    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    return kOk;
+}
+
+int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
+    if (!points || n == 0) return kInvalidInput;
+    ZK_ALLOC(pts_in_, (uint64_t)n * 64);
+    ZK_ALLOC(srs_, (uint64_t)n * 128);
+    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
+    launch_build_table(pts_in_.p, srs_.p, n, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
+    srs_n_ = n;
+    return kOk;
+}
+
+// b * 2^(8w) * G for w < 32, b < 256, affine Montgomery (entry 0 = (0,0))
+int32_t Engine::ensure_gtab() {
+    if (gtab_.p) return kOk;
+    std::vector<zkh::Xyzz> pts(32 * 256);
+    zkh::Fp gx = zkh::mul(zkh::Fp{{1, 0, 0, 0}}, zkh::r2());
+    zkh::Fp gy = zkh::mul(zkh::Fp{{2, 0, 0, 0}}, zkh::r2());
+    zkh::Xyzz base = zkh::from_affine(gx, gy);
+    for (int w = 0; w < 32; w++) {
+        zkh::Xyzz acc = zkh::identity();
+        pts[w * 256] = acc;
+        for (int b = 1; b < 256; b++) {
+            acc = zkh::xadd(acc, base);
+            pts[w * 256 + b] = acc;
+        }
+        base = zkh::xadd(acc, base);  // 256 * base
+    }
+    std::vector<zkh::Fp> xs, ys;
+    zkh::batch_to_affine(pts, xs, ys);
+    std::vector<uint8_t> blob(32 * 256 * 64);
+    for (size_t i = 0; i < pts.size(); i++) {
+        memcpy(&blob[i * 64], xs[i].v, 32);
+        memcpy(&blob[i * 64 + 32], ys[i].v, 32);
+    }
+    DevBuf tmp;
+    if (tmp.ensure(blob.size()) != hipSuccess) return kAllocFailed;
+    ZK_TRY(hipMemcpy(tmp.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    std::swap(gtab_.p, tmp.p);
+    std::swap(gtab_.cap, tmp.cap);
+    return kOk;
+}
+
+int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
+    if (n == 0) return kInvalidInput;
+    int rc = ensure_gtab();
+    if (rc != kOk) return rc;
+    std::vector<uint8_t> ks((uint64_t)n * 32);
+    zkh::gen_scalars(seed, n, 0, ks.data());
+    ZK_ALLOC(tmp_, (uint64_t)n * 32);
+    ZK_ALLOC(srs_, (uint64_t)n * 128);
+    ZK_TRY(hipMemcpyAsync(tmp_.p, ks.data(), ks.size(), hipMemcpyHostToDevice, stream_));
+    launch_gen_bases(tmp_.p, gtab_.p, n, srs_.p, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
+    srs_n_ = n;
+    return kOk;
+}
+
+int32_t Engine::srs_download(uint8_t* points, uint32_t n) {
+    if (!points || n > srs_n_) return kInvalidInput;
+    // table entries 2i are the bases themselves
+    ZK_TRY(hipMemcpy2DAsync(points, 64, srs_.p, 128, 64, n, hipMemcpyDeviceToHost, stream_));
+    ZK_TRY(hipStreamSynchronize(stream_));
+    return kOk;
+}
+
+int32_t Engine::msm_resident(const void* scalars, bool on_device, const uint32_t* counts,
+                             uint32_t nmsm, uint8_t* out96) {
+    if (!scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
+    if (srs_n_ == 0) return kInvalidInput;
+    std::vector<uint32_t> ns, idx;
+    std::vector<uint64_t> t0, sc_off;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        if (counts[j] > srs_n_) return kInvalidInput;
+        if (counts[j] == 0) {
+            memset(out96 + (uint64_t)j * 96, 0, 96);
+            continue;
+        }
+        ns.push_back(counts[j]);
+        t0.push_back(0);  // every MSM uses a prefix of the resident table
+        idx.push_back(j);
+        total += counts[j];
+    }
+    if (ns.empty()) return kOk;
+    if (total >= (1ull << 30)) return kInvalidInput;
+    Plan pl;
+    std::vector<uint32_t> cs(ns.size(), c_override_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    if (rc != kOk) return rc;
+    const void* d_sc = scalars;
+    if (!on_device) {
+        ZK_ALLOC(scalars_, total * 32);
+        ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
+        d_sc = scalars_.p;
+    }
+    std::vector<uint8_t> res(ns.size() * 96);
+    rc = run(pl, d_sc, srs_.p, res.data());
+    if (rc != kOk) return rc;
+    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    return kOk;
+}
+
+// ------------------------------------------------------------ unit probes
+
+int32_t Engine::probe_rates(double* out3) {
+    if (!out3) return kInvalidInput;
+    hipDeviceProp_t prop;
+    ZK_TRY(hipGetDeviceProperties(&prop, dev_));
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 8;
+    const uint64_t threads = (uint64_t)blocks * 256;
+    DevBuf o;
+    ZK_ALLOC(o, threads * 8);
+    const uint32_t iters[3] = {4096, 4096, 64};
+    const double per_iter[3] = {8.0, 8.0, 4.0};
+    for (int mode = 0; mode < 3; mode++) {
+        launch_probe(mode, blocks, 16, (uint64_t*)o.p, stream_);  // warm-up
+        ZK_TRY(hipEventRecord(ev_[0], stream_));
+        launch_probe(mode, blocks, iters[mode], (uint64_t*)o.p, stream_);
+        ZK_TRY(hipEventRecord(ev_[1], stream_));
+        ZK_TRY(hipEventSynchronize(ev_[1]));
+        float ms = 0;
+        ZK_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        out3[mode] = (double)threads * iters[mode] * per_iter[mode] / (ms * 1e-3);
+    }
+    return kOk;
+}
+
+int32_t Engine::test_fp(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, uint32_t n) {
+    if (!a || !b || !out || n == 0) return kInvalidInput;
+    DevBuf da, dbb, dout;
+    ZK_ALLOC(da, (uint64_t)n * 32);
+    ZK_ALLOC(dbb, (uint64_t)n * 32);
+    ZK_ALLOC(dout, (uint64_t)n * 32);
+    ZK_TRY(hipMemcpy(da.p, a, (uint64_t)n * 32, hipMemcpyHostToDevice));
+    ZK_TRY(hipMemcpy(dbb.p, b, (uint64_t)n * 32, hipMemcpyHostToDevice));
+    launch_test_fp(op, da.p, dbb.p, dout.p, n, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipMemcpy(out, dout.p, (uint64_t)n * 32, hipMemcpyDeviceToHost));
+    return kOk;
+}
+
+int32_t Engine::test_xyzz(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n) {
+    if (!p || !q || !out || n == 0) return kInvalidInput;
+    DevBuf dp, dq, dout;
+    ZK_ALLOC(dp, (uint64_t)n * 128);
+    ZK_ALLOC(dq, (uint64_t)n * 128);
+    ZK_ALLOC(dout, (uint64_t)n * 128);
+    ZK_TRY(hipMemcpy(dp.p, p, (uint64_t)n * 128, hipMemcpyHostToDevice));
+    ZK_TRY(hipMemcpy(dq.p, q, (uint64_t)n * 128, hipMemcpyHostToDevice));
+    launch_test_xyzz(op, dp.p, dq.p, dout.p, n, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipMemcpy(out, dout.p, (uint64_t)n * 128, hipMemcpyDeviceToHost));
+    return kOk;
+}
+
+int32_t Engine::test_glv(const uint8_t* scalars, uint8_t* out, uint32_t n) {
+    if (!scalars || !out || n == 0) return kInvalidInput;
+    DevBuf ds, dout;
+    ZK_ALLOC(ds, (uint64_t)n * 32);
+    ZK_ALLOC(dout, (uint64_t)n * 48);
+    ZK_TRY(hipMemcpy(ds.p, scalars, (uint64_t)n * 32, hipMemcpyHostToDevice));
+    launch_test_glv(ds.p, dout.p, n, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipMemcpy(out, dout.p, (uint64_t)n * 48, hipMemcpyDeviceToHost));
+    return kOk;
+}
+
+}  // namespace zk

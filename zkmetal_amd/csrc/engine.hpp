@@ -1,0 +1,136 @@
+// MSM engine: device workspace, batch planner, launch sequence and the host
+// tail.  One engine owns one HIP stream on one device; calls are serialised
+// by the engine mutex (the reference engine is not re-entrant at all,
+// Sources/zkMetal/MSM/MSMEngine.swift:232-277).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host_curve.hpp"
+#include "plan.hpp"
+
+namespace zk {
+
+enum Status : int32_t {
+    kOk = 0,
+    kNoGpu = -1,
+    kInvalidInput = -2,
+    kGpuError = -3,
+    kAllocFailed = -4,
+};
+
+// grow-only device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    ~DevBuf() { release(); }
+};
+
+// grow-only pinned host buffer
+struct HostBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    ~HostBuf() { release(); }
+};
+
+enum Stage {
+    kStDecompose = 0,
+    kStSort,
+    kStAccumulate,
+    kStMerge,
+    kStReduce,
+    kStCopyOut,
+    kNumStages
+};
+
+struct Timing {
+    float stage_ms[kNumStages];  // device time per stage (HIP events on the engine stream)
+    float device_ms;             // first to last event
+    float host_tail_ms;          // window combine + normalisation on the host
+    float wall_ms;               // whole call, host clock
+    uint32_t nmsm, window_bits_first, T, G;
+    uint64_t entries, buckets, lanes;
+};
+
+struct Plan {
+    std::vector<MsmDesc> msms;
+    std::vector<uint64_t> msm_scalar0;  // nmsm + 1
+    std::vector<SegDesc> segs;
+    std::vector<uint64_t> seg_lane0;  // nseg + 1
+    std::vector<uint64_t> seg_grp0;   // nseg + 1
+    std::vector<uint32_t> tile_seg;
+    uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0;
+    uint32_t tiles = 0, T = 0, G = 0;
+};
+
+// window width for an MSM of n points (cost model; overridable)
+uint32_t default_window_bits(uint64_t n);
+
+// build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
+int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
+              const uint32_t* cs /* nullable */, Plan& pl);
+
+class Engine {
+  public:
+    static int32_t create(Engine** out);
+    ~Engine();
+
+    // host buffers in, host results out (96 B Jacobian X|Y|Z per MSM)
+    int32_t msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t n, uint8_t* out96);
+    int32_t msm_batch_host(const uint8_t* points, const uint8_t* scalars,
+                           const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
+
+    // device-resident base table (SRS)
+    int32_t srs_upload(const uint8_t* points, uint32_t n);
+    int32_t srs_generate(uint64_t seed, uint32_t n);
+    int32_t srs_download(uint8_t* points, uint32_t n);
+    uint32_t srs_size() const { return srs_n_; }
+
+    // MSMs over prefixes of the resident table; scalars on host or device
+    int32_t msm_resident(const void* scalars, bool scalars_on_device, const uint32_t* counts,
+                         uint32_t nmsm, uint8_t* out96);
+
+    void set_window_bits(uint32_t c) { c_override_ = c; }
+    const Timing& last_timing() const { return timing_; }
+    int device() const { return dev_; }
+    hipStream_t stream() const { return stream_; }
+    std::mutex& mutex() { return mu_; }
+
+    // primitive tests (host in / host out)
+    int32_t test_fp(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, uint32_t n);
+    int32_t test_xyzz(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n);
+    int32_t test_glv(const uint8_t* scalars, uint8_t* out, uint32_t n);
+    // measured rates: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate
+    // steps/s, [2] field multiplies/s, all at full occupancy
+    int32_t probe_rates(double* out3);
+
+    // run a planned batch: scalars and table already on the device
+    int32_t run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96);
+
+  private:
+    Engine() = default;
+    int32_t ensure_gtab();
+
+    int dev_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::mutex mu_;
+    uint32_t c_override_ = 0;
+    Timing timing_{};
+
+    DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, segvalid_, buckets_;
+    DevBuf recAk_, recAp_, recBk_, recBp_, winsum_, err_;
+    DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, tmp_;
+    HostBuf hdesc_, hwin_;
+    uint32_t srs_n_ = 0;
+    hipEvent_t ev_[kNumStages + 1] = {};
+};
+
+}  // namespace zk

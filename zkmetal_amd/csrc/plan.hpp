@@ -1,0 +1,46 @@
+// Work descriptors shared by the host planner and the kernels.
+//
+// A launch processes a batch of MSMs.  Each MSM j has n_j scalars, window
+// width c_j and W_j = ceil(128 / c_j) signed windows; its GLV half-scalars
+// address a base table laid out as [P_0, phi(P_0), P_1, phi(P_1), ...]
+// (64-byte affine Montgomery points), so "entry" 2i+h of an MSM is base i
+// (h = 0) or its endomorphism image (h = 1).  Every (MSM, window) pair is a
+// segment: 2 n_j digit entries that are sorted by bucket, accumulated into
+// 2^(c_j - 1) buckets and reduced to one window sum.
+#pragma once
+#include <stdint.h>
+
+namespace zk {
+
+struct MsmDesc {
+    uint64_t scalar0;  // first scalar of this MSM in the batch scalar buffer
+    uint64_t table0;   // first table entry (affine point index)
+    uint32_t n;        // number of bases / scalars
+    uint32_t c;        // window width in bits (<= 16)
+    uint32_t nwin;     // ceil(128 / c)
+    uint32_t seg0;     // first segment
+};
+
+struct SegDesc {
+    uint64_t ent0;    // first entry in the digit / sorted arrays
+    uint64_t bkt0;    // first bucket in the bucket array
+    uint64_t lane0;   // first accumulation lane
+    uint64_t grp0;    // first bucket-reduction group
+    uint32_t len;     // entries = 2 n
+    uint32_t nbk;     // buckets = 2^(c-1)
+    uint32_t tile0;   // first sort tile
+    uint32_t ntiles;  // sort tiles of this segment
+    uint32_t msm;     // owning MSM
+    uint32_t win;     // window index
+};
+
+// Merge records: key bit 31 set = no payload (padding); bits 0..30 = index of
+// the bucket (or window sum) the payload belongs to.
+#define ZK_REC_INVALID 0x80000000u
+#define ZK_REC_KEYMASK 0x7fffffffu
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+}  // namespace zk
