@@ -1,0 +1,274 @@
+#!/usr/bin/env python3
+"""bench.py -- BN254 G1 multi-scalar multiplication on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on): a
+plain BN254 G1 MSM of n = 2^22 points, scalars uniform in Fr, bases
+k_i * G for seeded uniform k_i (affine, Montgomery), bases and scalars
+resident in HBM when the timed region starts.  One step = one complete MSM
+call (GLV/endomorphism table, decomposition, sort, accumulation, reduction,
+window combine, normalisation) returning the result to the host.
+
+Prints one JSON line.  `value` is ms per MSM at n = 2^22 (lower is better);
+`roofline` is the dominant kernel (bucket accumulation) against the measured
+v_mad_u64_u32 (32x32->64 integer multiply) peak; `cpu_baseline` is the CPU
+restatement of the reference's Pippenger on this host's cores, on the same
+inputs, timed in the same run.
+
+    python bench.py [--steps K] [--warmup W] [--logn 22] [--no-cpu]
+    python bench.py --config batched|skewed   (other BASELINE workloads)
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import zkmetal_amd as zk  # noqa: E402
+
+PRODUCTS_PER_FIELD_MUL = 136  # 8x8 a*b + 8x8 m*p + 8 m_i (32-bit limbs)
+
+
+def algorithmic_field_muls(n):
+    """F(n, c*) for GLV + signed windows with XYZZ arithmetic: 10 muls per
+    mixed addition over 2n half-scalar entries per window, 2 additions of 14
+    muls per bucket in the running-sum reduction, one phi multiply per base;
+    c* minimises F (so the count does not depend on this library's choice)."""
+    best = None
+    for c in range(4, 25):
+        w = math.ceil(128 / c)
+        f = 10 * (2 * n) * w + 28 * w * 2 ** (c - 1) + n
+        best = f if best is None else min(best, f)
+    return best
+
+
+def timed_steps(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    samples = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        fn()
+        samples.append((time.perf_counter() - s) * 1e3)
+    total = (time.perf_counter() - t0) * 1e3
+    return total / steps, samples
+
+
+def cpu_baseline(bases_host, scalars_host, n, gpu_result, threads):
+    from oracle import ref  # the oracle is only the checker / CPU baseline leg
+
+    t = []
+    res = None
+    for _ in range(2):
+        s = time.perf_counter()
+        res = ref.pippenger(bases_host, scalars_host, n, 0, threads)
+        t.append((time.perf_counter() - s) * 1e3)
+    aff = ref.jacobian_to_affine_bytes(res)
+    match = aff == bytes(gpu_result[:64])
+    return {
+        "value": round(min(t), 1),
+        "unit": "ms per MSM",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"2 runs of the full n=2^{int(math.log2(n))} MSM on the same inputs (best reported); "
+                  f"C restatement of bn254_pippenger_msm, one thread per 16-bit window",
+        "matches_gpu": bool(match),
+        "host_cpu": _cpu_model(),
+        "host_nproc": os.cpu_count(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def plain(args, eng, peak):
+    n = 1 << args.logn
+    eng.srs_generate(args.seed, n)
+    bases = eng.srs_download(n)
+    scal = zk.gen_scalars(args.seed + 1, n, 0)
+    dpts = zk.DeviceBuffer(bases.nbytes)
+    dpts.upload(bases)
+    dsc = zk.DeviceBuffer(scal.nbytes)
+    dsc.upload(scal)
+    result = {}
+
+    def step():
+        result["r"] = eng.msm_batch_device(dpts, dsc, [n])[0]
+        result.setdefault("t", []).append(eng.last_timing())
+
+    ms, samples = timed_steps(step, args.steps, args.warmup)
+    timings = result["t"][args.warmup:]
+    acc_ms = statistics.mean(t["accumulate_ms"] for t in timings)
+    valid = statistics.mean(t["valid_entries"] for t in timings)
+    stage = {k: round(statistics.mean(t[k] for t in timings), 4)
+             for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms",
+                       "copyout_ms", "device_ms", "host_tail_ms")}
+    acc_products = valid * 10 * PRODUCTS_PER_FIELD_MUL
+    achieved = acc_products / (acc_ms * 1e-3)
+    msm_products = algorithmic_field_muls(n) * PRODUCTS_PER_FIELD_MUL
+    extra = {
+        "ms_p50": round(statistics.median(samples), 4),
+        "msms_per_s": round(1e3 / ms, 2),
+        "stage_ms": stage,
+        "window_bits": int(timings[-1]["window_bits"]),
+        "msm_roofline_frac": round(msm_products / (ms * 1e-3) / peak, 4),
+        "msm_algorithmic_field_muls": algorithmic_field_muls(n),
+        "measured_rates": {k: float("%.4g" % v) for k, v in args.rates.items()},
+    }
+    # the metric also names n = 2^20: same buffers, first 2^20 points
+    if args.logn >= 20 and not args.skip_2_20:
+        n20 = 1 << 20
+        ms20, s20 = timed_steps(lambda: eng.msm_batch_device(dpts, dsc, [n20]), max(5, args.steps), 2)
+        extra["ms_2_20"] = round(ms20, 4)
+        extra["ms_2_20_p50"] = round(statistics.median(s20), 4)
+        extra["msm_roofline_frac_2_20"] = round(
+            algorithmic_field_muls(n20) * PRODUCTS_PER_FIELD_MUL / (ms20 * 1e-3) / peak, 4)
+    line = {
+        "metric": f"BN254 G1 MSM ms at n=2^{args.logn}",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain BN254 G1 MSM, n=2^{args.logn}, scalars uniform in Fr (seeded), "
+                        f"bases k_i*G (seeded), affine Montgomery; bases+scalars resident in HBM",
+            "n": n,
+            "scalars": "uniform Fr",
+            "window_bits": extra["window_bits"],
+        },
+        "roofline": {
+            "bound": "valu_int32_mul",
+            "kernel": "k_accumulate",
+            "achieved": float("%.4g" % (achieved / 1e12)),
+            "peak": float("%.4g" % (peak / 1e12)),
+            "unit": "T int32 products/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": args.traffic,
+        },
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(bases.tobytes(), scal.tobytes(), n, result["r"], args.cpu_threads)
+    dpts.free()
+    dsc.free()
+    return line
+
+
+def batched(args, eng, peak):
+    """64 MSMs sharing a 2^20 resident table, sizes log-uniform in [2^12, 2^20]."""
+    rnd = np.random.default_rng(args.seed)
+    n_tab = 1 << 20
+    counts = [int(2 ** u) for u in rnd.uniform(12, 20, size=64)] if args.config == "batched" \
+        else [1 << 18] * 32
+    dist = 0 if args.config == "batched" else 1
+    eng.srs_generate(args.seed, n_tab)
+    scal = zk.gen_scalars(args.seed + 2, sum(counts), dist)
+    dsc = zk.DeviceBuffer(scal.nbytes)
+    dsc.upload(scal)
+    ms_b, _ = timed_steps(lambda: eng.msm_batch_resident(dsc, counts), args.steps, args.warmup)
+    batched_res = eng.msm_batch_resident(dsc, counts)
+    # back-to-back plain launches of the same MSMs on the same device
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    views = []
+    for j, c in enumerate(counts):
+        b = zk.DeviceBuffer(c * 32)
+        b.upload(scal[offs[j]:offs[j] + c])
+        views.append(b)
+
+    def seq():
+        return [eng.msm_batch_resident(views[j], [c])[0] for j, c in enumerate(counts)]
+
+    ms_s, _ = timed_steps(seq, max(2, args.steps // 2), 1)
+    seq_res = seq()
+    for v in views:
+        v.free()
+    dsc.free()
+    same = seq_res == batched_res
+    return {
+        "metric": f"{'batched' if args.config == 'batched' else 'skewed batched'} BN254 G1 MSMs/s",
+        "value": round(len(counts) / (ms_b * 1e-3), 2),
+        "unit": "MSMs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_b, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(counts)} MSMs over a resident 2^20 table, sizes {min(counts)}..{max(counts)}, "
+                               f"scalars {'uniform Fr' if dist == 0 else 'witness-like mix'}",
+                   "msms": len(counts), "points": int(sum(counts))},
+        "extra": {"back_to_back_ms": round(ms_s, 3),
+                  "back_to_back_msms_per_s": round(len(counts) / (ms_s * 1e-3), 2),
+                  "speedup_vs_back_to_back": round(ms_s / ms_b, 3),
+                  "results_identical": bool(same)},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--logn", type=int, default=22)
+    ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--config", default="plain", choices=["plain", "batched", "skewed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-2-20", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
+              f"(a multi-GPU MSM would shard points across GPUs and gather one point per GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    args.traffic = _traffic_from_profiles()
+    eng = zk.MetalMSM()
+    args.rates = eng.probe_rates()
+    peak = args.rates["mad64_per_s"]
+    line = plain(args, eng, peak) if args.config == "plain" else batched(args, eng, peak)
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
+def _traffic_from_profiles():
+    """HBM bytes per k_accumulate launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), recorded in
+    profiles/accumulate_n22_pmc.md as a line 'traffic_bytes_per_launch: N'."""
+    path = os.path.join(ROOT, "profiles", "accumulate_n22_pmc.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,13 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
                                               const uint32_t* counts, uint32_t n_msms,
                                               uint8_t* results);
 
+/* Plain MSMs over bases and scalars that are both already in device memory
+ * (the zkmetal_bn254_msm_batch contract without the host copies): the
+ * [P, phi(P)] table is built inside the call. */
+ZkMetalStatus zkmsm_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                     const void* device_scalars, const uint32_t* counts,
+                                     uint32_t n_msms, uint8_t* results);
+
 /* ---- device memory for callers that keep scalars in HBM ----------------- */
 ZkMetalStatus zkmsm_device_alloc(uint64_t bytes, void** device_ptr);
 ZkMetalStatus zkmsm_device_free(void* device_ptr);
@@ -61,7 +68,8 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
  * device total; out[7]: host tail ms; out[8]: wall ms of the call;
  * out[9]: window bits of the first MSM; out[10]: entries per accumulation
  * lane; out[11]: buckets per reduction group; out[12]: sorted entries;
- * out[13]: accumulation lanes.  Returns the number of values written. */
+ * out[13]: accumulation lanes; out[14]: entries with a nonzero digit (the
+ * mixed additions performed).  Returns the number of values written. */
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
 /* ---- seeded synthetic scalars (host) ------------------------------------ */
@@ -86,6 +94,9 @@ ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, ui
  * (multiply-add + carry add) per second, out3[2] = 256-bit Montgomery field
  * multiplies per second.  out3[0] is the INT32-multiply roofline peak. */
 ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3);
+/* One launch of a plain device copy of `bytes` bytes (read + write), timed
+ * with HIP events; the calibration point for HBM byte counters. */
+ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms);
 
 #ifdef __cplusplus
 }

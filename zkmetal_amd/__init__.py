@@ -64,6 +64,7 @@ def lib():
         "zkmsm_srs_size": (u32, [vp]),
         "zkmsm_msm_batch_resident": (i32, [vp, vp, vp, u32, vp]),
         "zkmsm_msm_batch_resident_device": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_device_alloc": (i32, [u64, ctypes.POINTER(vp)]),
         "zkmsm_device_free": (i32, [vp]),
         "zkmsm_copy_to_device": (i32, [vp, vp, u64]),
@@ -76,6 +77,7 @@ def lib():
         "zkmsm_test_xyzz": (i32, [vp, i32, vp, vp, vp, u32]),
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -146,7 +148,7 @@ class DeviceBuffer:
 
 TIMING_KEYS = ["decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "copyout_ms",
                "device_ms", "host_tail_ms", "wall_ms", "window_bits", "entries_per_lane",
-               "buckets_per_group", "entries", "lanes"]
+               "buckets_per_group", "entries", "lanes", "valid_entries"]
 
 
 class MetalMSM:
@@ -200,6 +202,15 @@ class MetalMSM:
                                              len(counts), ctypes.c_void_p(out.ctypes.data)), "msm_batch")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
 
+    def msm_batch_device(self, dpoints, dscalars, counts):
+        """Plain MSMs with bases and scalars in DeviceBuffers."""
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        _check(lib().zkmsm_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
+                                            ctypes.c_void_p(counts.ctypes.data), len(counts),
+                                            ctypes.c_void_p(out.ctypes.data)), "msm_batch_device")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
     # ---- resident base table ----
     def srs_upload(self, points):
         n = _count(points, 64)
@@ -231,8 +242,8 @@ class MetalMSM:
         _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")
 
     def last_timing(self):
-        v = (ctypes.c_double * 14)()
-        k = lib().zkmsm_last_timing(self.handle, v, 14)
+        v = (ctypes.c_double * 15)()
+        k = lib().zkmsm_last_timing(self.handle, v, 15)
         return {TIMING_KEYS[i]: v[i] for i in range(k)}
 
     def probe_rates(self):
@@ -240,6 +251,12 @@ class MetalMSM:
         v = (ctypes.c_double * 3)()
         _check(lib().zkmsm_probe_rates(self.handle, v), "probe_rates")
         return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2]}
+
+    def probe_copy(self, nbytes):
+        """ms of one device copy of nbytes (HBM counter calibration)."""
+        ms = ctypes.c_double()
+        _check(lib().zkmsm_probe_copy(self.handle, nbytes, ctypes.byref(ms)), "probe_copy")
+        return ms.value
 
     # ---- primitive probes (GPU unit tests) ----
     def test_fp(self, op, a, b):

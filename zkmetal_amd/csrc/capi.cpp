@@ -195,6 +195,15 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
     return e->msm_resident(device_scalars, true, counts, n_msms, results);
 }
 
+ZkMetalStatus zkmsm_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                     const void* device_scalars, const uint32_t* counts,
+                                     uint32_t n_msms, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_device(device_points, device_scalars, counts, n_msms, results);
+}
+
 ZkMetalStatus zkmsm_device_alloc(uint64_t bytes, void** device_ptr) {
     if (!device_ptr || bytes == 0) return ZKMETAL_ERR_INVALID_INPUT;
     if (hipMalloc(device_ptr, bytes) != hipSuccess) {
@@ -238,7 +247,7 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     if (!e || !out) return 0;
     std::lock_guard<std::mutex> lk(e->mutex());
     const zk::Timing& t = e->last_timing();
-    double v[14];
+    double v[15];
     for (int s = 0; s < zk::kNumStages; s++) v[s] = t.stage_ms[s];
     v[6] = t.device_ms;
     v[7] = t.host_tail_ms;
@@ -248,7 +257,8 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     v[11] = t.G;
     v[12] = (double)t.entries;
     v[13] = (double)t.lanes;
-    int32_t k = max_values < 14 ? max_values : 14;
+    v[14] = (double)t.valid_entries;
+    int32_t k = max_values < 15 ? max_values : 15;
     for (int32_t i = 0; i < k; i++) out[i] = v[i];
     return k;
 }
@@ -287,6 +297,13 @@ ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3) {
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->probe_rates(out3);
+}
+
+ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->probe_copy(bytes, ms);
 }
 
 }  // extern "C"

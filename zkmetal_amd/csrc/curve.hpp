@@ -48,7 +48,7 @@ __device__ __forceinline__ Xyzz xyzz_select(bool c, const Xyzz& a, const Xyzz& b
 }
 
 // 2*(x, y) for an affine, non-identity point (mdbl-2008-s-1)
-__device__ __noinline__ Xyzz xyzz_dbl_aff(const Fp& x, const Fp& y) {
+__device__ __forceinline__ Xyzz xyzz_dbl_aff(const Fp& x, const Fp& y) {
     Fp U = fp_dbl(y);
     Fp V = fp_sqr(U);
     Fp W = fp_mul(U, V);
@@ -64,7 +64,7 @@ __device__ __noinline__ Xyzz xyzz_dbl_aff(const Fp& x, const Fp& y) {
 }
 
 // 2*P (dbl-2008-s-1); identity stays identity (ZZ3 = V*ZZ1 = 0)
-__device__ __noinline__ Xyzz xyzz_dbl(const Xyzz& p) {
+__device__ __forceinline__ Xyzz xyzz_dbl(const Xyzz& p) {
     Fp U = fp_dbl(p.Y);
     Fp V = fp_sqr(U);
     Fp W = fp_mul(U, V);

@@ -120,6 +120,9 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
             s.ntiles = (s.len + kSortTile - 1) / kSortTile;
             s.msm = j;
             s.win = w;
+            s.chunk0 = pl.chunks;
+            s.nchunks = (256u * s.ntiles + kScanChunk - 1) / kScanChunk;
+            pl.chunks += s.nchunks;
             pl.entries += s.len;
             pl.buckets += s.nbk;
             pl.tiles += s.ntiles;
@@ -132,10 +135,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     const uint32_t nseg = (uint32_t)pl.segs.size();
     // accumulation: ~2^18 lanes in flight, at least 4 entries per lane
     pl.T = (uint32_t)std::max<uint64_t>(4, (pl.entries + (1u << 18) - 1) >> 18);
-    // bucket reduction: ~2^16 groups
-    uint32_t G = 2;
-    while (G < 64 && (pl.buckets + G - 1) / G > (1u << 16)) G *= 2;
-    pl.G = G;
+    pl.G = 0;
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
     for (uint32_t s = 0; s < nseg; s++) {
@@ -145,8 +145,11 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         pl.seg_lane0[s] = pl.lanes;
         pl.seg_grp0[s] = pl.groups;
         pl.lanes += (sg.len + pl.T - 1) / pl.T;
-        pl.groups += (sg.nbk + G - 1) / G;
+        // bucket reduction items: 2^h rows + 2^g columns (c-1 = g + h bits)
+        const uint32_t cb = 31 - __builtin_clz(sg.nbk), g = cb / 2, h = cb - g;
+        pl.groups += (1u << h) + (1u << g);
         for (uint32_t t = 0; t < sg.ntiles; t++) pl.tile_seg.push_back(s);
+        for (uint32_t t = 0; t < sg.nchunks; t++) pl.chunk_seg.push_back(s);
     }
     pl.seg_lane0[nseg] = pl.lanes;
     pl.seg_grp0[nseg] = pl.groups;
@@ -187,7 +190,7 @@ Engine::~Engine() {
 
 // pack the plan's descriptor arrays into one blob (16-byte aligned sections)
 struct DescLayout {
-    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, tile_seg, total;
+    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, tile_seg, chunk_seg, total;
 };
 
 static DescLayout layout_for(const Plan& pl) {
@@ -204,11 +207,13 @@ static DescLayout layout_for(const Plan& pl) {
     L.seg_lane0 = take(pl.seg_lane0.size() * 8);
     L.seg_grp0 = take(pl.seg_grp0.size() * 8);
     L.tile_seg = take(pl.tile_seg.size() * 4);
+    L.chunk_seg = take(pl.chunk_seg.size() * 4);
     L.total = off;
     return L;
 }
 
 static constexpr uint32_t kMergeFanIn = 8;
+static constexpr uint32_t kRunLimit = 32;
 
 int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
@@ -216,7 +221,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     const uint32_t nseg = (uint32_t)pl.segs.size();
     const DescLayout L = layout_for(pl);
 
-    const uint64_t rec_cap = std::max<uint64_t>(2 * pl.lanes, pl.groups) + 2;
+    const uint64_t rec_cap = 2 * pl.lanes + 2;
     ZK_ALLOC(desc_, L.total);
     ZK_ALLOC(hdesc_, L.total);
     ZK_ALLOC(digits_, pl.entries * 4);
@@ -225,15 +230,17 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_ALLOC(keysB_, pl.entries * 2);
     ZK_ALLOC(valsB_, pl.entries * 4);
     ZK_ALLOC(hist_, (uint64_t)pl.tiles * 256 * 4);
+    ZK_ALLOC(part_, (uint64_t)pl.chunks * 4);
     ZK_ALLOC(segvalid_, (uint64_t)nseg * 4);
     ZK_ALLOC(buckets_, pl.buckets * 128);
     ZK_ALLOC(recAk_, rec_cap * 4);
     ZK_ALLOC(recAp_, rec_cap * 128);
     ZK_ALLOC(recBk_, rec_cap * 4);
     ZK_ALLOC(recBp_, rec_cap * 128);
-    ZK_ALLOC(winsum_, (uint64_t)nseg * 128);
-    ZK_ALLOC(err_, 4);
-    ZK_ALLOC(hwin_, (uint64_t)nseg * 128 + 16);
+    ZK_ALLOC(rc_, pl.groups * 128);
+    ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * 128);
+    ZK_ALLOC(err_, 16);
+    ZK_ALLOC(hwin_, (uint64_t)nseg * (16 * 128 + 4) + 16);
 
     uint8_t* hb = (uint8_t*)hdesc_.p;
     memcpy(hb + L.msms, pl.msms.data(), pl.msms.size() * sizeof(MsmDesc));
@@ -242,6 +249,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     memcpy(hb + L.seg_lane0, pl.seg_lane0.data(), pl.seg_lane0.size() * 8);
     memcpy(hb + L.seg_grp0, pl.seg_grp0.data(), pl.seg_grp0.size() * 8);
     memcpy(hb + L.tile_seg, pl.tile_seg.data(), pl.tile_seg.size() * 4);
+    memcpy(hb + L.chunk_seg, pl.chunk_seg.data(), pl.chunk_seg.size() * 4);
     uint8_t* db = (uint8_t*)desc_.p;
     const MsmDesc* d_msms = (const MsmDesc*)(db + L.msms);
     const uint64_t* d_msm_scalar0 = (const uint64_t*)(db + L.msm_scalar0);
@@ -249,6 +257,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     const uint64_t* d_seg_lane0 = (const uint64_t*)(db + L.seg_lane0);
     const uint64_t* d_seg_grp0 = (const uint64_t*)(db + L.seg_grp0);
     const uint32_t* d_tile_seg = (const uint32_t*)(db + L.tile_seg);
+    const uint32_t* d_chunk_seg = (const uint32_t*)(db + L.chunk_seg);
     uint32_t* d_err = (uint32_t*)err_.p;
     hipStream_t st = stream_;
 
@@ -257,57 +266,51 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_TRY(hipEventRecord(ev_[0], st));
     launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, pl.scalars, digits_.p, d_err, st);
     ZK_TRY(hipEventRecord(ev_[1], st));
-    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, digits_.p, nullptr, 0, (uint32_t*)hist_.p,
-                     (uint32_t*)segvalid_.p, (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
-    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, keysA_.p, (const uint32_t*)valsA_.p, 1,
-                     (uint32_t*)hist_.p, (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p,
-                     (uint32_t*)valsB_.p, st);
-    ZK_TRY(hipEventRecord(ev_[2], st));
+    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
+                     nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
+                     (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
+    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
+                     (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
+                     (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
     launch_fill_zero(buckets_.p, pl.buckets * 128, st);
+    ZK_TRY(hipEventRecord(ev_[2], st));
     launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
                       (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
                       buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
     ZK_TRY(hipEventRecord(ev_[3], st));
-    // merge bucket partials until one lane holds every remaining run
-    uint32_t* ck = (uint32_t*)recAk_.p;
-    void* cp = recAp_.p;
-    uint32_t* ok = (uint32_t*)recBk_.p;
-    void* op = recBp_.p;
-    uint64_t nrec = 2 * pl.lanes;
-    for (;;) {
-        uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
-        launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, buckets_.p, st);
-        if (lanes <= 1) break;
-        nrec = 2 * lanes;
-        std::swap(ck, ok);
-        std::swap(cp, op);
+    // bucket partials that crossed lane boundaries: short runs in one pass,
+    // long runs (heavy buckets) through gated merge levels
+    uint32_t* d_long = d_err + 1;
+    launch_merge_runs((uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p, d_long, st);
+    {
+        uint32_t* ck = (uint32_t*)recAk_.p;
+        void* cp = recAp_.p;
+        uint32_t* ok = (uint32_t*)recBk_.p;
+        void* op = recBp_.p;
+        uint64_t nrec = 2 * pl.lanes;
+        for (;;) {
+            uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
+            launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, buckets_.p, d_long, st);
+            if (lanes <= 1) break;
+            nrec = 2 * lanes;
+            std::swap(ck, ok);
+            std::swap(cp, op);
+        }
     }
     ZK_TRY(hipEventRecord(ev_[4], st));
-    launch_fill_zero(winsum_.p, (uint64_t)nseg * 128, st);
-    launch_bucket_reduce(d_segs, d_seg_grp0, nseg, pl.G, pl.groups, buckets_.p,
-                         (uint32_t*)recAk_.p, recAp_.p, st);
-    ck = (uint32_t*)recAk_.p;
-    cp = recAp_.p;
-    ok = (uint32_t*)recBk_.p;
-    op = recBp_.p;
-    nrec = pl.groups;
-    for (;;) {
-        uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
-        launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, winsum_.p, st);
-        if (lanes <= 1) break;
-        nrec = 2 * lanes;
-        std::swap(ck, ok);
-        std::swap(cp, op);
-    }
+    launch_bucket_reduce(d_segs, d_seg_grp0, nseg, pl.groups, buckets_.p, rc_.p, winsum_.p, st);
     ZK_TRY(hipEventRecord(ev_[5], st));
     uint8_t* hw = (uint8_t*)hwin_.p;
-    ZK_TRY(hipMemcpyAsync(hw, winsum_.p, (uint64_t)nseg * 128, hipMemcpyDeviceToHost, st));
-    ZK_TRY(hipMemcpyAsync(hw + (uint64_t)nseg * 128, err_.p, 4, hipMemcpyDeviceToHost, st));
+    const uint64_t bits_bytes = (uint64_t)nseg * 16 * 128;
+    ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
+    ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 4, hipMemcpyDeviceToHost, st));
+    ZK_TRY(hipMemcpyAsync(hw + bits_bytes + 16, segvalid_.p, (uint64_t)nseg * 4,
+                          hipMemcpyDeviceToHost, st));
     ZK_TRY(hipEventRecord(ev_[6], st));
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
     uint32_t err;
-    memcpy(&err, hw + (uint64_t)nseg * 128, 4);
+    memcpy(&err, hw + bits_bytes, 4);
     if (err) return kGpuError;
 
     for (int s = 0; s < kNumStages; s++) {
@@ -323,13 +326,25 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     auto tail = [&](uint32_t jb, uint32_t je) {
         for (uint32_t j = jb; j < je; j++) {
             const MsmDesc& m = pl.msms[j];
+            const uint32_t cb = m.c - 1, g = cb / 2;
             zkh::Xyzz acc = zkh::identity();
             for (int w = (int)m.nwin - 1; w >= 0; w--) {
                 if (w != (int)m.nwin - 1)
                     for (uint32_t b = 0; b < m.c; b++) acc = zkh::xdbl(acc);
-                zkh::Xyzz s;
-                memcpy(&s, hw + (uint64_t)(m.seg0 + w) * 128, 128);
-                acc = zkh::xadd(acc, s);
+                // window sum: U + sum_{t < c-1} 2^t Q_t, Q_t = Pl_t (t < g) or Ph_{t-g}
+                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * 128;
+                zkh::Xyzz sw = zkh::identity();
+                for (int t = (int)cb - 1; t >= 0; t--) {
+                    sw = zkh::xdbl(sw);
+                    const uint32_t slot = (uint32_t)t < g ? 1 + t : 1 + g + (t - g);
+                    zkh::Xyzz q;
+                    memcpy(&q, bs + (uint64_t)slot * 128, 128);
+                    sw = zkh::xadd(sw, q);
+                }
+                zkh::Xyzz u;
+                memcpy(&u, bs, 128);
+                sw = zkh::xadd(sw, u);
+                acc = zkh::xadd(acc, sw);
             }
             res[j] = acc;
         }
@@ -371,6 +386,10 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     timing_.entries = pl.entries;
     timing_.buckets = pl.buckets;
     timing_.lanes = pl.lanes;
+    uint64_t valid = 0;
+    const uint32_t* sv = (const uint32_t*)(hw + bits_bytes + 16);
+    for (uint32_t s = 0; s < nseg; s++) valid += sv[s];
+    timing_.valid_entries = valid;
     return kOk;
 }
 
@@ -413,6 +432,37 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     launch_build_table(pts_in_.p, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
     rc = run(pl, scalars_.p, ptable_.p, res.data());
+    if (rc != kOk) return rc;
+    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    return kOk;
+}
+
+int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
+                           uint32_t nmsm, uint8_t* out96) {
+    if (!d_points || !d_scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
+    std::vector<uint32_t> ns, idx;
+    std::vector<uint64_t> t0;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        if (counts[j] == 0) {
+            memset(out96 + (uint64_t)j * 96, 0, 96);
+            continue;
+        }
+        ns.push_back(counts[j]);
+        t0.push_back(2 * total);
+        idx.push_back(j);
+        total += counts[j];
+    }
+    if (ns.empty()) return kOk;
+    if (total >= (1ull << 30)) return kInvalidInput;
+    Plan pl;
+    std::vector<uint32_t> cs(ns.size(), c_override_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    if (rc != kOk) return rc;
+    ZK_ALLOC(ptable_, total * 128);
+    launch_build_table(d_points, ptable_.p, total, stream_);
+    std::vector<uint8_t> res(ns.size() * 96);
+    rc = run(pl, d_scalars, ptable_.p, res.data());
     if (rc != kOk) return rc;
     for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
     return kOk;
@@ -523,6 +573,23 @@ int32_t Engine::msm_resident(const void* scalars, bool on_device, const uint32_t
 }
 
 // ------------------------------------------------------------ unit probes
+
+int32_t Engine::probe_copy(uint64_t bytes, double* ms_out) {
+    if (bytes < 16 || !ms_out) return kInvalidInput;
+    DevBuf a, b;
+    ZK_ALLOC(a, bytes);
+    ZK_ALLOC(b, bytes);
+    launch_fill_zero(a.p, bytes, stream_);
+    launch_copy(a.p, b.p, bytes, stream_);  // warm-up
+    ZK_TRY(hipEventRecord(ev_[0], stream_));
+    launch_copy(a.p, b.p, bytes, stream_);
+    ZK_TRY(hipEventRecord(ev_[1], stream_));
+    ZK_TRY(hipEventSynchronize(ev_[1]));
+    float ms = 0;
+    ZK_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    *ms_out = ms;
+    return kOk;
+}
 
 int32_t Engine::probe_rates(double* out3) {
     if (!out3) return kInvalidInput;

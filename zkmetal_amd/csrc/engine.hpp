@@ -58,6 +58,7 @@ struct Timing {
     float wall_ms;               // whole call, host clock
     uint32_t nmsm, window_bits_first, T, G;
     uint64_t entries, buckets, lanes;
+    uint64_t valid_entries;  // sorted entries with a nonzero digit (mixed additions done)
 };
 
 struct Plan {
@@ -65,10 +66,11 @@ struct Plan {
     std::vector<uint64_t> msm_scalar0;  // nmsm + 1
     std::vector<SegDesc> segs;
     std::vector<uint64_t> seg_lane0;  // nseg + 1
-    std::vector<uint64_t> seg_grp0;   // nseg + 1
+    std::vector<uint64_t> seg_grp0;   // nseg + 1: first bucket-reduction row/column item
     std::vector<uint32_t> tile_seg;
+    std::vector<uint32_t> chunk_seg;
     uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0;
-    uint32_t tiles = 0, T = 0, G = 0;
+    uint32_t tiles = 0, chunks = 0, T = 0, G = 0;
 };
 
 // window width for an MSM of n points (cost model; overridable)
@@ -87,6 +89,10 @@ class Engine {
     int32_t msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t n, uint8_t* out96);
     int32_t msm_batch_host(const uint8_t* points, const uint8_t* scalars,
                            const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
+
+    // bases and scalars already in device memory (plain MSMs, phi table built per call)
+    int32_t msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
+                       uint32_t nmsm, uint8_t* out96);
 
     // device-resident base table (SRS)
     int32_t srs_upload(const uint8_t* points, uint32_t n);
@@ -111,6 +117,8 @@ class Engine {
     // measured rates: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate
     // steps/s, [2] field multiplies/s, all at full occupancy
     int32_t probe_rates(double* out3);
+    // one launch of a plain copy kernel over `bytes` (HBM counter calibration)
+    int32_t probe_copy(uint64_t bytes, double* ms_out);
 
     // run a planned batch: scalars and table already on the device
     int32_t run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96);
@@ -125,8 +133,8 @@ class Engine {
     uint32_t c_override_ = 0;
     Timing timing_{};
 
-    DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, segvalid_, buckets_;
-    DevBuf recAk_, recAp_, recBk_, recBp_, winsum_, err_;
+    DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;
+    DevBuf recAk_, recAp_, recBk_, recBp_, rc_, winsum_, err_;
     DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;

@@ -4,18 +4,22 @@
 //   k_build_table     bases -> [P, phi(P)] table              (glv_kernels.metal:260-297)
 //   k_decompose       scalars -> GLV halves -> signed digits  (glv_kernels.metal:172-256,
 //                                                              msm_kernels.metal:399-439)
-//   k_sort_hist/scan/scatter x2   LSD radix sort of bucket keys per segment
-//                                 (replaces MSMEngine.swift:595-661, CPU counting sort)
+//   k_sort_hist / k_scan_* / k_sort_scatter, two passes
+//                     stable LSD radix sort of bucket keys per segment
+//                     (replaces the CPU counting sort, MSMEngine.swift:595-661)
 //   k_accumulate      load-balanced XYZZ mixed additions into buckets
-//                                 (replaces msm_kernels.metal:16-62)
-//   k_merge           segmented reduction of bucket partials across lanes
-//   k_bucket_reduce   per-group weighted bucket sums      (replaces msm_kernels.metal:152-248)
-//   k_merge           -> one XYZZ window sum per segment
-// The window combine (Horner over windows) and affine normalisation are a
-// ~1000-multiply serial tail and run on the host (MSMEngine.swift:899-904
-// does the same).
+//                     (replaces msm_kernels.metal:16-62)
+//   k_merge_runs / k_merge   bucket partials that crossed lane boundaries
+//   k_br_rowcol / k_br_bits  bucket reduction: window sum = U + sum_b 2^b Q_b,
+//                     Q_b bit-plane sums of the bucket index (replaces the
+//                     running sums of msm_kernels.metal:152-248)
+// The weighting by 2^b, the window combine (Horner over windows) and the
+// affine normalisation are a short serial tail that runs on the host
+// (MSMEngine.swift:899-904 does its Horner there too).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "curve.hpp"
 #include "glv.hpp"
@@ -37,9 +41,7 @@ __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n,
     return lo;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    return (1ull << __lane_id()) - 1ull;
-}
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
 // mask of active lanes whose 8-bit digit equals this lane's
 __device__ __forceinline__ uint64_t match_digit8(uint32_t digit, bool active) {
@@ -51,6 +53,54 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t digit, bool active) {
         peers &= bit ? m : ~m;
     }
     return active ? peers : 0ull;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+// 256-thread block: exclusive prefix of v, block total in *total
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds4,
+                                                         uint32_t* total) {
+    const uint32_t inc = wave_inclusive_scan(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 63u) lds4[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        uint32_t x = lds4[i];
+        off += (i < w) ? x : 0u;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// Sum of the 64 lanes' points of each wave of a 256-thread block; lane 0 of
+// every wave returns its wave's total.  buf: 4 x 64 XYZZ slots in LDS.  Every
+// thread of the block must call this (it contains block barriers).
+__device__ __forceinline__ Xyzz wave_sum_xyzz(Xyzz v, Xyzz* buf) {
+    const uint32_t lane = __lane_id();
+    Xyzz* mine = buf + (threadIdx.x & ~63u);
+#pragma unroll 1
+    for (uint32_t s = 32; s >= 1; s >>= 1) {
+        if (lane >= s && lane < 2 * s) mine[lane] = v;
+        __syncthreads();
+        if (lane < s) {
+            Xyzz o = mine[lane + s];
+            xyzz_add(v, o);
+        }
+        __syncthreads();
+    }
+    return v;
 }
 
 // -------------------------------------------------------- base table / phi
@@ -105,9 +155,13 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
 
 // ------------------------------------------------------ radix sort passes
 // Two stable LSD passes of 8 bits over the 16-bit bucket key, independently
-// per segment.  Tiles of 4096 entries; per tile 256-bin histograms are
-// written bin-major per segment (hist[tile0*256 + bin*ntiles + t]) so one
-// exclusive scan per segment yields every (bin, tile) destination.
+// per segment.  A tile is 4096 consecutive entries of one segment; wave w of
+// the 256-thread block owns tile positions [1024w, 1024w + 1024) and walks
+// them 64 at a time, so the stable order is (wave, round, lane).  Per-wave
+// digit counters live in LDS and are only touched by their own wave (ranks
+// from ballot matches, no atomics, no block barrier per round).  Tile
+// histograms are stored bin-major per segment, hist[tile0*256 + bin*ntiles +
+// t], so one exclusive scan per segment yields every (bin, tile) destination.
 
 __device__ __forceinline__ uint32_t sort_key(const void* in, int pass, uint64_t pos) {
     if (pass == 0) return reinterpret_cast<const uint32_t*>(in)[pos] & 0xffffu;
@@ -118,119 +172,200 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SegDesc* __res
                                                             const uint32_t* __restrict__ tile_seg,
                                                             const void* __restrict__ in, int pass,
                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[4][256];
     const uint32_t t = blockIdx.x;
     const SegDesc sg = segs[tile_seg[t]];
     const uint32_t tin = t - sg.tile0;
     const uint32_t start = tin * kSortTile;
     const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
     const uint32_t shift = pass ? 8u : 0u;
-    cnt[threadIdx.x] = 0;
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+#pragma unroll
+    for (int i = 0; i < 4; i++) cnt[i][threadIdx.x] = 0;
     __syncthreads();
     for (int r = 0; r < kSortItems; r++) {
-        uint32_t idx = r * kSortThreads + threadIdx.x;
-        bool act = idx < len;
-        uint32_t digit = act ? (sort_key(in, pass, sg.ent0 + start + idx) >> shift) & 0xffu : 0u;
-        uint64_t peers = match_digit8(digit, act);
-        if (act && (peers & lanemask_lt()) == 0) atomicAdd(&cnt[digit], (uint32_t)__popcll(peers));
+        const uint32_t pos = w * 1024u + r * 64u + lane;
+        const bool act = pos < len;
+        const uint32_t digit = act ? (sort_key(in, pass, sg.ent0 + start + pos) >> shift) & 0xffu : 0u;
+        const uint64_t peers = match_digit8(digit, act);
+        if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] += (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    hist[(uint64_t)sg.tile0 * 256 + (uint64_t)threadIdx.x * sg.ntiles + tin] = cnt[threadIdx.x];
+    const uint32_t d = threadIdx.x;
+    hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin] =
+        cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
 }
 
-// One block per segment: exclusive scan of its 256*ntiles counts, turned into
-// global destinations (+ ent0).  Pass 1 also records the number of entries
-// with a real bucket (everything before bin 0xFF, which only the "no bucket"
-// key 0xFFFF reaches).
-__global__ __launch_bounds__(256) void k_sort_scan(const SegDesc* __restrict__ segs,
-                                                   uint32_t* __restrict__ hist, int pass,
-                                                   uint32_t* __restrict__ seg_valid) {
-    __shared__ uint32_t part[256];
-    const SegDesc sg = segs[blockIdx.x];
-    uint32_t* h = hist + (uint64_t)sg.tile0 * 256;
+// Exclusive scan of each segment's 256*ntiles counts into global destinations
+// (+ ent0): reduce-then-scan over chunks of kScanChunk counts -- up (chunk
+// sums), spine (per segment, over chunk sums), down (rescan + offsets).  Pass 1
+// also records the number of entries with a real bucket: the exclusive
+// prefix of bin 0xFF, which only the "no bucket" key 0xFFFF reaches.
+
+__device__ __forceinline__ void load16(const uint32_t* p, uint32_t v[16]) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint4 x = q[i];
+        v[4 * i] = x.x;
+        v[4 * i + 1] = x.y;
+        v[4 * i + 2] = x.z;
+        v[4 * i + 3] = x.w;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_up(const SegDesc* __restrict__ segs,
+                                                 const uint32_t* __restrict__ chunk_seg,
+                                                 const uint32_t* __restrict__ hist,
+                                                 uint32_t* __restrict__ part) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t c = blockIdx.x;
+    const SegDesc sg = segs[chunk_seg[c]];
     const uint32_t M = 256u * sg.ntiles;
-    const uint32_t chunk = (M + 255u) / 256u;
-    const uint32_t b = min(M, threadIdx.x * chunk), e = min(M, b + chunk);
+    const uint32_t i0 = (c - sg.chunk0) * kScanChunk + 16u * threadIdx.x;
     uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++) s += h[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < 256; i++) {
-            uint32_t v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
+    if (i0 < M) {
+        uint32_t v[16];
+        load16(hist + (uint64_t)sg.tile0 * 256 + i0, v);
+#pragma unroll
+        for (int i = 0; i < 16; i++) s += v[i];
     }
-    __syncthreads();
-    uint32_t acc = part[threadIdx.x];
-    const uint32_t none_bin = 0xffu * sg.ntiles;
-    for (uint32_t i = b; i < e; i++) {
-        uint32_t v = h[i];
-        if (pass == 1 && i == none_bin) seg_valid[blockIdx.x] = acc;
-        h[i] = (uint32_t)sg.ent0 + acc;
-        acc += v;
+    uint32_t tot;
+    (void)block_exclusive_scan(s, lds4, &tot);
+    if (threadIdx.x == 0) part[c] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_spine(const SegDesc* __restrict__ segs,
+                                                    uint32_t* __restrict__ part) {
+    __shared__ uint32_t lds4[4];
+    const SegDesc sg = segs[blockIdx.x];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < sg.nchunks; b += 256) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < sg.nchunks ? part[sg.chunk0 + i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, lds4, &tot);
+        if (i < sg.nchunks) part[sg.chunk0 + i] = carry + ex;
+        carry += tot;
     }
 }
 
+__global__ __launch_bounds__(256) void k_scan_down(const SegDesc* __restrict__ segs,
+                                                   const uint32_t* __restrict__ chunk_seg,
+                                                   uint32_t* __restrict__ hist,
+                                                   const uint32_t* __restrict__ part, int pass,
+                                                   uint32_t* __restrict__ seg_valid) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t c = blockIdx.x;
+    const uint32_t s = chunk_seg[c];
+    const SegDesc sg = segs[s];
+    const uint32_t M = 256u * sg.ntiles;
+    const uint32_t i0 = (c - sg.chunk0) * kScanChunk + 16u * threadIdx.x;
+    uint32_t v[16];
+    uint32_t sum = 0;
+    uint32_t* h = hist + (uint64_t)sg.tile0 * 256 + i0;
+    if (i0 < M) {
+        load16(h, v);
+#pragma unroll
+        for (int i = 0; i < 16; i++) sum += v[i];
+    }
+    uint32_t tot;
+    uint32_t acc = part[c] + block_exclusive_scan(sum, lds4, &tot);
+    if (i0 < M) {
+        const uint32_t none_bin = 0xffu * sg.ntiles;
+        uint32_t o[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if (pass == 1 && i0 + i == none_bin) seg_valid[s] = acc;
+            o[i] = (uint32_t)sg.ent0 + acc;
+            acc += v[i];
+        }
+        uint4* q = reinterpret_cast<uint4*>(h);
+#pragma unroll
+        for (int i = 0; i < 4; i++) q[i] = make_uint4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
+    }
+}
+
+// Scatter: rank every entry of the tile (wave-private counters, as in
+// k_sort_hist), sort the tile in LDS, then write each digit's run to its
+// global destination with consecutive threads writing consecutive entries.
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const SegDesc* __restrict__ segs, const uint32_t* __restrict__ tile_seg,
     const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, int pass,
     const uint32_t* __restrict__ hist, uint16_t* __restrict__ out_keys,
     uint32_t* __restrict__ out_vals) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t dstart[256];
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint16_t lkey[kSortTile];
+    __shared__ uint32_t lval[kSortTile];
     const uint32_t t = blockIdx.x;
     const SegDesc sg = segs[tile_seg[t]];
     const uint32_t tin = t - sg.tile0;
     const uint32_t start = tin * kSortTile;
     const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
     const uint32_t shift = pass ? 8u : 0u;
-    const uint32_t wave = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[(uint64_t)sg.tile0 * 256 + (uint64_t)threadIdx.x * sg.ntiles + tin];
-    for (int r = 0; r < kSortItems; r++) {
-        const uint32_t idx = r * kSortThreads + threadIdx.x;
-        const bool act = idx < len;
-        const uint64_t pos = sg.ent0 + start + idx;
-        uint32_t key = 0, val = 0;
-        if (act) {
-            if (pass == 0) {
-                uint32_t wd = reinterpret_cast<const uint32_t*>(in_keys)[pos];
-                key = wd & 0xffffu;
-                val = (start + idx) | (wd & 0x80000000u);
-            } else {
-                key = reinterpret_cast<const uint16_t*>(in_keys)[pos];
-                val = in_vals[pos];
-            }
-        }
-        const uint32_t digit = (key >> shift) & 0xffu;
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
 #pragma unroll
-        for (int w = 0; w < 4; w++) wc[w][threadIdx.x] = 0;
-        __syncthreads();
-        const uint64_t peers = match_digit8(digit, act);
-        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
-        if (act && rank == 0) wc[wave][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            const uint32_t d = threadIdx.x;
-            uint32_t base = run[d];
-            uint32_t c0 = wc[0][d], c1 = wc[1][d], c2 = wc[2][d], c3 = wc[3][d];
-            wc[0][d] = base;
-            wc[1][d] = base + c0;
-            wc[2][d] = base + c0 + c1;
-            wc[3][d] = base + c0 + c1 + c2;
-            run[d] = base + c0 + c1 + c2 + c3;
-        }
-        __syncthreads();
+    for (int i = 0; i < 4; i++) cnt[i][threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t pos = w * 1024u + r * 64u + lane;
+        const bool act = pos < len;
+        key[r] = ZK_KEY_NONE;
+        val[r] = 0;
         if (act) {
-            const uint32_t dst = wc[wave][digit] + rank;
-            if (pass == 0 || key != ZK_KEY_NONE) {
-                out_keys[dst] = (uint16_t)key;
-                out_vals[dst] = val;
+            const uint64_t gp = sg.ent0 + start + pos;
+            if (pass == 0) {
+                const uint32_t wd = reinterpret_cast<const uint32_t*>(in_keys)[gp];
+                key[r] = wd & 0xffffu;
+                val[r] = (start + pos) | (wd & 0x80000000u);
+            } else {
+                key[r] = reinterpret_cast<const uint16_t*>(in_keys)[gp];
+                val[r] = in_vals[gp];
             }
         }
-        __syncthreads();
+        const uint32_t digit = (key[r] >> shift) & 0xffu;
+        const uint64_t peers = match_digit8(digit, act);
+        const uint32_t base = cnt[w][digit];
+        rank[r] = base + (uint32_t)__popcll(peers & lanemask_lt());
+        if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] = base + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        const uint32_t d = threadIdx.x;
+        const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(c0 + c1 + c2 + c3, lds4, &tot);
+        dstart[d] = ex;
+        cnt[0][d] = ex;
+        cnt[1][d] = ex + c0;
+        cnt[2][d] = ex + c0 + c1;
+        cnt[3][d] = ex + c0 + c1 + c2;
+        goff[d] = hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t pos = w * 1024u + r * 64u + lane;
+        if (pos < len) {
+            const uint32_t j = cnt[w][(key[r] >> shift) & 0xffu] + rank[r];
+            lkey[j] = (uint16_t)key[r];
+            lval[j] = val[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < len; j += kSortThreads) {
+        const uint32_t k = lkey[j];
+        const uint32_t d = (k >> shift) & 0xffu;
+        const uint32_t g = goff[d] + (j - dstart[d]);
+        if (pass == 0 || k != ZK_KEY_NONE) {
+            out_keys[g] = (uint16_t)k;
+            out_vals[g] = lval[j];
+        }
     }
 }
 
@@ -313,17 +448,55 @@ __global__ __launch_bounds__(256) void k_accumulate(
     }
 }
 
-// ------------------------------------------------- segmented merge level
-// Records [L*T, (L+1)*T) per lane; runs of equal key are summed (padding
-// records skipped).  A run closed on both sides with at least one payload is
-// final and written to dest[key]; open runs become the lane's output records.
+// ------------------------------------------------------ partial merging
+// Records with equal key are contiguous (a bucket's partials come from
+// consecutive lanes).  k_merge_runs: the lane at the first record of a run of
+// at most `limit` records sums it, writes dest[key] and marks the records
+// consumed (padding); longer runs raise *long_runs and are left to k_merge,
+// which only does work when that flag is set.
 
+__global__ __launch_bounds__(256) void k_merge_runs(uint32_t* __restrict__ keys,
+                                                    const uint4* __restrict__ pts, uint64_t nrec,
+                                                    uint32_t limit, uint4* __restrict__ dest,
+                                                    uint32_t* __restrict__ long_runs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec) return;
+    const uint32_t k = keys[i] & ZK_REC_KEYMASK;
+    if (k == ZK_REC_KEYMASK) return;  // padding of empty lanes
+    if (i > 0 && (keys[i - 1] & ZK_REC_KEYMASK) == k) return;
+    uint64_t j = i;
+    while (j < nrec && j - i < limit && (keys[j] & ZK_REC_KEYMASK) == k) j++;
+    if (j < nrec && (keys[j] & ZK_REC_KEYMASK) == k) {
+        atomicOr(long_runs, 1u);
+        return;
+    }
+    bool has = false;
+    Xyzz acc = xyzz_identity();
+    for (uint64_t p = i; p < j; p++) {
+        const uint32_t kr = keys[p];
+        if (!(kr & ZK_REC_INVALID)) {
+            Xyzz q = load_xyzz(pts, p);
+            if (has) xyzz_add(acc, q);
+            else acc = q;
+            has = true;
+            keys[p] = kr | ZK_REC_INVALID;
+        }
+    }
+    if (has) store_xyzz(dest, k, acc);
+}
+
+// One level of the general segmented merge: records [L*T, (L+1)*T) per lane;
+// runs of equal key are summed (padding skipped).  A run closed on both sides
+// with at least one payload is final and written to dest[key]; open runs
+// become the lane's two output records.  Returns at once unless *gate != 0.
 __global__ __launch_bounds__(256) void k_merge(const uint32_t* __restrict__ in_keys,
                                                const uint4* __restrict__ in_pts, uint64_t nrec,
                                                uint32_t T, uint64_t lanes,
                                                uint32_t* __restrict__ out_keys,
                                                uint4* __restrict__ out_pts,
-                                               uint4* __restrict__ dest) {
+                                               uint4* __restrict__ dest,
+                                               const uint32_t* __restrict__ gate) {
+    if (*gate == 0) return;
     const uint64_t L = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= lanes) return;
     const uint64_t b = L * T;
@@ -372,46 +545,89 @@ __global__ __launch_bounds__(256) void k_merge(const uint32_t* __restrict__ in_k
 }
 
 // ------------------------------------------------------ bucket reduction
-// Group g of segment s covers buckets [lo, lo+G): with the running sum from
-// the top, acc = sum (k - lo + 1) B_k; adding lo * run gives
-// sum (k + 1) B_k, the group's share of the window sum (bucket key k holds
-// digit magnitude k + 1).  Groups become records keyed by segment.
+// Window sum S = sum_k (k+1) B_k over bucket keys k < 2^(c-1).  Write
+// k = hi * 2^g + lo (g = (c-1)/2 low bits, h = c-1-g high bits).  Then
+//   S = U + sum_{b<g} 2^b Pl_b + sum_{b<h} 2^(g+b) Ph_b,
+//   U = sum_k B_k,  Pl_b = sum_{lo: bit b} C_lo,  Ph_b = sum_{hi: bit b} R_hi,
+// with row sums R_hi = sum_lo B_{hi,lo} and column sums C_lo = sum_hi B_{hi,lo}.
+// k_br_rowcol: one wave per row or column (2 bucket additions per bucket
+// overall); k_br_bits: one wave per output point (c per window), written as
+// bits[s*16 + j]: j = 0 -> U, 1..g -> Pl, g+1..c-1 -> Ph.  The host applies
+// the powers of two.
 
-__device__ __forceinline__ Xyzz xyzz_mul_small(const Xyzz& p, uint32_t m) {
-    Xyzz r = xyzz_identity();
-    if (m == 0) return r;
-    int top = 31 - __clz(m);
-    r = p;
-    for (int bit = top - 1; bit >= 0; bit--) {
-        r = xyzz_dbl(r);
-        if ((m >> bit) & 1u) xyzz_add(r, p);
+__global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
+                                                   const uint64_t* __restrict__ seg_item0,
+                                                   uint32_t nseg, uint64_t items,
+                                                   const uint4* __restrict__ buckets,
+                                                   uint4* __restrict__ rc) {
+    __shared__ Xyzz buf[256];
+    const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = __lane_id();
+    Xyzz acc = xyzz_identity();
+    uint32_t s = 0;
+    if (item < items) {
+        s = upper_index(seg_item0, nseg, item);
+        const SegDesc sg = segs[s];
+        const uint32_t cb = 31 - __clz(sg.nbk);  // c - 1
+        const uint32_t g = cb >> 1, h = cb - g;
+        const uint32_t r = (uint32_t)(item - seg_item0[s]);
+        if (r < (1u << h)) {  // row r: buckets [r 2^g, (r+1) 2^g)
+            for (uint32_t lo = lane; lo < (1u << g); lo += 64) {
+                Xyzz q = load_xyzz(buckets, sg.bkt0 + ((uint64_t)r << g) + lo);
+                xyzz_add(acc, q);
+            }
+        } else {  // column lo: buckets lo + hi 2^g
+            const uint32_t lo = r - (1u << h);
+            for (uint32_t hi = lane; hi < (1u << h); hi += 64) {
+                Xyzz q = load_xyzz(buckets, sg.bkt0 + ((uint64_t)hi << g) + lo);
+                xyzz_add(acc, q);
+            }
+        }
     }
-    return r;
+    acc = wave_sum_xyzz(acc, buf);
+    if (item < items && lane == 0) store_xyzz(rc, item, acc);
 }
 
-__global__ __launch_bounds__(256) void k_bucket_reduce(
-    const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_grp0, uint32_t nseg,
-    uint32_t G, uint64_t total_groups, const uint4* __restrict__ buckets,
-    uint32_t* __restrict__ rec_keys, uint4* __restrict__ rec_pts) {
-    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total_groups) return;
-    const uint32_t s = upper_index(seg_grp0, nseg, gid);
-    const SegDesc sg = segs[s];
-    const uint32_t g = (uint32_t)(gid - sg.grp0);
-    const uint32_t lo = g * G;
-    const uint32_t hi = min(lo + G, sg.nbk);
-    Xyzz run = xyzz_identity(), acc = xyzz_identity();
-    for (uint32_t k = hi; k > lo; k--) {
-        Xyzz bk = load_xyzz(buckets, sg.bkt0 + k - 1);
-        xyzz_add(run, bk);
-        xyzz_add(acc, run);
+__global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
+                                                 const uint64_t* __restrict__ seg_item0,
+                                                 uint32_t nseg, uint64_t items,
+                                                 const uint4* __restrict__ rc,
+                                                 uint4* __restrict__ bits) {
+    __shared__ Xyzz buf[256];
+    const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = __lane_id();
+    Xyzz acc = xyzz_identity();
+    uint32_t s = 0, j = 0;
+    if (item < items) {
+        s = (uint32_t)(item >> 4);
+        j = (uint32_t)(item & 15u);
+        const SegDesc sg = segs[s];
+        const uint32_t cb = 31 - __clz(sg.nbk);
+        const uint32_t g = cb >> 1, h = cb - g;
+        const uint64_t R0 = seg_item0[s], C0 = R0 + (1u << h);
+        if (j == 0) {  // U = sum of row sums
+            for (uint32_t i = lane; i < (1u << h); i += 64) {
+                Xyzz q = load_xyzz(rc, R0 + i);
+                xyzz_add(acc, q);
+            }
+        } else if (j <= g) {  // Pl_{j-1}
+            const uint32_t b = j - 1;
+            for (uint32_t i = lane; i < (1u << g); i += 64)
+                if ((i >> b) & 1u) {
+                    Xyzz q = load_xyzz(rc, C0 + i);
+                    xyzz_add(acc, q);
+                }
+        } else if (j <= cb) {  // Ph_{j-1-g}
+            const uint32_t b = j - 1 - g;
+            for (uint32_t i = lane; i < (1u << h); i += 64)
+                if ((i >> b) & 1u) {
+                    Xyzz q = load_xyzz(rc, R0 + i);
+                    xyzz_add(acc, q);
+                }
+        }
     }
-    if (lo > 0) {
-        Xyzz t = xyzz_mul_small(run, lo);
-        xyzz_add(acc, t);
-    }
-    rec_keys[gid] = s;
-    store_xyzz(rec_pts, gid, acc);
+    acc = wave_sum_xyzz(acc, buf);
+    if (item < items && lane == 0) store_xyzz(bits, item, acc);
 }
 
 // ------------------------------------------------ synthetic base table
@@ -452,7 +668,7 @@ __global__ __launch_bounds__(256) void k_gen_bases(const uint4* __restrict__ ks,
 // ------------------------------------------------------ test entry points
 // Field / curve primitives exposed for the GPU unit tests.
 
-__global__ void k_test_fp(int op, const uint4* __restrict__ a, const uint4* __restrict__ b,
+__global__ __launch_bounds__(64) void k_test_fp(int op, const uint4* __restrict__ a, const uint4* __restrict__ b,
                           uint4* __restrict__ out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -469,7 +685,7 @@ __global__ void k_test_fp(int op, const uint4* __restrict__ a, const uint4* __re
 }
 
 // op 0: P + Q (XYZZ add), 1: 2P, 2: P + affine(Q.X, Q.Y) with (0,0) as identity
-__global__ void k_test_xyzz(int op, const uint4* __restrict__ p, const uint4* __restrict__ q,
+__global__ __launch_bounds__(64) void k_test_xyzz(int op, const uint4* __restrict__ p, const uint4* __restrict__ q,
                             uint4* __restrict__ out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -485,7 +701,7 @@ __global__ void k_test_xyzz(int op, const uint4* __restrict__ p, const uint4* __
     store_xyzz(out, i, a);
 }
 
-__global__ void k_test_glv(const uint4* __restrict__ scalars, uint4* __restrict__ out,
+__global__ __launch_bounds__(64) void k_test_glv(const uint4* __restrict__ scalars, uint4* __restrict__ out,
                            uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -556,9 +772,15 @@ __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_
     out[t] = r;
 }
 
-void launch_probe(int mode, uint32_t blocks, uint32_t iters, uint64_t* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(256), 0, st, mode, iters, out);
+// plain 16-byte vector copy, the calibration kernel for HBM byte counters
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                              uint64_t n16) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
 }
+
+
 
 // ------------------------------------------------------------ launchers
 
@@ -581,13 +803,18 @@ void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t
 }
 
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
-                      uint32_t ntiles, const void* in_keys, const uint32_t* in_vals, int pass,
-                      uint32_t* hist, uint32_t* seg_valid, uint16_t* out_keys,
+                      uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
+                      const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
+                      uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
                       uint32_t* out_vals, hipStream_t st) {
     if (!ntiles) return;
     hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
                        in_keys, pass, hist);
-    hipLaunchKernelGGL(k_sort_scan, dim3(nseg), dim3(256), 0, st, segs, hist, pass, seg_valid);
+    hipLaunchKernelGGL(k_scan_up, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg,
+                       (const uint32_t*)hist, part);
+    hipLaunchKernelGGL(k_scan_spine, dim3(nseg), dim3(256), 0, st, segs, part);
+    hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg, hist,
+                       (const uint32_t*)part, pass, seg_valid);
     hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
                        in_keys, in_vals, pass, (const uint32_t*)hist, out_keys, out_vals);
 }
@@ -610,22 +837,32 @@ void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t 
                        lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
 }
 
+void launch_merge_runs(uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+                       void* dest, uint32_t* long_runs, hipStream_t st) {
+    if (!nrec) return;
+    hipLaunchKernelGGL(k_merge_runs, dim3(blocks_for(nrec, 256)), dim3(256), 0, st, keys,
+                       (const uint4*)pts, nrec, limit, (uint4*)dest, long_runs);
+}
+
 void launch_merge(const uint32_t* in_keys, const void* in_pts, uint64_t nrec, uint32_t T,
-                  uint32_t* out_keys, void* out_pts, void* dest, hipStream_t st) {
+                  uint32_t* out_keys, void* out_pts, void* dest, const uint32_t* gate,
+                  hipStream_t st) {
     uint64_t lanes = (nrec + T - 1) / T;
     if (!lanes) return;
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(lanes, 256)), dim3(256), 0, st, in_keys,
                        (const uint4*)in_pts, nrec, T, lanes, out_keys, (uint4*)out_pts,
-                       (uint4*)dest);
+                       (uint4*)dest, gate);
 }
 
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_grp0, uint32_t nseg,
-                          uint32_t G, uint64_t groups, const void* buckets, uint32_t* rec_keys,
-                          void* rec_pts, hipStream_t st) {
-    if (!groups) return;
-    hipLaunchKernelGGL(k_bucket_reduce, dim3(blocks_for(groups, 256)), dim3(256), 0, st, segs,
-                       seg_grp0, nseg, G, groups, (const uint4*)buckets, rec_keys,
-                       (uint4*)rec_pts);
+void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_item0, uint32_t nseg,
+                          uint64_t rc_items, const void* buckets, void* rc, void* bits,
+                          hipStream_t st) {
+    if (!rc_items) return;
+    hipLaunchKernelGGL(k_br_rowcol, dim3(blocks_for(rc_items, 4)), dim3(256), 0, st, segs,
+                       seg_item0, nseg, rc_items, (const uint4*)buckets, (uint4*)rc);
+    const uint64_t bit_items = (uint64_t)nseg * 16;
+    hipLaunchKernelGGL(k_br_bits, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
+                       seg_item0, nseg, bit_items, (const uint4*)rc, (uint4*)bits);
 }
 
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
@@ -653,6 +890,17 @@ void launch_test_glv(const void* scalars, void* out, uint64_t n, hipStream_t st)
     if (!n) return;
     hipLaunchKernelGGL(k_test_glv, dim3(blocks_for(n, 64)), dim3(64), 0, st,
                        (const uint4*)scalars, (uint4*)out, n);
+}
+
+void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st) {
+    uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    unsigned blocks = (unsigned)std::min<uint64_t>((n16 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n16);
+}
+
+void launch_probe(int mode, uint32_t blocks, uint32_t iters, uint64_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(256), 0, st, mode, iters, out);
 }
 
 }  // namespace zk

@@ -12,19 +12,23 @@ void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t
                       const SegDesc* segs, const void* scalars, uint64_t total, void* digits,
                       uint32_t* err, hipStream_t st);
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
-                      uint32_t ntiles, const void* in_keys, const uint32_t* in_vals, int pass,
-                      uint32_t* hist, uint32_t* seg_valid, uint16_t* out_keys,
+                      uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
+                      const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
+                      uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
                       uint32_t* out_vals, hipStream_t st);
 void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
+void launch_merge_runs(uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+                       void* dest, uint32_t* long_runs, hipStream_t st);
 void launch_merge(const uint32_t* in_keys, const void* in_pts, uint64_t nrec, uint32_t T,
-                  uint32_t* out_keys, void* out_pts, void* dest, hipStream_t st);
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_grp0, uint32_t nseg,
-                          uint32_t G, uint64_t groups, const void* buckets, uint32_t* rec_keys,
-                          void* rec_pts, hipStream_t st);
+                  uint32_t* out_keys, void* out_pts, void* dest, const uint32_t* gate,
+                  hipStream_t st);
+void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_item0, uint32_t nseg,
+                          uint64_t rc_items, const void* buckets, void* rc, void* bits,
+                          hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,
@@ -32,6 +36,7 @@ void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,
 void launch_test_xyzz(int op, const void* p, const void* q, void* out, uint64_t n,
                       hipStream_t st);
 void launch_test_glv(const void* scalars, void* out, uint64_t n, hipStream_t st);
+void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
 void launch_probe(int mode, uint32_t blocks, uint32_t iters, uint64_t* out, hipStream_t st);
 
 }  // namespace zk

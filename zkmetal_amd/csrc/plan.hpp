@@ -32,6 +32,8 @@ struct SegDesc {
     uint32_t ntiles;  // sort tiles of this segment
     uint32_t msm;     // owning MSM
     uint32_t win;     // window index
+    uint32_t chunk0;  // first histogram-scan chunk (kScanChunk counts each)
+    uint32_t nchunks;
 };
 
 // Merge records: key bit 31 set = no payload (padding); bits 0..30 = index of
@@ -42,5 +44,6 @@ struct SegDesc {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanChunk = 4096;  // histogram counts per scan block (16 per thread)
 
 }  // namespace zk
