@@ -72,6 +72,52 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
  * mixed additions performed).  Returns the number of values written. */
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
+/* Per-size-class window widths: c_by_log2n[ceil(log2 n)] for 0 <= index < len
+ * (<= 33 entries); 0 keeps the cost-model default for that class.  Tables
+ * come from an autotuning sweep (zkmetal_amd.tune) and apply to all calls. */
+ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
+                                     uint32_t len);
+
+/* ---- request server ------------------------------------------------------ */
+/* Continuous batching of MSM requests over the resident table of an engine
+ * (which must keep that table and outlive the server).  `slots` engines, each
+ * with its own stream and workspace, take queued requests in arrival order
+ * (at most max_batch per launch) while their estimated device footprint fits
+ * the free HBM budget; batches on different slots overlap on the device.
+ * Submission copies the scalars; all calls are thread-safe. */
+typedef void* ZkmsmServer;
+typedef struct {
+    uint32_t slots;            /* 0 -> 2 */
+    uint32_t max_batch;        /* 0 -> 64 */
+    uint64_t hbm_budget_bytes; /* 0 -> 16 GiB */
+} ZkmsmServerConfig;
+typedef struct {
+    uint64_t submitted, completed, failed, batches;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result latency */
+    double msms_per_s;                      /* completed / (last result - first submit) */
+    double mean_batch;
+    uint64_t hbm_peak_bytes, hbm_in_use_bytes; /* admitted footprint */
+} ZkmsmServerStats;
+ZkMetalStatus zkmsm_server_create(ZkMetalMSMEngine srs_engine, const ZkmsmServerConfig* cfg,
+                                  ZkmsmServer* out);
+/* n in [1, SRS size]; *ticket identifies the request */
+ZkMetalStatus zkmsm_server_submit(ZkmsmServer server, const uint8_t* scalars, uint32_t n,
+                                  uint64_t* ticket);
+/* same without the copy: the caller keeps `scalars` valid and unchanged until
+ * the request is done (wait returns or poll reports done) */
+ZkMetalStatus zkmsm_server_submit_ref(ZkmsmServer server, const uint8_t* scalars, uint32_t n,
+                                      uint64_t* ticket);
+/* block until the request is done; result: 96 B X|Y|Z; releases the ticket */
+ZkMetalStatus zkmsm_server_wait(ZkmsmServer server, uint64_t ticket, uint8_t* result,
+                                double* latency_ms);
+/* non-blocking: *done = 0 while pending; on done, result written, ticket released */
+ZkMetalStatus zkmsm_server_poll(ZkmsmServer server, uint64_t ticket, uint8_t* result,
+                                int32_t* done);
+ZkMetalStatus zkmsm_server_stats(ZkmsmServer server, ZkmsmServerStats* out);
+void zkmsm_server_destroy(ZkmsmServer server);
+/* bytes of the HBM budget a request of n points is charged */
+uint64_t zkmsm_server_footprint(uint32_t n);
+
 /* ---- seeded synthetic scalars (host) ------------------------------------ */
 /* dist 0: uniform in Fr; 1: 40% 0, 30% < 2^16, 10% in {1, r-1}, 20% uniform;
  * 2: uniform 256-bit.  out: n x 32 B. */
@@ -89,11 +135,12 @@ ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t
 ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, uint8_t* out,
                              uint32_t n);
 
-/* Measured device rates at full occupancy: out3[0] = v_mad_u64_u32
- * (32x32->64 multiply-add) per second, out3[1] = FIPS accumulate steps
- * (multiply-add + carry add) per second, out3[2] = 256-bit Montgomery field
- * multiplies per second.  out3[0] is the INT32-multiply roofline peak. */
-ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3);
+/* Measured device rates: out4[0] = v_mad_u64_u32 (32x32->64 multiply-add)
+ * per second at full occupancy -- the INT32-multiply roofline peak;
+ * out4[1] = FIPS accumulate steps (multiply-add + carry add) per second;
+ * out4[2] = 256-bit Montgomery multiplies per second; out4[3] = ns per
+ * dependent multiply in a single lane (one wave per SIMD). */
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out4);
 /* One launch of a plain device copy of `bytes` bytes (read + write), timed
  * with HIP events; the calibration point for HBM byte counters. */
 ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms);

@@ -107,3 +107,9 @@ def test_default_window_bits_policy():
     assert all(2 <= c <= 16 for c in ws)
     assert ws == sorted(ws)  # non-decreasing with n
     assert zk.default_window_bits(1 << 22) == 16
+
+
+def test_server_footprint_monotone():
+    f = [zk.server_footprint(1 << k) for k in range(0, 24)]
+    assert all(a <= b for a, b in zip(f, f[1:]))
+    assert f[22] > 64 * (1 << 22)  # at least the scalars and digit arrays

@@ -22,6 +22,20 @@ SUCCESS, ERR_NO_GPU, ERR_INVALID_INPUT, ERR_GPU_ERROR, ERR_ALLOC_FAILED = 0, -1,
 _lib = None
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("msms_per_s", ctypes.c_double), ("mean_batch", ctypes.c_double),
+                ("hbm_peak_bytes", ctypes.c_uint64), ("hbm_in_use_bytes", ctypes.c_uint64)]
+
+
 class MSMError(Exception):
     """Mirror of the reference's MSMError (MSMEngine.swift:8-15)."""
 
@@ -78,6 +92,15 @@ def lib():
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
+        "zkmsm_set_window_table": (i32, [vp, vp, u32]),
+        "zkmsm_server_create": (i32, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "zkmsm_server_submit": (i32, [vp, vp, u32, ctypes.POINTER(u64)]),
+        "zkmsm_server_submit_ref": (i32, [vp, vp, u32, ctypes.POINTER(u64)]),
+        "zkmsm_server_wait": (i32, [vp, u64, vp, ctypes.POINTER(ctypes.c_double)]),
+        "zkmsm_server_poll": (i32, [vp, u64, vp, ctypes.POINTER(i32)]),
+        "zkmsm_server_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "zkmsm_server_destroy": (None, [vp]),
+        "zkmsm_server_footprint": (u64, [u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -238,6 +261,12 @@ class MetalMSM:
         _check(rc, "msm_batch_resident")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
 
+    def set_window_table(self, table):
+        """table[ceil(log2 n)] = window width for that size class (0 = default)."""
+        arr = np.ascontiguousarray(np.asarray(table, dtype=np.uint32))
+        _check(lib().zkmsm_set_window_table(self.handle, ctypes.c_void_p(arr.ctypes.data), len(arr)),
+               "set_window_table")
+
     def set_window_bits(self, c):
         _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")
 
@@ -247,10 +276,10 @@ class MetalMSM:
         return {TIMING_KEYS[i]: v[i] for i in range(k)}
 
     def probe_rates(self):
-        """Measured {mad64_per_s, fips_step_per_s, fpmul_per_s} at full occupancy."""
-        v = (ctypes.c_double * 3)()
+        """Measured multiply throughputs (full occupancy) and single-lane latencies."""
+        v = (ctypes.c_double * 4)()
         _check(lib().zkmsm_probe_rates(self.handle, v), "probe_rates")
-        return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2]}
+        return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2], "fpmul_chain_ns": v[3]}
 
     def probe_copy(self, nbytes):
         """ms of one device copy of nbytes (HBM counter calibration)."""
@@ -286,3 +315,62 @@ class MetalMSM:
 def _count(x, size):
     nb = x.nbytes if isinstance(x, np.ndarray) else len(x)
     return nb // size
+
+
+class Server:
+    """Continuous-batching request server over an engine's resident SRS
+    (zkmsm_server_*).  submit() returns a ticket; wait()/poll() return the
+    96-byte result."""
+
+    def __init__(self, engine, slots=2, max_batch=64, hbm_budget_bytes=0):
+        cfg = ServerConfig(slots, max_batch, hbm_budget_bytes)
+        self.handle = ctypes.c_void_p()
+        self._engine = engine  # keeps the SRS owner alive
+        _check(lib().zkmsm_server_create(engine.handle, ctypes.byref(cfg), ctypes.byref(self.handle)),
+               "server_create")
+
+    def submit(self, scalars, copy=True):
+        """copy=False reads the scalars in place: pass a contiguous numpy array
+        and keep it alive and unchanged until the request is done."""
+        sp, keep = _buf(scalars)
+        t = ctypes.c_uint64()
+        if copy:
+            rc = lib().zkmsm_server_submit(self.handle, sp, _count(scalars, 32), ctypes.byref(t))
+        else:
+            if not (isinstance(scalars, np.ndarray) and scalars.flags["C_CONTIGUOUS"]):
+                raise MSMError(ERR_INVALID_INPUT, "submit(copy=False) needs a contiguous numpy array")
+            rc = lib().zkmsm_server_submit_ref(self.handle, sp, _count(scalars, 32), ctypes.byref(t))
+        _check(rc, "server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        out = (ctypes.c_uint8 * 96)()
+        lat = ctypes.c_double()
+        _check(lib().zkmsm_server_wait(self.handle, ticket, out, ctypes.byref(lat)), "server_wait")
+        return bytes(out), lat.value
+
+    def poll(self, ticket):
+        out = (ctypes.c_uint8 * 96)()
+        done = ctypes.c_int32()
+        _check(lib().zkmsm_server_poll(self.handle, ticket, out, ctypes.byref(done)), "server_poll")
+        return bytes(out) if done.value else None
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().zkmsm_server_stats(self.handle, ctypes.byref(s)), "server_stats")
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().zkmsm_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_footprint(n):
+    return int(lib().zkmsm_server_footprint(n))

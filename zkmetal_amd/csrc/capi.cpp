@@ -10,6 +10,7 @@
 #include "../../include/zkmsm.h"
 #include "engine.hpp"
 #include "gen.hpp"
+#include "server.hpp"
 
 using zk::Engine;
 
@@ -292,11 +293,11 @@ ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, ui
     return e->test_glv(scalars, out, n);
 }
 
-ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out3) {
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out4) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
-    return e->probe_rates(out3);
+    return e->probe_rates(out4);
 }
 
 ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms) {
@@ -305,5 +306,91 @@ ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* 
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->probe_copy(bytes, ms);
 }
+
+ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
+                                     uint32_t len) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->set_window_table(c_by_log2n, len);
+}
+
+// ---------------------------------------------------------------- server
+
+ZkMetalStatus zkmsm_server_create(ZkMetalMSMEngine srs_engine, const ZkmsmServerConfig* cfg,
+                                  ZkmsmServer* out) {
+    Engine* e = as_engine(srs_engine);
+    if (!e || !out) return ZKMETAL_ERR_INVALID_INPUT;
+    zk::ServerConfig sc;
+    if (cfg) {
+        if (cfg->slots) sc.slots = cfg->slots;
+        if (cfg->max_batch) sc.max_batch = cfg->max_batch;
+        if (cfg->hbm_budget_bytes) sc.hbm_budget = cfg->hbm_budget_bytes;
+    }
+    const void* table;
+    uint32_t n;
+    {
+        std::lock_guard<std::mutex> lk(e->mutex());
+        table = e->srs_table();
+        n = e->srs_size();
+    }
+    if (!table || n == 0) return ZKMETAL_ERR_INVALID_INPUT;
+    zk::Server* s = nullptr;
+    int32_t rc = zk::Server::create(table, n, sc, &s);
+    *out = s;
+    return rc;
+}
+
+ZkMetalStatus zkmsm_server_submit(ZkmsmServer server, const uint8_t* scalars, uint32_t n,
+                                  uint64_t* ticket) {
+    zk::Server* s = static_cast<zk::Server*>(server);
+    if (!s) return ZKMETAL_ERR_INVALID_INPUT;
+    return s->submit(scalars, n, ticket, true);
+}
+
+ZkMetalStatus zkmsm_server_submit_ref(ZkmsmServer server, const uint8_t* scalars, uint32_t n,
+                                      uint64_t* ticket) {
+    zk::Server* s = static_cast<zk::Server*>(server);
+    if (!s) return ZKMETAL_ERR_INVALID_INPUT;
+    return s->submit(scalars, n, ticket, false);
+}
+
+ZkMetalStatus zkmsm_server_wait(ZkmsmServer server, uint64_t ticket, uint8_t* result,
+                                double* latency_ms) {
+    zk::Server* s = static_cast<zk::Server*>(server);
+    if (!s) return ZKMETAL_ERR_INVALID_INPUT;
+    return s->wait(ticket, result, latency_ms);
+}
+
+ZkMetalStatus zkmsm_server_poll(ZkmsmServer server, uint64_t ticket, uint8_t* result,
+                                int32_t* done) {
+    zk::Server* s = static_cast<zk::Server*>(server);
+    if (!s) return ZKMETAL_ERR_INVALID_INPUT;
+    return s->poll(ticket, result, done);
+}
+
+ZkMetalStatus zkmsm_server_stats(ZkmsmServer server, ZkmsmServerStats* out) {
+    zk::Server* s = static_cast<zk::Server*>(server);
+    if (!s || !out) return ZKMETAL_ERR_INVALID_INPUT;
+    zk::ServerStats st;
+    s->stats(&st);
+    out->submitted = st.submitted;
+    out->completed = st.completed;
+    out->failed = st.failed;
+    out->batches = st.batches;
+    out->p50_ms = st.p50_ms;
+    out->p99_ms = st.p99_ms;
+    out->mean_ms = st.mean_ms;
+    out->max_ms = st.max_ms;
+    out->msms_per_s = st.msms_per_s;
+    out->mean_batch = st.mean_batch;
+    out->hbm_peak_bytes = st.hbm_peak;
+    out->hbm_in_use_bytes = st.hbm_in_use;
+    return ZKMETAL_SUCCESS;
+}
+
+void zkmsm_server_destroy(ZkmsmServer server) { delete static_cast<zk::Server*>(server); }
+
+uint64_t zkmsm_server_footprint(uint32_t n) { return zk::Server::footprint(n); }
 
 }  // extern "C"

@@ -66,13 +66,13 @@ __device__ __forceinline__ Xyzz xyzz_dbl_aff(const Fp& x, const Fp& y) {
 // 2*P (dbl-2008-s-1); identity stays identity (ZZ3 = V*ZZ1 = 0)
 __device__ __forceinline__ Xyzz xyzz_dbl(const Xyzz& p) {
     Fp U = fp_dbl(p.Y);
-    Fp V = fp_sqr(U);
+    Fp V = fp_mul(U, U);
     Fp W = fp_mul(U, V);
     Fp S = fp_mul(p.X, V);
-    Fp xx = fp_sqr(p.X);
+    Fp xx = fp_mul(p.X, p.X);
     Fp M = fp_add(fp_dbl(xx), xx);
     Xyzz r;
-    r.X = fp_sub(fp_sqr(M), fp_dbl(S));
+    r.X = fp_sub(fp_mul(M, M), fp_dbl(S));
     r.Y = fp_sub(fp_mul(M, fp_sub(S, r.X)), fp_mul(W, p.Y));
     r.ZZ = fp_mul(V, p.ZZ);
     r.ZZZ = fp_mul(W, p.ZZZ);
@@ -117,11 +117,11 @@ __device__ __forceinline__ void xyzz_add(Xyzz& acc, const Xyzz& q) {
     Fp P = fp_sub(U2, U1);
     Fp R = fp_sub(S2, S1);
     const bool dbl = fp_is_zero(P) && fp_is_zero(R) && !p_id && !q_id;
-    Fp PP = fp_sqr(P);
+    Fp PP = fp_mul(P, P);
     Fp PPP = fp_mul(P, PP);
     Fp Q = fp_mul(U1, PP);
     Xyzz r;
-    r.X = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
+    r.X = fp_sub(fp_sub(fp_mul(R, R), PPP), fp_dbl(Q));
     r.Y = fp_sub(fp_mul(R, fp_sub(Q, r.X)), fp_mul(S1, PPP));
     r.ZZ = fp_mul(fp_mul(acc.ZZ, q.ZZ), PP);
     r.ZZZ = fp_mul(fp_mul(acc.ZZZ, q.ZZZ), PPP);

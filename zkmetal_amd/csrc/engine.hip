@@ -133,26 +133,32 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     // 32-bit positions / bucket ids on the device
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
-    // accumulation: ~2^18 lanes in flight, at least 4 entries per lane
-    pl.T = (uint32_t)std::max<uint64_t>(4, (pl.entries + (1u << 18) - 1) >> 18);
+    // accumulation: ~2^17 lanes (two waves per SIMD on 256 CUs), >= 4 entries each
+    pl.T = (uint32_t)std::max<uint64_t>(4, (pl.entries + (1u << 17) - 1) >> 17);
     pl.G = 0;
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
+    pl.seg_rcw0.resize(nseg + 1);
     for (uint32_t s = 0; s < nseg; s++) {
         SegDesc& sg = pl.segs[s];
         sg.lane0 = pl.lanes;
         sg.grp0 = pl.groups;
         pl.seg_lane0[s] = pl.lanes;
         pl.seg_grp0[s] = pl.groups;
+        pl.seg_rcw0[s] = pl.rc_waves;
         pl.lanes += (sg.len + pl.T - 1) / pl.T;
-        // bucket reduction items: 2^h rows + 2^g columns (c-1 = g + h bits)
+        // bucket reduction: 2^h rows of 2^g buckets and 2^g columns of 2^h,
+        // max(1, len/8) lanes per row / column, rows and columns wave-aligned
         const uint32_t cb = 31 - __builtin_clz(sg.nbk), g = cb / 2, h = cb - g;
+        const uint64_t pr = (1u << g) >= 8 ? (1u << g) / 8 : 1, pc = (1u << h) >= 8 ? (1u << h) / 8 : 1;
+        pl.rc_waves += (((uint64_t)1 << h) * pr + 63) / 64 + (((uint64_t)1 << g) * pc + 63) / 64;
         pl.groups += (1u << h) + (1u << g);
         for (uint32_t t = 0; t < sg.ntiles; t++) pl.tile_seg.push_back(s);
         for (uint32_t t = 0; t < sg.nchunks; t++) pl.chunk_seg.push_back(s);
     }
     pl.seg_lane0[nseg] = pl.lanes;
     pl.seg_grp0[nseg] = pl.groups;
+    pl.seg_rcw0[nseg] = pl.rc_waves;
     return kOk;
 }
 
@@ -190,7 +196,7 @@ Engine::~Engine() {
 
 // pack the plan's descriptor arrays into one blob (16-byte aligned sections)
 struct DescLayout {
-    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, tile_seg, chunk_seg, total;
+    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, seg_rcw0, tile_seg, chunk_seg, total;
 };
 
 static DescLayout layout_for(const Plan& pl) {
@@ -206,14 +212,14 @@ static DescLayout layout_for(const Plan& pl) {
     L.segs = take(pl.segs.size() * sizeof(SegDesc));
     L.seg_lane0 = take(pl.seg_lane0.size() * 8);
     L.seg_grp0 = take(pl.seg_grp0.size() * 8);
+    L.seg_rcw0 = take(pl.seg_rcw0.size() * 8);
     L.tile_seg = take(pl.tile_seg.size() * 4);
     L.chunk_seg = take(pl.chunk_seg.size() * 4);
     L.total = off;
     return L;
 }
 
-static constexpr uint32_t kMergeFanIn = 8;
-static constexpr uint32_t kRunLimit = 32;
+static constexpr uint32_t kRunLimit = 8;
 
 int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
@@ -236,7 +242,6 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_ALLOC(recAk_, rec_cap * 4);
     ZK_ALLOC(recAp_, rec_cap * 128);
     ZK_ALLOC(recBk_, rec_cap * 4);
-    ZK_ALLOC(recBp_, rec_cap * 128);
     ZK_ALLOC(rc_, pl.groups * 128);
     ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * 128);
     ZK_ALLOC(err_, 16);
@@ -248,6 +253,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     memcpy(hb + L.segs, pl.segs.data(), pl.segs.size() * sizeof(SegDesc));
     memcpy(hb + L.seg_lane0, pl.seg_lane0.data(), pl.seg_lane0.size() * 8);
     memcpy(hb + L.seg_grp0, pl.seg_grp0.data(), pl.seg_grp0.size() * 8);
+    memcpy(hb + L.seg_rcw0, pl.seg_rcw0.data(), pl.seg_rcw0.size() * 8);
     memcpy(hb + L.tile_seg, pl.tile_seg.data(), pl.tile_seg.size() * 4);
     memcpy(hb + L.chunk_seg, pl.chunk_seg.data(), pl.chunk_seg.size() * 4);
     uint8_t* db = (uint8_t*)desc_.p;
@@ -256,6 +262,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     const SegDesc* d_segs = (const SegDesc*)(db + L.segs);
     const uint64_t* d_seg_lane0 = (const uint64_t*)(db + L.seg_lane0);
     const uint64_t* d_seg_grp0 = (const uint64_t*)(db + L.seg_grp0);
+    const uint64_t* d_seg_rcw0 = (const uint64_t*)(db + L.seg_rcw0);
     const uint32_t* d_tile_seg = (const uint32_t*)(db + L.tile_seg);
     const uint32_t* d_chunk_seg = (const uint32_t*)(db + L.chunk_seg);
     uint32_t* d_err = (uint32_t*)err_.p;
@@ -278,27 +285,14 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                       (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
                       buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
     ZK_TRY(hipEventRecord(ev_[3], st));
-    // bucket partials that crossed lane boundaries: short runs in one pass,
-    // long runs (heavy buckets) through gated merge levels
-    uint32_t* d_long = d_err + 1;
-    launch_merge_runs((uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p, d_long, st);
-    {
-        uint32_t* ck = (uint32_t*)recAk_.p;
-        void* cp = recAp_.p;
-        uint32_t* ok = (uint32_t*)recBk_.p;
-        void* op = recBp_.p;
-        uint64_t nrec = 2 * pl.lanes;
-        for (;;) {
-            uint64_t lanes = (nrec + kMergeFanIn - 1) / kMergeFanIn;
-            launch_merge(ck, cp, nrec, kMergeFanIn, ok, op, buckets_.p, d_long, st);
-            if (lanes <= 1) break;
-            nrec = 2 * lanes;
-            std::swap(ck, ok);
-            std::swap(cp, op);
-        }
-    }
+    // bucket partials that crossed lane boundaries: short runs by their first
+    // lane, long runs (heavy buckets) by one wave each
+    const uint32_t long_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 30);
+    launch_merge_runs((const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
+                      d_err + 1, (uint32_t*)recBk_.p, long_cap, st);
     ZK_TRY(hipEventRecord(ev_[4], st));
-    launch_bucket_reduce(d_segs, d_seg_grp0, nseg, pl.groups, buckets_.p, rc_.p, winsum_.p, st);
+    launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, buckets_.p, rc_.p,
+                         winsum_.p, st);
     ZK_TRY(hipEventRecord(ev_[5], st));
     uint8_t* hw = (uint8_t*)hwin_.p;
     const uint64_t bits_bytes = (uint64_t)nseg * 16 * 128;
@@ -421,8 +415,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    std::vector<uint32_t> cs(ns.size(), c_override_);
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * 64);
     ZK_ALLOC(ptable_, total * 128);
@@ -456,8 +449,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    std::vector<uint32_t> cs(ns.size(), c_override_);
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
     if (rc != kOk) return rc;
     ZK_ALLOC(ptable_, total * 128);
     launch_build_table(d_points, ptable_.p, total, stream_);
@@ -537,38 +529,88 @@ int32_t Engine::srs_download(uint8_t* points, uint32_t n) {
 
 int32_t Engine::msm_resident(const void* scalars, bool on_device, const uint32_t* counts,
                              uint32_t nmsm, uint8_t* out96) {
-    if (!scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
     if (srs_n_ == 0) return kInvalidInput;
+    return msm_table(srs_.p, srs_n_, scalars, on_device, counts, nmsm, out96);
+}
+
+int32_t Engine::msm_table(const void* table, uint32_t table_n, const void* scalars,
+                          bool on_device, const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
+    if (!scalars || !counts) return kInvalidInput;
+    if (on_device) return msm_table_parts(table, table_n, nullptr, scalars, counts, nmsm, out96);
+    // host scalars, concatenated: one part per MSM
+    std::vector<const uint8_t*> parts(nmsm);
+    uint64_t off = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        parts[j] = (const uint8_t*)scalars + off * 32;
+        off += counts[j];
+    }
+    return msm_table_parts(table, table_n, parts.data(), nullptr, counts, nmsm, out96);
+}
+
+int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
+                                const uint8_t* const* host_parts, const void* d_scalars,
+                                const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
+    if (!table || !counts || !out96 || nmsm == 0) return kInvalidInput;
+    if (!host_parts && !d_scalars) return kInvalidInput;
     std::vector<uint32_t> ns, idx;
-    std::vector<uint64_t> t0, sc_off;
+    std::vector<uint64_t> t0;
     uint64_t total = 0;
     for (uint32_t j = 0; j < nmsm; j++) {
-        if (counts[j] > srs_n_) return kInvalidInput;
+        if (counts[j] > table_n) return kInvalidInput;
         if (counts[j] == 0) {
             memset(out96 + (uint64_t)j * 96, 0, 96);
             continue;
         }
+        if (host_parts && !host_parts[j]) return kInvalidInput;
         ns.push_back(counts[j]);
-        t0.push_back(0);  // every MSM uses a prefix of the resident table
+        t0.push_back(0);  // every MSM uses a prefix of the table
         idx.push_back(j);
         total += counts[j];
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    std::vector<uint32_t> cs(ns.size(), c_override_);
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), cs.data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
     if (rc != kOk) return rc;
-    const void* d_sc = scalars;
-    if (!on_device) {
+    const void* d_sc = d_scalars;
+    if (host_parts) {
         ZK_ALLOC(scalars_, total * 32);
-        ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
+        uint64_t off = 0;
+        for (uint32_t k = 0; k < idx.size(); k++) {
+            const uint64_t bytes = (uint64_t)ns[k] * 32;
+            ZK_TRY(hipMemcpyAsync((uint8_t*)scalars_.p + off, host_parts[idx[k]], bytes,
+                                  hipMemcpyHostToDevice, stream_));
+            off += bytes;
+        }
         d_sc = scalars_.p;
     }
     std::vector<uint8_t> res(ns.size() * 96);
-    rc = run(pl, d_sc, srs_.p, res.data());
+    rc = run(pl, d_sc, table, res.data());
     if (rc != kOk) return rc;
     for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    return kOk;
+}
+
+std::vector<uint32_t> Engine::window_bits_for(const std::vector<uint32_t>& ns) const {
+    std::vector<uint32_t> cs(ns.size());
+    for (size_t j = 0; j < ns.size(); j++) {
+        uint32_t c = c_override_;
+        if (!c) {
+            uint32_t cls = 0;
+            while ((1ull << cls) < ns[j]) cls++;  // ceil(log2 n)
+            c = ctab_[std::min<uint32_t>(cls, 32)];
+        }
+        cs[j] = c;  // 0 = cost-model default
+    }
+    return cs;
+}
+
+int32_t Engine::set_window_table(const uint32_t* c_by_log2n, uint32_t len) {
+    for (uint32_t i = 0; i <= 32; i++) {
+        uint32_t c = (c_by_log2n && i < len) ? c_by_log2n[i] : 0;
+        if (c != 0 && (c < 2 || c > 16)) return kInvalidInput;
+        ctab_[i] = c;
+    }
     return kOk;
 }
 
@@ -591,25 +633,29 @@ int32_t Engine::probe_copy(uint64_t bytes, double* ms_out) {
     return kOk;
 }
 
-int32_t Engine::probe_rates(double* out3) {
-    if (!out3) return kInvalidInput;
+int32_t Engine::probe_rates(double* out4) {
+    if (!out4) return kInvalidInput;
     hipDeviceProp_t prop;
     ZK_TRY(hipGetDeviceProperties(&prop, dev_));
     const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 8;
     const uint64_t threads = (uint64_t)blocks * 256;
     DevBuf o;
     ZK_ALLOC(o, threads * 8);
-    const uint32_t iters[3] = {4096, 4096, 64};
-    const double per_iter[3] = {8.0, 8.0, 4.0};
-    for (int mode = 0; mode < 3; mode++) {
-        launch_probe(mode, blocks, 16, (uint64_t*)o.p, stream_);  // warm-up
+    // modes 0-2: throughput at full occupancy; 3: one 256-thread block of
+    // dependent multiplies (one wave per SIMD: a single lane's chain)
+    const uint32_t iters[4] = {4096, 4096, 64, 256};
+    const double per_iter[4] = {8.0, 8.0, 4.0, 1.0};
+    for (int mode = 0; mode < 4; mode++) {
+        const uint32_t nb = mode < 3 ? blocks : 1;
+        launch_probe(mode, nb, 16, (uint64_t*)o.p, stream_);  // warm-up
         ZK_TRY(hipEventRecord(ev_[0], stream_));
-        launch_probe(mode, blocks, iters[mode], (uint64_t*)o.p, stream_);
+        launch_probe(mode, nb, iters[mode], (uint64_t*)o.p, stream_);
         ZK_TRY(hipEventRecord(ev_[1], stream_));
         ZK_TRY(hipEventSynchronize(ev_[1]));
         float ms = 0;
         ZK_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-        out3[mode] = (double)threads * iters[mode] * per_iter[mode] / (ms * 1e-3);
+        if (mode < 3) out4[mode] = (double)threads * iters[mode] * per_iter[mode] / (ms * 1e-3);
+        else out4[mode] = ms * 1e6 / iters[mode];  // ns per dependent multiply
     }
     return kOk;
 }

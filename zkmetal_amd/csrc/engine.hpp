@@ -66,10 +66,11 @@ struct Plan {
     std::vector<uint64_t> msm_scalar0;  // nmsm + 1
     std::vector<SegDesc> segs;
     std::vector<uint64_t> seg_lane0;  // nseg + 1
-    std::vector<uint64_t> seg_grp0;   // nseg + 1: first bucket-reduction row/column item
+    std::vector<uint64_t> seg_grp0;   // nseg + 1: first row/column sum of the segment
+    std::vector<uint64_t> seg_rcw0;   // nseg + 1: first row/column wave of the segment
     std::vector<uint32_t> tile_seg;
     std::vector<uint32_t> chunk_seg;
-    uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0;
+    uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0, rc_waves = 0;
     uint32_t tiles = 0, chunks = 0, T = 0, G = 0;
 };
 
@@ -103,6 +104,20 @@ class Engine {
     // MSMs over prefixes of the resident table; scalars on host or device
     int32_t msm_resident(const void* scalars, bool scalars_on_device, const uint32_t* counts,
                          uint32_t nmsm, uint8_t* out96);
+    // same over any device table [P_0, phi(P_0), ...] of table_n bases (shared SRS)
+    int32_t msm_table(const void* table, uint32_t table_n, const void* scalars,
+                      bool scalars_on_device, const uint32_t* counts, uint32_t nmsm,
+                      uint8_t* out96);
+    // scalars given per MSM: host_parts[j] (n_j x 32 B, host) or, when
+    // host_parts is null, one device buffer holding all MSMs' scalars
+    int32_t msm_table_parts(const void* table, uint32_t table_n,
+                            const uint8_t* const* host_parts, const void* d_scalars,
+                            const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
+    const void* srs_table() const { return srs_.p; }
+
+    // per-size-class window widths: c_by_log2n[ceil(log2 n)], 0 = cost model
+    int32_t set_window_table(const uint32_t* c_by_log2n, uint32_t len);
+    std::vector<uint32_t> window_bits_for(const std::vector<uint32_t>& ns) const;
 
     void set_window_bits(uint32_t c) { c_override_ = c; }
     const Timing& last_timing() const { return timing_; }
@@ -114,9 +129,10 @@ class Engine {
     int32_t test_fp(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, uint32_t n);
     int32_t test_xyzz(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n);
     int32_t test_glv(const uint8_t* scalars, uint8_t* out, uint32_t n);
-    // measured rates: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate
-    // steps/s, [2] field multiplies/s, all at full occupancy
-    int32_t probe_rates(double* out3);
+    // measured: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate steps/s,
+    // [2] field multiplies/s (all at full occupancy); [3] ns per dependent
+    // field multiply in a single lane (one wave per SIMD)
+    int32_t probe_rates(double* out4);
     // one launch of a plain copy kernel over `bytes` (HBM counter calibration)
     int32_t probe_copy(uint64_t bytes, double* ms_out);
 
@@ -131,10 +147,11 @@ class Engine {
     hipStream_t stream_ = nullptr;
     std::mutex mu_;
     uint32_t c_override_ = 0;
+    uint32_t ctab_[33] = {};
     Timing timing_{};
 
     DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;
-    DevBuf recAk_, recAp_, recBk_, recBp_, rc_, winsum_, err_;
+    DevBuf recAk_, recAp_, recBk_, rc_, winsum_, err_;  // recBk_: long-run list
     DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;

@@ -9,7 +9,7 @@
 //                     (replaces the CPU counting sort, MSMEngine.swift:595-661)
 //   k_accumulate      load-balanced XYZZ mixed additions into buckets
 //                     (replaces msm_kernels.metal:16-62)
-//   k_merge_runs / k_merge   bucket partials that crossed lane boundaries
+//   k_merge_runs / k_merge_long   bucket partials that crossed lane boundaries
 //   k_br_rowcol / k_br_bits  bucket reduction: window sum = U + sum_b 2^b Q_b,
 //                     Q_b bit-plane sums of the bucket index (replaces the
 //                     running sums of msm_kernels.metal:152-248)
@@ -84,21 +84,31 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return off + inc - v;
 }
 
-// Sum of the 64 lanes' points of each wave of a 256-thread block; lane 0 of
-// every wave returns its wave's total.  buf: 4 x 64 XYZZ slots in LDS.  Every
-// thread of the block must call this (it contains block barriers).
-__device__ __forceinline__ Xyzz wave_sum_xyzz(Xyzz v, Xyzz* buf) {
+__device__ __forceinline__ Fp shfl_down_fp(const Fp& a, uint32_t d) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = __shfl_down(a.v[i], d);
+    return r;
+}
+
+__device__ __forceinline__ Xyzz shfl_down_xyzz(const Xyzz& a, uint32_t d) {
+    Xyzz r;
+    r.X = shfl_down_fp(a.X, d);
+    r.Y = shfl_down_fp(a.Y, d);
+    r.ZZ = shfl_down_fp(a.ZZ, d);
+    r.ZZZ = shfl_down_fp(a.ZZZ, d);
+    return r;
+}
+
+// Sum over aligned groups of P lanes (P a power of two <= 64, uniform across
+// the wave); the group's first lane returns the group total.  Cross-lane data
+// moves with ds_bpermute shuffles, no LDS, no barriers.
+__device__ __forceinline__ Xyzz group_sum_xyzz(Xyzz v, uint32_t P) {
     const uint32_t lane = __lane_id();
-    Xyzz* mine = buf + (threadIdx.x & ~63u);
 #pragma unroll 1
-    for (uint32_t s = 32; s >= 1; s >>= 1) {
-        if (lane >= s && lane < 2 * s) mine[lane] = v;
-        __syncthreads();
-        if (lane < s) {
-            Xyzz o = mine[lane + s];
-            xyzz_add(v, o);
-        }
-        __syncthreads();
+    for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
+        Xyzz o = shfl_down_xyzz(v, s);
+        if ((lane & (P - 1)) < s) xyzz_add(v, o);
     }
     return v;
 }
@@ -414,9 +424,14 @@ __global__ __launch_bounds__(256) void k_accumulate(
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
     Xyzz acc = xyzz_identity();
+    // software pipeline: the next entry's key, value and base are loaded
+    // before the current mixed addition, hiding the random 64-byte gather
+    uint32_t k = cur, v = vals[b];
+    Aff q = load_aff(tb, v & 0x7fffffffu);
     for (uint64_t p = b; p < e; p++) {
-        const uint32_t k = keys[p];
-        const uint32_t v = vals[p];
+        const uint64_t pn = min(p + 1, e - 1);
+        const uint32_t kn = keys[pn], vn = vals[pn];
+        const Aff qn = load_aff(tb, vn & 0x7fffffffu);
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
             const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
@@ -430,10 +445,12 @@ __global__ __launch_bounds__(256) void k_accumulate(
             acc = xyzz_identity();
             cur = k;
         }
-        Aff q = load_aff(tb, v & 0x7fffffffu);
         const bool q_id = aff_is_id(q);
         if (v >> 31) q.y = fp_neg(q.y);
         xyzz_madd(acc, q.x, q.y, q_id);
+        k = kn;
+        v = vn;
+        q = qn;
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
     const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
@@ -450,184 +467,174 @@ __global__ __launch_bounds__(256) void k_accumulate(
 
 // ------------------------------------------------------ partial merging
 // Records with equal key are contiguous (a bucket's partials come from
-// consecutive lanes).  k_merge_runs: the lane at the first record of a run of
-// at most `limit` records sums it, writes dest[key] and marks the records
-// consumed (padding); longer runs raise *long_runs and are left to k_merge,
-// which only does work when that flag is set.
+// consecutive lanes; padding of empty lanes carries key 0x7fffffff).
+// k_merge_runs: the lane holding the first record of a run of at most `limit`
+// records sums it and writes dest[key]; a longer run is appended to
+// long_list.  k_merge_long: one wave per listed run, lanes striding over its
+// records, then a shuffle tree: depth len/64 + 6 additions.
 
-__global__ __launch_bounds__(256) void k_merge_runs(uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     uint32_t limit, uint4* __restrict__ dest,
-                                                    uint32_t* __restrict__ long_runs) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec) return;
-    const uint32_t k = keys[i] & ZK_REC_KEYMASK;
-    if (k == ZK_REC_KEYMASK) return;  // padding of empty lanes
-    if (i > 0 && (keys[i - 1] & ZK_REC_KEYMASK) == k) return;
-    uint64_t j = i;
-    while (j < nrec && j - i < limit && (keys[j] & ZK_REC_KEYMASK) == k) j++;
-    if (j < nrec && (keys[j] & ZK_REC_KEYMASK) == k) {
-        atomicOr(long_runs, 1u);
-        return;
+                                                    uint32_t* __restrict__ long_count,
+                                                    uint32_t* __restrict__ long_list,
+                                                    uint32_t long_cap) {
+    // lane i looks at records 2i and 2i+1 (one accumulation lane's head and
+    // tail slots); at most one of them starts a run with a payload
+    const uint64_t i2 = 2 * ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i2 >= nrec) return;
+    for (uint64_t i = i2; i < i2 + 2 && i < nrec; i++) {
+        const uint32_t k = keys[i] & ZK_REC_KEYMASK;
+        if (k == ZK_REC_KEYMASK) continue;  // padding
+        if (i > 0 && (keys[i - 1] & ZK_REC_KEYMASK) == k) continue;
+        uint64_t j = i;
+        while (j < nrec && j - i < limit && (keys[j] & ZK_REC_KEYMASK) == k) j++;
+        if (j < nrec && (keys[j] & ZK_REC_KEYMASK) == k) {
+            const uint32_t slot = atomicAdd(long_count, 1u);
+            if (slot < long_cap) long_list[slot] = (uint32_t)i;
+            continue;
+        }
+        bool has = false;
+        Xyzz acc = xyzz_identity();
+        for (uint64_t p = i; p < j; p++) {
+            if (!(keys[p] & ZK_REC_INVALID)) {
+                Xyzz q = load_xyzz(pts, p);
+                if (has) xyzz_add(acc, q);
+                else acc = q;
+                has = true;
+            }
+        }
+        if (has) store_xyzz(dest, k, acc);
     }
+}
+
+__global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__ keys,
+                                                    const uint4* __restrict__ pts, uint64_t nrec,
+                                                    const uint32_t* __restrict__ long_count,
+                                                    const uint32_t* __restrict__ long_list,
+                                                    uint32_t long_cap, uint4* __restrict__ dest) {
+    const uint32_t run = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t n = min(*long_count, long_cap);
+    if (run >= n) return;
+    const uint32_t lane = __lane_id();
+    const uint64_t start = long_list[run];
+    const uint32_t k = keys[start] & ZK_REC_KEYMASK;
     bool has = false;
     Xyzz acc = xyzz_identity();
-    for (uint64_t p = i; p < j; p++) {
+    for (uint64_t p = start + lane; p < nrec; p += 64) {
         const uint32_t kr = keys[p];
+        if ((kr & ZK_REC_KEYMASK) != k) break;
         if (!(kr & ZK_REC_INVALID)) {
             Xyzz q = load_xyzz(pts, p);
             if (has) xyzz_add(acc, q);
             else acc = q;
             has = true;
-            keys[p] = kr | ZK_REC_INVALID;
         }
     }
-    if (has) store_xyzz(dest, k, acc);
-}
-
-// One level of the general segmented merge: records [L*T, (L+1)*T) per lane;
-// runs of equal key are summed (padding skipped).  A run closed on both sides
-// with at least one payload is final and written to dest[key]; open runs
-// become the lane's two output records.  Returns at once unless *gate != 0.
-__global__ __launch_bounds__(256) void k_merge(const uint32_t* __restrict__ in_keys,
-                                               const uint4* __restrict__ in_pts, uint64_t nrec,
-                                               uint32_t T, uint64_t lanes,
-                                               uint32_t* __restrict__ out_keys,
-                                               uint4* __restrict__ out_pts,
-                                               uint4* __restrict__ dest,
-                                               const uint32_t* __restrict__ gate) {
-    if (*gate == 0) return;
-    const uint64_t L = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= lanes) return;
-    const uint64_t b = L * T;
-    const uint64_t e = min(b + T, nrec);
-    if (b >= e) {
-        out_keys[2 * L] = ZK_REC_INVALID | ZK_REC_KEYMASK;
-        out_keys[2 * L + 1] = ZK_REC_INVALID | ZK_REC_KEYMASK;
-        return;
-    }
-    uint32_t cur = in_keys[b] & ZK_REC_KEYMASK;
-    const bool head_open = (b > 0) && (in_keys[b - 1] & ZK_REC_KEYMASK) == cur;
-    bool first = true, has = false;
-    Xyzz acc = xyzz_identity();
-    for (uint64_t p = b; p < e; p++) {
-        const uint32_t kr = in_keys[p];
-        const uint32_t k = kr & ZK_REC_KEYMASK;
-        if (k != cur) {
-            if (first && head_open) {
-                put_record(out_keys, out_pts, 2 * L, cur, has, acc);
-            } else {
-                if (has) store_xyzz(dest, cur, acc);
-                if (first) put_record(out_keys, out_pts, 2 * L, cur, false, acc);
-            }
-            first = false;
-            has = false;
-            acc = xyzz_identity();
-            cur = k;
-        }
-        if (!(kr & ZK_REC_INVALID)) {
-            Xyzz q = load_xyzz(in_pts, p);
-            if (has) xyzz_add(acc, q);
-            else acc = q;
-            has = true;
-        }
-    }
-    const bool tail_open = (e < nrec) && (in_keys[e] & ZK_REC_KEYMASK) == cur;
-    if (first) {
-        const bool open = head_open || tail_open;
-        if (!open && has) store_xyzz(dest, cur, acc);
-        put_record(out_keys, out_pts, 2 * L, cur, open && has, acc);
-        put_record(out_keys, out_pts, 2 * L + 1, cur, false, acc);
-    } else {
-        if (!tail_open && has) store_xyzz(dest, cur, acc);
-        put_record(out_keys, out_pts, 2 * L + 1, cur, tail_open && has, acc);
-    }
+    acc = group_sum_xyzz(acc, 64);  // identity lanes add nothing
+    if (__ballot(has) && lane == 0) store_xyzz(dest, k, acc);
 }
 
 // ------------------------------------------------------ bucket reduction
 // Window sum S = sum_k (k+1) B_k over bucket keys k < 2^(c-1).  Write
 // k = hi * 2^g + lo (g = (c-1)/2 low bits, h = c-1-g high bits).  Then
 //   S = U + sum_{b<g} 2^b Pl_b + sum_{b<h} 2^(g+b) Ph_b,
-//   U = sum_k B_k,  Pl_b = sum_{lo: bit b} C_lo,  Ph_b = sum_{hi: bit b} R_hi,
-// with row sums R_hi = sum_lo B_{hi,lo} and column sums C_lo = sum_hi B_{hi,lo}.
-// k_br_rowcol: one wave per row or column (2 bucket additions per bucket
-// overall); k_br_bits: one wave per output point (c per window), written as
-// bits[s*16 + j]: j = 0 -> U, 1..g -> Pl, g+1..c-1 -> Ph.  The host applies
-// the powers of two.
+//   U = sum_lo C_lo,  Pl_b = sum_{lo: bit b} C_lo,  Ph_b = sum_{hi: bit b} R_hi,
+// with row sums R_hi = sum_lo B_{hi,lo} and column sums C_lo = sum_hi B_{hi,lo}
+// (two additions per bucket in all).  k_br_rowcol: a row of 2^g buckets is
+// summed by P_r = max(1, 2^g/8) lanes (8 buckets each, then a log2(P_r)
+// shuffle tree), a column of 2^h buckets by P_c = max(1, 2^h/8) lanes; rows
+// and columns of a segment start on wave boundaries so P is wave-uniform.
+// k_br_bits: one wave per output point, bits[s*16 + j]: j = 0 -> U,
+// 1..g -> Pl_{j-1}, g+1..c-1 -> Ph_{j-1-g}.  The host applies the powers of 2.
+
+__device__ __forceinline__ void seg_shape(uint32_t nbk, uint32_t& g, uint32_t& h) {
+    const uint32_t cb = 31 - __clz(nbk);  // c - 1
+    g = cb >> 1;
+    h = cb - g;
+}
+
+__device__ __forceinline__ uint32_t lanes_per(uint32_t len) { return len >= 8 ? len / 8 : 1; }
 
 __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
-                                                   const uint64_t* __restrict__ seg_item0,
-                                                   uint32_t nseg, uint64_t items,
+                                                   const uint64_t* __restrict__ seg_wave0,
+                                                   const uint64_t* __restrict__ seg_rc0,
+                                                   uint32_t nseg, uint64_t waves,
                                                    const uint4* __restrict__ buckets,
                                                    uint4* __restrict__ rc) {
-    __shared__ Xyzz buf[256];
-    const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= waves) return;
     const uint32_t lane = __lane_id();
+    const uint32_t s = upper_index(seg_wave0, nseg, wave);
+    const SegDesc sg = segs[s];
+    uint32_t g, h;
+    seg_shape(sg.nbk, g, h);
+    const uint32_t nrow = 1u << h, ncol = 1u << g;
+    const uint32_t Pr = lanes_per(ncol), Pc = lanes_per(nrow);
+    const uint32_t row_waves = (nrow * Pr + 63) / 64;
+    uint32_t w = (uint32_t)(wave - seg_wave0[s]);
+    const bool rows = w < row_waves;
+    if (!rows) w -= row_waves;
+    const uint32_t P = rows ? Pr : Pc;
+    const uint32_t li = w * 64 + lane;
+    const uint32_t item = li / P, part = li % P;
+    const uint32_t nitem = rows ? nrow : ncol;
+    const uint32_t len = rows ? ncol : nrow;  // buckets per item
+    const uint32_t per = len / P;
     Xyzz acc = xyzz_identity();
-    uint32_t s = 0;
-    if (item < items) {
-        s = upper_index(seg_item0, nseg, item);
-        const SegDesc sg = segs[s];
-        const uint32_t cb = 31 - __clz(sg.nbk);  // c - 1
-        const uint32_t g = cb >> 1, h = cb - g;
-        const uint32_t r = (uint32_t)(item - seg_item0[s]);
-        if (r < (1u << h)) {  // row r: buckets [r 2^g, (r+1) 2^g)
-            for (uint32_t lo = lane; lo < (1u << g); lo += 64) {
-                Xyzz q = load_xyzz(buckets, sg.bkt0 + ((uint64_t)r << g) + lo);
-                xyzz_add(acc, q);
-            }
-        } else {  // column lo: buckets lo + hi 2^g
-            const uint32_t lo = r - (1u << h);
-            for (uint32_t hi = lane; hi < (1u << h); hi += 64) {
-                Xyzz q = load_xyzz(buckets, sg.bkt0 + ((uint64_t)hi << g) + lo);
-                xyzz_add(acc, q);
-            }
+    if (item < nitem) {
+        for (uint32_t e = 0; e < per; e++) {
+            const uint32_t x = part * per + e;
+            const uint64_t k = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
+            Xyzz q = load_xyzz(buckets, sg.bkt0 + k);
+            if (e == 0) acc = q;
+            else xyzz_add(acc, q);
         }
     }
-    acc = wave_sum_xyzz(acc, buf);
-    if (item < items && lane == 0) store_xyzz(rc, item, acc);
+    acc = group_sum_xyzz(acc, P);
+    if (item < nitem && part == 0) store_xyzz(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
+}
+
+// index of the i-th value with bit b set: insert a 1 at bit position b
+__device__ __forceinline__ uint32_t with_bit(uint32_t i, uint32_t b) {
+    return ((i >> b) << (b + 1)) | (1u << b) | (i & ((1u << b) - 1u));
 }
 
 __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
-                                                 const uint64_t* __restrict__ seg_item0,
-                                                 uint32_t nseg, uint64_t items,
-                                                 const uint4* __restrict__ rc,
+                                                 const uint64_t* __restrict__ seg_rc0,
+                                                 uint32_t nseg, const uint4* __restrict__ rc,
                                                  uint4* __restrict__ bits) {
-    __shared__ Xyzz buf[256];
     const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t s = (uint32_t)(item >> 4), j = (uint32_t)(item & 15u);
+    if (s >= nseg) return;
     const uint32_t lane = __lane_id();
-    Xyzz acc = xyzz_identity();
-    uint32_t s = 0, j = 0;
-    if (item < items) {
-        s = (uint32_t)(item >> 4);
-        j = (uint32_t)(item & 15u);
-        const SegDesc sg = segs[s];
-        const uint32_t cb = 31 - __clz(sg.nbk);
-        const uint32_t g = cb >> 1, h = cb - g;
-        const uint64_t R0 = seg_item0[s], C0 = R0 + (1u << h);
-        if (j == 0) {  // U = sum of row sums
-            for (uint32_t i = lane; i < (1u << h); i += 64) {
-                Xyzz q = load_xyzz(rc, R0 + i);
-                xyzz_add(acc, q);
-            }
-        } else if (j <= g) {  // Pl_{j-1}
-            const uint32_t b = j - 1;
-            for (uint32_t i = lane; i < (1u << g); i += 64)
-                if ((i >> b) & 1u) {
-                    Xyzz q = load_xyzz(rc, C0 + i);
-                    xyzz_add(acc, q);
-                }
-        } else if (j <= cb) {  // Ph_{j-1-g}
-            const uint32_t b = j - 1 - g;
-            for (uint32_t i = lane; i < (1u << h); i += 64)
-                if ((i >> b) & 1u) {
-                    Xyzz q = load_xyzz(rc, R0 + i);
-                    xyzz_add(acc, q);
-                }
-        }
+    const SegDesc sg = segs[s];
+    uint32_t g, h;
+    seg_shape(sg.nbk, g, h);
+    if (j > g + h) return;
+    const uint64_t R0 = seg_rc0[s], C0 = R0 + (1u << h);
+    // element list: U -> all columns; Pl_b -> columns with bit b; Ph_b -> rows with bit b
+    uint64_t base;
+    uint32_t count, b = 0;
+    bool masked;
+    if (j == 0) {
+        base = C0; count = 1u << g; masked = false;
+    } else if (j <= g) {
+        base = C0; b = j - 1; count = 1u << (g - 1); masked = true;
+    } else {
+        base = R0; b = j - 1 - g; count = 1u << (h - 1); masked = true;
     }
-    acc = wave_sum_xyzz(acc, buf);
-    if (item < items && lane == 0) store_xyzz(bits, item, acc);
+    Xyzz acc = xyzz_identity();
+    bool first = true;
+    for (uint32_t i = lane; i < count; i += 64) {
+        const uint32_t idx = masked ? with_bit(i, b) : i;
+        Xyzz q = load_xyzz(rc, base + idx);
+        if (first) acc = q;
+        else xyzz_add(acc, q);
+        first = false;
+    }
+    acc = group_sum_xyzz(acc, 64);
+    if (lane == 0) store_xyzz(bits, item, acc);
 }
 
 // ------------------------------------------------ synthetic base table
@@ -723,6 +730,20 @@ __global__ __launch_bounds__(64) void k_test_glv(const uint4* __restrict__ scala
 
 __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_t* __restrict__ out) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 3) {  // latency: one dependent chain per lane
+        Fp a, b;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            b.v[i] = t * 2654435761u + i;
+            a.v[i] = t * 40503u + i * 7u;
+        }
+        for (uint32_t it = 0; it < iters; it++) a = fp_mul(a, b);
+        uint64_t x = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) x ^= a.v[i];
+        out[t] = x;
+        return;
+    }
     if (mode == 2) {
         Fp a[4], b;
 #pragma unroll
@@ -837,32 +858,28 @@ void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t 
                        lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
 }
 
-void launch_merge_runs(uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
-                       void* dest, uint32_t* long_runs, hipStream_t st) {
+void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+                       void* dest, uint32_t* long_count, uint32_t* long_list, uint32_t long_cap,
+                       hipStream_t st) {
     if (!nrec) return;
-    hipLaunchKernelGGL(k_merge_runs, dim3(blocks_for(nrec, 256)), dim3(256), 0, st, keys,
-                       (const uint4*)pts, nrec, limit, (uint4*)dest, long_runs);
+    hipLaunchKernelGGL(k_merge_runs, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
+                       (const uint4*)pts, nrec, limit, (uint4*)dest, long_count, long_list,
+                       long_cap);
+    if (!long_cap) return;
+    hipLaunchKernelGGL(k_merge_long, dim3(blocks_for(long_cap, 4)), dim3(256), 0, st, keys,
+                       (const uint4*)pts, nrec, (const uint32_t*)long_count,
+                       (const uint32_t*)long_list, long_cap, (uint4*)dest);
 }
 
-void launch_merge(const uint32_t* in_keys, const void* in_pts, uint64_t nrec, uint32_t T,
-                  uint32_t* out_keys, void* out_pts, void* dest, const uint32_t* gate,
-                  hipStream_t st) {
-    uint64_t lanes = (nrec + T - 1) / T;
-    if (!lanes) return;
-    hipLaunchKernelGGL(k_merge, dim3(blocks_for(lanes, 256)), dim3(256), 0, st, in_keys,
-                       (const uint4*)in_pts, nrec, T, lanes, out_keys, (uint4*)out_pts,
-                       (uint4*)dest, gate);
-}
-
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_item0, uint32_t nseg,
-                          uint64_t rc_items, const void* buckets, void* rc, void* bits,
-                          hipStream_t st) {
-    if (!rc_items) return;
-    hipLaunchKernelGGL(k_br_rowcol, dim3(blocks_for(rc_items, 4)), dim3(256), 0, st, segs,
-                       seg_item0, nseg, rc_items, (const uint4*)buckets, (uint4*)rc);
+void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
+                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves,
+                          const void* buckets, void* rc, void* bits, hipStream_t st) {
+    if (!rc_waves) return;
+    hipLaunchKernelGGL(k_br_rowcol, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
+                       seg_wave0, seg_rc0, nseg, rc_waves, (const uint4*)buckets, (uint4*)rc);
     const uint64_t bit_items = (uint64_t)nseg * 16;
     hipLaunchKernelGGL(k_br_bits, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
-                       seg_item0, nseg, bit_items, (const uint4*)rc, (uint4*)bits);
+                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);
 }
 
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,

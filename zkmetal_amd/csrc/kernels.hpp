@@ -21,14 +21,12 @@ void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t 
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
-void launch_merge_runs(uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
-                       void* dest, uint32_t* long_runs, hipStream_t st);
-void launch_merge(const uint32_t* in_keys, const void* in_pts, uint64_t nrec, uint32_t T,
-                  uint32_t* out_keys, void* out_pts, void* dest, const uint32_t* gate,
-                  hipStream_t st);
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_item0, uint32_t nseg,
-                          uint64_t rc_items, const void* buckets, void* rc, void* bits,
-                          hipStream_t st);
+void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+                       void* dest, uint32_t* long_count, uint32_t* long_list, uint32_t long_cap,
+                       hipStream_t st);
+void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
+                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves,
+                          const void* buckets, void* rc, void* bits, hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,
