@@ -53,6 +53,10 @@ ZkMetalStatus zkmsm_msm_batch_device(ZkMetalMSMEngine engine, const void* device
 ZkMetalStatus zkmsm_device_alloc(uint64_t bytes, void** device_ptr);
 ZkMetalStatus zkmsm_device_free(void* device_ptr);
 ZkMetalStatus zkmsm_copy_to_device(void* device_dst, const void* host_src, uint64_t bytes);
+/* page-locked host memory: scalars submitted from it reach the device by DMA
+ * without a staging copy */
+ZkMetalStatus zkmsm_host_alloc(uint64_t bytes, void** host_ptr);
+ZkMetalStatus zkmsm_host_free(void* host_ptr);
 ZkMetalStatus zkmsm_copy_to_host(void* host_dst, const void* device_src, uint64_t bytes);
 
 /* ---- tuning / introspection -------------------------------------------- */
@@ -135,12 +139,13 @@ ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t
 ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, uint8_t* out,
                              uint32_t n);
 
-/* Measured device rates: out4[0] = v_mad_u64_u32 (32x32->64 multiply-add)
+/* Measured device rates: out6[0] = v_mad_u64_u32 (32x32->64 multiply-add)
  * per second at full occupancy -- the INT32-multiply roofline peak;
- * out4[1] = FIPS accumulate steps (multiply-add + carry add) per second;
- * out4[2] = 256-bit Montgomery multiplies per second; out4[3] = ns per
- * dependent multiply in a single lane (one wave per SIMD). */
-ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out4);
+ * out6[1] = FIPS accumulate steps (multiply-add + carry add) per second;
+ * out6[2] = 256-bit Montgomery multiplies per second (8 x 32-bit limbs);
+ * out6[3] = ns per dependent multiply in a single lane (one wave per SIMD);
+ * out6[4], out6[5] = the same two for the 9 x 29-bit limb multiply. */
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out6);
 /* One launch of a plain device copy of `bytes` bytes (read + write), timed
  * with HIP events; the calibration point for HBM byte counters. */
 ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms);

@@ -41,6 +41,17 @@ def test_fp_mul_add_sub(engine):
         assert rd(sub, i) == B.to_mont((x - y) % B.P), i
 
 
+def test_fq29_mul_matches_montgomery(engine):
+    """9 x 29-bit limb multiply (R = 2^261), converted in and out of the
+    8 x 32-bit form: must equal the 2^256-Montgomery product exactly."""
+    a, b = _inputs(2048, 11)
+    A = b"".join(fe(B.to_mont(x)) for x in a)
+    Bb = b"".join(fe(B.to_mont(x)) for x in b)
+    out = engine.test_fp(6, A, Bb)
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert rd(out, i) == B.to_mont(x * y % B.P), i
+
+
 def test_fp_inv_neg_canon(engine):
     a, _ = _inputs(512, 2)
     a = [x for x in a if x]

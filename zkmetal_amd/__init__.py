@@ -81,6 +81,8 @@ def lib():
         "zkmsm_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_device_alloc": (i32, [u64, ctypes.POINTER(vp)]),
         "zkmsm_device_free": (i32, [vp]),
+        "zkmsm_host_alloc": (i32, [u64, ctypes.POINTER(vp)]),
+        "zkmsm_host_free": (i32, [vp]),
         "zkmsm_copy_to_device": (i32, [vp, vp, u64]),
         "zkmsm_copy_to_host": (i32, [vp, vp, u64]),
         "zkmsm_set_window_bits": (i32, [vp, u32]),
@@ -160,6 +162,29 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             lib().zkmsm_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Page-locked host memory; `.array` is a uint8 numpy view of it."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        _check(lib().zkmsm_host_alloc(nbytes, ctypes.byref(self.ptr)), "host_alloc")
+        self.nbytes = nbytes
+        raw = (ctypes.c_uint8 * nbytes).from_address(self.ptr.value)
+        self.array = np.frombuffer(raw, dtype=np.uint8)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().zkmsm_host_free(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):
@@ -277,9 +302,10 @@ class MetalMSM:
 
     def probe_rates(self):
         """Measured multiply throughputs (full occupancy) and single-lane latencies."""
-        v = (ctypes.c_double * 4)()
+        v = (ctypes.c_double * 6)()
         _check(lib().zkmsm_probe_rates(self.handle, v), "probe_rates")
-        return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2], "fpmul_chain_ns": v[3]}
+        return {"mad64_per_s": v[0], "fips_step_per_s": v[1], "fpmul_per_s": v[2], "fpmul_chain_ns": v[3],
+                "fq29mul_per_s": v[4], "fq29mul_chain_ns": v[5]}
 
     def probe_copy(self, nbytes):
         """ms of one device copy of nbytes (HBM counter calibration)."""

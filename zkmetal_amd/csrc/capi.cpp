@@ -220,6 +220,21 @@ ZkMetalStatus zkmsm_device_free(void* device_ptr) {
     return hipFree(device_ptr) == hipSuccess ? ZKMETAL_SUCCESS : ZKMETAL_ERR_GPU_ERROR;
 }
 
+ZkMetalStatus zkmsm_host_alloc(uint64_t bytes, void** host_ptr) {
+    if (!host_ptr || bytes == 0) return ZKMETAL_ERR_INVALID_INPUT;
+    if (hipHostMalloc(host_ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *host_ptr = nullptr;
+        return ZKMETAL_ERR_ALLOC_FAILED;
+    }
+    return ZKMETAL_SUCCESS;
+}
+
+ZkMetalStatus zkmsm_host_free(void* host_ptr) {
+    if (!host_ptr) return ZKMETAL_SUCCESS;
+    return hipHostFree(host_ptr) == hipSuccess ? ZKMETAL_SUCCESS : ZKMETAL_ERR_GPU_ERROR;
+}
+
 ZkMetalStatus zkmsm_copy_to_device(void* dst, const void* src, uint64_t bytes) {
     if (!dst || !src) return ZKMETAL_ERR_INVALID_INPUT;
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? ZKMETAL_SUCCESS
@@ -293,11 +308,11 @@ ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, ui
     return e->test_glv(scalars, out, n);
 }
 
-ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out4) {
+ZkMetalStatus zkmsm_probe_rates(ZkMetalMSMEngine engine, double* out6) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
-    return e->probe_rates(out4);
+    return e->probe_rates(out6);
 }
 
 ZkMetalStatus zkmsm_probe_copy(ZkMetalMSMEngine engine, uint64_t bytes, double* ms) {

@@ -133,8 +133,12 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     // 32-bit positions / bucket ids on the device
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
-    // accumulation: ~2^17 lanes (two waves per SIMD on 256 CUs), >= 4 entries each
-    pl.T = (uint32_t)std::max<uint64_t>(4, (pl.entries + (1u << 17) - 1) >> 17);
+    // accumulation: ~2^17 lanes (two waves per SIMD on 256 CUs), >= 4 entries
+    // each, and (up to 32) at least half a bucket's mean entry count, so that
+    // a bucket's partials span few lanes and merge without a long-run pass
+    const uint64_t half_bucket = pl.entries / (2 * std::max<uint64_t>(1, pl.buckets));
+    pl.T = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(4, std::min<uint64_t>(32, half_bucket)),
+                                        (pl.entries + (1u << 17) - 1) >> 17);
     pl.G = 0;
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
@@ -633,20 +637,21 @@ int32_t Engine::probe_copy(uint64_t bytes, double* ms_out) {
     return kOk;
 }
 
-int32_t Engine::probe_rates(double* out4) {
-    if (!out4) return kInvalidInput;
+int32_t Engine::probe_rates(double* out6) {
+    if (!out6) return kInvalidInput;
     hipDeviceProp_t prop;
     ZK_TRY(hipGetDeviceProperties(&prop, dev_));
     const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 8;
     const uint64_t threads = (uint64_t)blocks * 256;
     DevBuf o;
     ZK_ALLOC(o, threads * 8);
-    // modes 0-2: throughput at full occupancy; 3: one 256-thread block of
-    // dependent multiplies (one wave per SIMD: a single lane's chain)
-    const uint32_t iters[4] = {4096, 4096, 64, 256};
-    const double per_iter[4] = {8.0, 8.0, 4.0, 1.0};
-    for (int mode = 0; mode < 4; mode++) {
-        const uint32_t nb = mode < 3 ? blocks : 1;
+    // modes 0-2, 4: throughput at full occupancy; 3, 5: one 256-thread block
+    // of dependent multiplies (one wave per SIMD: a single lane's chain)
+    const uint32_t iters[6] = {4096, 4096, 64, 256, 64, 256};
+    const double per_iter[6] = {8.0, 8.0, 4.0, 1.0, 4.0, 1.0};
+    for (int mode = 0; mode < 6; mode++) {
+        const bool chain = mode == 3 || mode == 5;
+        const uint32_t nb = chain ? 1 : blocks;
         launch_probe(mode, nb, 16, (uint64_t*)o.p, stream_);  // warm-up
         ZK_TRY(hipEventRecord(ev_[0], stream_));
         launch_probe(mode, nb, iters[mode], (uint64_t*)o.p, stream_);
@@ -654,8 +659,8 @@ int32_t Engine::probe_rates(double* out4) {
         ZK_TRY(hipEventSynchronize(ev_[1]));
         float ms = 0;
         ZK_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-        if (mode < 3) out4[mode] = (double)threads * iters[mode] * per_iter[mode] / (ms * 1e-3);
-        else out4[mode] = ms * 1e6 / iters[mode];  // ns per dependent multiply
+        if (!chain) out6[mode] = (double)threads * iters[mode] * per_iter[mode] / (ms * 1e-3);
+        else out6[mode] = ms * 1e6 / iters[mode];  // ns per dependent multiply
     }
     return kOk;
 }

@@ -132,7 +132,7 @@ class Engine {
     // measured: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate steps/s,
     // [2] field multiplies/s (all at full occupancy); [3] ns per dependent
     // field multiply in a single lane (one wave per SIMD)
-    int32_t probe_rates(double* out4);
+    int32_t probe_rates(double* out6);
     // one launch of a plain copy kernel over `bytes` (HBM counter calibration)
     int32_t probe_copy(uint64_t bytes, double* ms_out);
 

@@ -1,0 +1,125 @@
+// BN254 base field Fp on gfx950 in nine 29-bit limbs (radix 2^29), Montgomery
+// form with R = 2^261.
+//
+// Why 29 bits: a column of the FIPS Montgomery product sums at most 9 limb
+// products of a*b and 9 of m*p, each < 2^58, so the column stays far below
+// 2^64 and every limb product is a single v_mad_u64_u32 into a plain 64-bit
+// accumulator -- no carry-out instruction per product as with 32-bit limbs.
+// 162 multiply-adds per product instead of 136 multiply-adds + 136 add-with-
+// carry; with 29-bit limbs up to 2^30 (two normalised values added limb-wise)
+// the column bound is 9*2^60 + 9*2^58 + 2^36 < 2^64.
+//
+// Values are "lazy": a product of inputs below 8p is below 1.5p (R > 2^7 p),
+// so there is no final subtraction; each caller states the bound of what it
+// holds.  Limbs are normalised (< 2^29) except where noted.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "field.hpp"
+
+namespace zk {
+
+struct Fq {
+    uint32_t v[9];
+};
+
+#define ZQ_MASK 0x1fffffffu
+#define ZQ_PINV 0x04866389u  // -p^-1 mod 2^29
+
+__device__ __forceinline__ uint32_t q_p(int i) {
+    switch (i) {
+        case 0: return 0x187cfd47u;
+        case 1: return 0x010460b6u;
+        case 2: return 0x1c72a34fu;
+        case 3: return 0x02d522d0u;
+        case 4: return 0x1585d978u;
+        case 5: return 0x02db40c0u;
+        case 6: return 0x00a6e141u;
+        case 7: return 0x0e5c2634u;
+        default: return 0x0030644eu;
+    }
+}
+
+// column accumulation: acc += x * y (one v_mad_u64_u32)
+__device__ __forceinline__ void qmac(uint64_t& acc, uint32_t x, uint32_t y) {
+    acc += (uint64_t)x * y;
+}
+
+// a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
+// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
+__device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
+    uint32_t m[9];
+    Fq t;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+#pragma unroll
+        for (int j = 0; j < k; j++) {
+            qmac(acc, a.v[j], b.v[k - j]);
+            qmac(acc, m[j], q_p(k - j));
+        }
+        qmac(acc, a.v[k], b.v[0]);
+        m[k] = ((uint32_t)acc * ZQ_PINV) & ZQ_MASK;
+        qmac(acc, m[k], q_p(0));
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; k++) {
+#pragma unroll
+        for (int j = k - 8; j < 9; j++) {
+            qmac(acc, a.v[j], b.v[k - j]);
+            qmac(acc, m[j], q_p(k - j));
+        }
+        t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
+        acc >>= 29;
+    }
+    t.v[8] = (uint32_t)acc;
+    return t;
+}
+
+// 256-bit integer (8 x 32) <-> 9 x 29 bit slices of the same integer
+__device__ __forceinline__ Fq fq_slice(const Fp& a) {
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int bit = 29 * i, w = bit >> 5, s = bit & 31;
+        uint32_t lo = a.v[w] >> s;
+        if (s > 3 && w + 1 < 8) lo |= a.v[w + 1] << (32 - s);
+        r.v[i] = lo & ZQ_MASK;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Fp fq_pack(const Fq& a) {
+    Fp r;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        const int bit = 32 * w, i = bit / 29, s = bit % 29;
+        uint32_t x = a.v[i] >> s;
+        if (i + 1 < 9) x |= a.v[i + 1] << (29 - s);
+        if (s > 26 && i + 2 < 9) x |= a.v[i + 2] << (58 - s);
+        r.v[w] = x;
+    }
+    return r;
+}
+
+// x * 2^256 mod p (the 32-bit-limb Montgomery form) -> x * 2^261 mod p
+__device__ __forceinline__ Fq fq_from_fp(const Fp& a) {
+    Fq k;  // 2^5 * 2^261 mod p
+    k.v[0] = 0x13349ca1u; k.v[1] = 0x1a5d84a8u; k.v[2] = 0x0a3e5cacu; k.v[3] = 0x100249e0u;
+    k.v[4] = 0x12b951e8u; k.v[5] = 0x0e92d304u; k.v[6] = 0x14cb95b3u; k.v[7] = 0x041b9d3du;
+    k.v[8] = 0x00058003u;
+    return fq_mul(fq_slice(a), k);
+}
+
+// value < 2^255 with normalised limbs -> [0, p) in the 32-bit form
+__device__ __forceinline__ Fp fq_to_fp(const Fq& a) {
+    Fq k;  // 2^256 mod p
+    k.v[0] = 0x058f0d9du; k.v[1] = 0x1aea1c6eu; k.v[2] = 0x11c2cf74u; k.v[3] = 0x11d651ebu;
+    k.v[4] = 0x1462c0a7u; k.v[5] = 0x11b7bc3cu; k.v[6] = 0x1cbd99bau; k.v[7] = 0x183340fbu;
+    k.v[8] = 0x000e0a77u;
+    return fp_reduce_once(fq_pack(fq_mul(a, k)));  // product < 1.5p
+}
+
+}  // namespace zk

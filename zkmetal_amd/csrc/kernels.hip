@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "curve.hpp"
+#include "field29.hpp"
 #include "glv.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
@@ -468,10 +469,12 @@ __global__ __launch_bounds__(256) void k_accumulate(
 // ------------------------------------------------------ partial merging
 // Records with equal key are contiguous (a bucket's partials come from
 // consecutive lanes; padding of empty lanes carries key 0x7fffffff).
-// k_merge_runs: the lane holding the first record of a run of at most `limit`
-// records sums it and writes dest[key]; a longer run is appended to
-// long_list.  k_merge_long: one wave per listed run, lanes striding over its
-// records, then a shuffle tree: depth len/64 + 6 additions.
+// k_merge_runs: the lane holding the first record of a run with at most
+// `limit` payload records (and at most 3*limit records in all) sums it and
+// writes dest[key]; a longer run is appended to long_list.  k_merge_long: one
+// 256-lane workgroup per listed run, lanes striding over its records, a
+// shuffle tree per wave and a two-level tree over the four wave partials:
+// depth len/256 + 8 additions.
 
 __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
@@ -488,8 +491,14 @@ __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__
         if (k == ZK_REC_KEYMASK) continue;  // padding
         if (i > 0 && (keys[i - 1] & ZK_REC_KEYMASK) == k) continue;
         uint64_t j = i;
-        while (j < nrec && j - i < limit && (keys[j] & ZK_REC_KEYMASK) == k) j++;
-        if (j < nrec && (keys[j] & ZK_REC_KEYMASK) == k) {
+        uint32_t valid = 0;
+        while (j < nrec && valid <= limit && j - i <= 3 * limit) {
+            const uint32_t kj = keys[j];
+            if ((kj & ZK_REC_KEYMASK) != k) break;
+            valid += (kj & ZK_REC_INVALID) ? 0u : 1u;
+            j++;
+        }
+        if (valid > limit || (j - i > 3 * limit && j < nrec && (keys[j] & ZK_REC_KEYMASK) == k)) {
             const uint32_t slot = atomicAdd(long_count, 1u);
             if (slot < long_cap) long_list[slot] = (uint32_t)i;
             continue;
@@ -513,26 +522,45 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ long_count,
                                                     const uint32_t* __restrict__ long_list,
                                                     uint32_t long_cap, uint4* __restrict__ dest) {
-    const uint32_t run = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint4 part[4][8];
+    __shared__ uint32_t part_has[4];
     const uint32_t n = min(*long_count, long_cap);
-    if (run >= n) return;
-    const uint32_t lane = __lane_id();
-    const uint64_t start = long_list[run];
-    const uint32_t k = keys[start] & ZK_REC_KEYMASK;
-    bool has = false;
-    Xyzz acc = xyzz_identity();
-    for (uint64_t p = start + lane; p < nrec; p += 64) {
-        const uint32_t kr = keys[p];
-        if ((kr & ZK_REC_KEYMASK) != k) break;
-        if (!(kr & ZK_REC_INVALID)) {
-            Xyzz q = load_xyzz(pts, p);
-            if (has) xyzz_add(acc, q);
-            else acc = q;
-            has = true;
+    const uint32_t t = threadIdx.x, wv = t >> 6, lane = __lane_id();
+    for (uint32_t run = blockIdx.x; run < n; run += gridDim.x) {  // uniform per workgroup
+        const uint64_t start = long_list[run];
+        const uint32_t k = keys[start] & ZK_REC_KEYMASK;
+        bool has = false;
+        Xyzz acc = xyzz_identity();
+        for (uint64_t p = start + t; p < nrec; p += 256) {
+            const uint32_t kr = keys[p];
+            if ((kr & ZK_REC_KEYMASK) != k) break;
+            if (!(kr & ZK_REC_INVALID)) {
+                Xyzz q = load_xyzz(pts, p);
+                if (has) xyzz_add(acc, q);
+                else acc = q;
+                has = true;
+            }
         }
+        acc = group_sum_xyzz(acc, 64);  // identity lanes add nothing
+        const bool wave_has = __ballot(has) != 0;
+        if (lane == 0) {
+            store_xyzz(&part[wv][0], 0, acc);
+            part_has[wv] = wave_has ? 1u : 0u;
+        }
+        __syncthreads();
+        if (lane == 0 && (wv & 1) == 0) {
+            Xyzz o = load_xyzz(&part[wv + 1][0], 0);
+            xyzz_add(acc, o);
+            store_xyzz(&part[wv][0], 0, acc);
+        }
+        __syncthreads();
+        if (t == 0) {
+            Xyzz o = load_xyzz(&part[2][0], 0);
+            xyzz_add(acc, o);
+            if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_xyzz(dest, k, acc);
+        }
+        __syncthreads();  // LDS partials are reused by the next run
     }
-    acc = group_sum_xyzz(acc, 64);  // identity lanes add nothing
-    if (__ballot(has) && lane == 0) store_xyzz(dest, k, acc);
 }
 
 // ------------------------------------------------------ bucket reduction
@@ -633,7 +661,7 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
         else xyzz_add(acc, q);
         first = false;
     }
-    acc = group_sum_xyzz(acc, 64);
+    acc = group_sum_xyzz(acc, count >= 64 ? 64u : count);  // count is a power of 2
     if (lane == 0) store_xyzz(bits, item, acc);
 }
 
@@ -686,6 +714,7 @@ __global__ __launch_bounds__(64) void k_test_fp(int op, const uint4* __restrict_
         case 2: r = fp_sub(x, y); break;
         case 3: r = fp_inv(x); break;
         case 4: r = fp_neg(x); break;
+        case 6: r = fq_to_fp(fq_mul(fq_from_fp(x), fq_from_fp(y))); break;
         default: r = fp_canon(x); break;
     }
     store_fp(out + 2 * i, r);
@@ -730,6 +759,39 @@ __global__ __launch_bounds__(64) void k_test_glv(const uint4* __restrict__ scala
 
 __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_t* __restrict__ out) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 5) {  // 29-bit limbs, latency
+        Fq a, b;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            b.v[i] = (t * 2654435761u + i) & ZQ_MASK;
+            a.v[i] = (t * 40503u + i * 7u) & ZQ_MASK;
+        }
+        for (uint32_t it = 0; it < iters; it++) a = fq_mul(a, b);
+        uint64_t x = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) x ^= a.v[i];
+        out[t] = x;
+        return;
+    }
+    if (mode == 4) {  // 29-bit limbs, throughput
+        Fq a[4], b;
+#pragma unroll
+        for (int i = 0; i < 9; i++) b.v[i] = (t * 2654435761u + i) & ZQ_MASK;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) a[k].v[i] = ((t + k) * 40503u + i * 7u) & ZQ_MASK;
+        for (uint32_t it = 0; it < iters; it++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] = fq_mul(a[k], b);
+        uint64_t x = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) x ^= a[k].v[i];
+        out[t] = x;
+        return;
+    }
     if (mode == 3) {  // latency: one dependent chain per lane
         Fp a, b;
 #pragma unroll
@@ -866,7 +928,7 @@ void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uin
                        (const uint4*)pts, nrec, limit, (uint4*)dest, long_count, long_list,
                        long_cap);
     if (!long_cap) return;
-    hipLaunchKernelGGL(k_merge_long, dim3(blocks_for(long_cap, 4)), dim3(256), 0, st, keys,
+    hipLaunchKernelGGL(k_merge_long, dim3(std::min<uint32_t>(long_cap, 2048)), dim3(256), 0, st, keys,
                        (const uint4*)pts, nrec, (const uint32_t*)long_count,
                        (const uint32_t*)long_list, long_cap, (uint4*)dest);
 }
