@@ -190,6 +190,9 @@ def batched(args, eng, peak):
     dsc.upload(scal)
     ms_b, _ = timed_steps(lambda: eng.msm_batch_resident(dsc, counts), args.steps, args.warmup)
     batched_res = eng.msm_batch_resident(dsc, counts)
+    tb = eng.last_timing()
+    stage = {k: round(tb[k], 4) for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms",
+                                           "reduce_ms", "copyout_ms", "device_ms", "host_tail_ms")}
     # back-to-back plain launches of the same MSMs on the same device
     offs = np.concatenate([[0], np.cumsum(counts)])
     views = []
@@ -223,7 +226,8 @@ def batched(args, eng, peak):
         "config": {"workload": f"{len(counts)} MSMs over a resident 2^20 table, sizes {min(counts)}..{max(counts)}, "
                                f"scalars {'uniform Fr' if dist == 0 else 'witness-like mix'}",
                    "msms": len(counts), "points": int(sum(counts))},
-        "extra": {"back_to_back_ms": round(ms_s, 3),
+        "extra": {"stage_ms": stage, "entries_per_lane": tb["entries_per_lane"],
+                  "back_to_back_ms": round(ms_s, 3),
                   "back_to_back_msms_per_s": round(len(counts) / (ms_s * 1e-3), 2),
                   "speedup_vs_back_to_back": round(ms_s / ms_b, 3),
                   "results_identical": bool(same)},

@@ -77,10 +77,24 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
 /* Per-size-class window widths: c_by_log2n[ceil(log2 n)] for 0 <= index < len
- * (<= 33 entries); 0 keeps the cost-model default for that class.  Tables
- * come from an autotuning sweep (zkmetal_amd.tune) and apply to all calls. */
+ * (<= 33 entries); 0 keeps the cost-model default for that class.  The plain
+ * table applies to calls with one MSM, the batch table to each MSM of a call
+ * with several (falling back to the plain table where it holds 0).  Tables
+ * come from the autotuning sweep in zkmetal_amd/tune.py; a request server
+ * copies both tables from the engine it is created on. */
 ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
                                      uint32_t len);
+ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
+                                           uint32_t len);
+
+/* Graph mode (off by default): the device launch sequence of a call (all of
+ * its MSMs: decomposition, sort, accumulation, merge, reduction, copy-out) is
+ * captured once into a hipGraph and replayed by later calls of the same
+ * shape (MSM sizes, window widths, table and scalar addresses); stage times
+ * in zkmsm_last_timing are then 0 except the device total.  A call of a new
+ * shape captures again.  zkmsm_graph_captures counts the captures. */
+ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
+uint64_t zkmsm_graph_captures(ZkMetalMSMEngine engine);
 
 /* ---- request server ------------------------------------------------------ */
 /* Continuous batching of MSM requests over the resident table of an engine

@@ -1,0 +1,278 @@
+"""CPU model of the 29-bit-limb field and XYZZ arithmetic used by the MSM
+kernels (zkmetal_amd/csrc/field29.hpp, curve29.hpp), limb for limb.
+
+What it pins: the design invariants the GPU cannot report -- no column of the
+Montgomery product exceeds 64 bits, no signed limb step leaves int32, and
+every value stays below its stated bound (products < 1.5p, stored X, Y < 4p,
+ZZ, ZZZ < 2p) -- over random and extreme inputs, and the point results
+against the oracle's affine group law.  The GPU parity tests check the
+kernels themselves."""
+import random
+
+from oracle import bn254 as B
+
+P = B.P
+MASK = (1 << 29) - 1
+R = 1 << 261
+PINV = (-pow(P, -1, 1 << 29)) % (1 << 29)
+U64 = 1 << 64
+
+
+def limbs(x):
+    return [(x >> (29 * i)) & MASK for i in range(9)]
+
+
+def val(a):
+    return sum(v << (29 * i) for i, v in enumerate(a))
+
+
+PL = limbs(P)
+
+
+def kp(k):
+    return limbs(k * P)
+
+
+def fq_mul(a, b):
+    assert all(v < (1 << 30) for v in a + b)
+    assert val(a) * val(b) < P * R // 2
+    m = [0] * 9
+    t = [0] * 9
+    acc = 0
+
+    def mac(x, y):
+        nonlocal acc
+        acc += x * y
+        assert acc < U64, "column overflow"
+
+    for k in range(9):
+        for j in range(k):
+            mac(a[j], b[k - j])
+            mac(m[j], PL[k - j])
+        mac(a[k], b[0])
+        m[k] = ((acc & 0xFFFFFFFF) * PINV) & MASK
+        mac(m[k], PL[0])
+        assert acc & MASK == 0
+        acc >>= 29
+    for k in range(9, 17):
+        for j in range(k - 8, 9):
+            mac(a[j], b[k - j])
+            mac(m[j], PL[k - j])
+        t[k - 9] = acc & MASK
+        acc >>= 29
+    t[8] = acc
+    assert t[8] < (1 << 32)
+    r = val(t)
+    assert r % P == val(a) * val(b) * pow(R, -1, P) % P
+    assert r < val(a) * val(b) // R + P + 1
+    return t
+
+
+def fq_add(a, b):
+    r, c = [], 0
+    for i in range(9):
+        s = a[i] + b[i] + c
+        assert s < (1 << 32)
+        r.append(s & MASK)
+        c = s >> 29
+    assert c == 0
+    return r
+
+
+def fq_sub(k, a, b):
+    assert val(b) < k * P
+    K = kp(k)
+    r, c = [], 0
+    for i in range(9):
+        s = (a[i] + K[i]) - b[i] + c
+        assert -(1 << 31) <= s < (1 << 31) and a[i] + K[i] < (1 << 32)
+        r.append(s & MASK)
+        c = s >> 29
+    assert c == 0
+    assert val(r) == val(a) - val(b) + k * P
+    return r
+
+
+def fq_csub(k, a):
+    K = kp(k)
+    d, c = [], 0
+    for i in range(9):
+        s = a[i] - K[i] + c
+        d.append(s & MASK)
+        c = s >> 29
+    r = a if c < 0 else d
+    assert val(r) == (val(a) if val(a) < k * P else val(a) - k * P)
+    return r
+
+
+def iz2(a):
+    assert val(a) < 2 * P
+    return val(a) in (0, P)
+
+
+ONE = limbs(R % P)
+
+
+def lt(a, bound):
+    assert val(a) < bound * P, (val(a) / P, bound)
+
+
+def dbl_core(X, Y):
+    U = fq_add(Y, Y)
+    lt(U, 8)
+    V = fq_mul(U, U)
+    W = fq_mul(U, V)
+    S = fq_mul(X, V)
+    xx = fq_mul(X, X)
+    M = fq_add(fq_add(xx, xx), xx)
+    lt(M, 4.5)
+    X3 = fq_csub(2, fq_sub(3, fq_mul(M, M), fq_add(S, S)))
+    lt(X3, 2.5)
+    Y3 = fq_sub(2, fq_mul(M, fq_sub(3, S, X3)), fq_mul(W, Y))
+    lt(Y3, 3.5)
+    for v in (V, W):
+        lt(v, 1.5)
+    return X3, Y3, V, W
+
+
+def madd(acc, x2, y2, q_id):
+    X1, Y1, ZZ1, ZZZ1 = acc
+    lt(X1, 4), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
+    p_id = iz2(ZZ1)
+    U2 = fq_mul(x2, ZZ1)
+    S2 = fq_mul(y2, ZZZ1)
+    Pp = fq_sub(4, U2, X1)
+    Rr = fq_sub(4, S2, Y1)
+    PP = fq_mul(Pp, Pp)
+    PPP = fq_mul(Pp, PP)
+    Q = fq_mul(X1, PP)
+    RR = fq_mul(Rr, Rr)
+    dbl = iz2(PP) and iz2(RR) and not p_id and not q_id
+    X3 = fq_csub(3, fq_sub(5, RR, fq_add(fq_add(Q, Q), PPP)))
+    lt(X3, 3.5)
+    Y3 = fq_sub(2, fq_mul(Rr, fq_sub(4, Q, X3)), fq_mul(Y1, PPP))
+    r = (X3, Y3, fq_mul(ZZ1, PP), fq_mul(ZZZ1, PPP))
+    if dbl:
+        r = dbl_core(x2, y2)
+    if p_id:
+        r = (x2, y2, ONE, ONE)
+    if q_id:
+        r = acc
+    return r
+
+
+def add(acc, q):
+    X1, Y1, ZZ1, ZZZ1 = acc
+    X2, Y2, ZZ2, ZZZ2 = q
+    p_id, q_id = iz2(ZZ1), iz2(ZZ2)
+    U1 = fq_mul(X1, ZZ2)
+    U2 = fq_mul(X2, ZZ1)
+    S1 = fq_mul(Y1, ZZZ2)
+    S2 = fq_mul(Y2, ZZZ1)
+    Pp = fq_sub(2, U2, U1)
+    Rr = fq_sub(2, S2, S1)
+    PP = fq_mul(Pp, Pp)
+    PPP = fq_mul(Pp, PP)
+    Q = fq_mul(U1, PP)
+    RR = fq_mul(Rr, Rr)
+    dbl = iz2(PP) and iz2(RR) and not p_id and not q_id
+    X3 = fq_csub(3, fq_sub(5, RR, fq_add(fq_add(Q, Q), PPP)))
+    Y3 = fq_sub(2, fq_mul(Rr, fq_sub(4, Q, X3)), fq_mul(S1, PPP))
+    r = (X3, Y3, fq_mul(fq_mul(ZZ1, ZZ2), PP), fq_mul(fq_mul(ZZZ1, ZZZ2), PPP))
+    if dbl:
+        X3, Y3, V, W = dbl_core(X1, Y1)
+        r = (X3, Y3, fq_mul(V, ZZ1), fq_mul(W, ZZZ1))
+    if p_id:
+        r = q
+    if q_id:
+        r = acc
+    return r
+
+
+def to_int(x):  # field element -> internal limbs of x * R mod p, canonical
+    return limbs(x * R % P)
+
+
+def from_int(a):
+    return val(a) * pow(R, -1, P) % P
+
+
+def affine(pt):
+    X, Y, ZZ, ZZZ = (from_int(c) for c in pt)
+    if ZZ == 0:
+        return None
+    return (X * pow(ZZ, -1, P) % P, Y * pow(ZZZ, -1, P) % P)
+
+
+def worst(k):
+    """a value just below k*p with normalised limbs"""
+    return limbs(k * P - 1)
+
+
+def test_mul_extremes():
+    for a in (worst(8), worst(5), limbs(0), limbs(1), worst(1), worst(2)):
+        for b in (worst(8), worst(4), limbs(P - 1), limbs(1)):
+            if val(a) * val(b) < P * R // 2:
+                fq_mul(a, b)
+    # all-ones limbs (limbs < 2^30 as after a carry-free add) at a safe value
+    a = [MASK] * 8 + [0]
+    fq_mul(a, worst(8))
+
+
+def test_group_law_and_bounds():
+    rnd = random.Random(7)
+    pts = [B.mul(B.G, rnd.randrange(1, B.R_ORDER)) for _ in range(40)]
+    acc = (limbs(0), limbs(0), limbs(0), limbs(0))
+    ref = None
+    for i, pt in enumerate(pts):
+        q = pt if i % 5 else B.neg(pt)
+        acc = madd(acc, to_int(q[0]), to_int(q[1]), False)
+        ref = B.add(ref, q)
+        assert affine(acc) == ref
+    # doubling and cancellation through madd
+    a1 = madd((limbs(0),) * 4, to_int(pts[0][0]), to_int(pts[0][1]), False)
+    a2 = madd(a1, to_int(pts[0][0]), to_int(pts[0][1]), False)
+    assert affine(a2) == B.add(pts[0], pts[0])
+    ny = to_int(B.neg(pts[0])[1])
+    a3 = madd(a1, to_int(pts[0][0]), ny, False)
+    assert affine(a3) is None
+    a4 = madd(a3, to_int(pts[1][0]), to_int(pts[1][1]), False)
+    assert affine(a4) == pts[1]
+    # XYZZ adds: random projective factors, doubling, cancellation, identity
+    def proj(pt, z):
+        zz, zzz = z * z % P, z * z * z % P
+        return (to_int(pt[0] * zz % P), to_int(pt[1] * zzz % P), to_int(zz), to_int(zzz))
+
+    for i in range(len(pts) - 1):
+        s = add(proj(pts[i], rnd.randrange(1, P)), proj(pts[i + 1], rnd.randrange(1, P)))
+        assert affine(s) == B.add(pts[i], pts[i + 1])
+        s2 = add(s, s)
+        assert affine(s2) == B.add(affine(s), affine(s))
+    x = proj(pts[2], 5)
+    assert affine(add(x, proj(pts[2], 9))) == B.add(pts[2], pts[2])
+    assert affine(add(x, proj(B.neg(pts[2]), 9))) is None
+    assert affine(add((limbs(0),) * 4, x)) == pts[2]
+
+
+def test_bounds_at_extremes():
+    """Accumulators at the top of their stated bounds still give the right
+    group element (the formulas only see residues)."""
+    rnd = random.Random(11)
+    for _ in range(30):
+        pt = B.mul(B.G, rnd.randrange(1, B.R_ORDER))
+        q = B.mul(B.G, rnd.randrange(1, B.R_ORDER))
+        z = rnd.randrange(1, P)
+        zz, zzz = z * z % P, z * z * z % P
+        X, Y = pt[0] * zz % P, pt[1] * zzz % P
+        # lift each coordinate to the largest representative under its bound
+        def lift(v, bound):
+            r = v * R % P
+            while r + P < bound * P:
+                r += P
+            return limbs(r)
+
+        acc = (lift(X, 4), lift(Y, 4), lift(zz, 1.5), lift(zzz, 1.5))
+        out = madd(acc, to_int(q[0]), to_int(q[1]), False)
+        assert affine(out) == B.add(pt, q)
+        out2 = add(acc, acc)
+        assert affine(out2) == B.add(pt, pt)

@@ -86,7 +86,10 @@ def _xyzz_affine(buf, i):
     return (X * pow(ZZ, -1, B.P) % B.P, Y * pow(ZZZ, -1, B.P) % B.P)
 
 
-def test_xyzz_add_dbl_madd(engine):
+@pytest.mark.parametrize("base", [0, 3], ids=["limb29", "limb32"])
+def test_xyzz_add_dbl_madd(engine, base):
+    """base 0: the 29-bit-limb arithmetic of the MSM kernels; base 3: the
+    32-bit-limb arithmetic of the base generator."""
     rnd = random.Random(5)
     pts = [B.mul(B.G, rnd.randrange(1, B.R_ORDER)) for _ in range(24)]
     cases = []
@@ -96,11 +99,11 @@ def test_xyzz_add_dbl_madd(engine):
     cases += [(P, P), (P, B.neg(P)), (P, None), (None, P), (None, None), (B.G, B.G)]
     Pb = b"".join(_xyzz_bytes(p, rnd.randrange(1, B.P)) for p, _ in cases)
     Qb = b"".join(_xyzz_bytes(q, rnd.randrange(1, B.P)) for _, q in cases)
-    add = engine.test_xyzz(0, Pb, Qb)
-    dbl = engine.test_xyzz(1, Pb, Qb)
+    add = engine.test_xyzz(base + 0, Pb, Qb)
+    dbl = engine.test_xyzz(base + 1, Pb, Qb)
     # madd uses q's X, Y as an affine point: pass Z = 1 and (0,0) for None
     Qa = b"".join(_xyzz_bytes(q, 1) if q is not None else bytes(128) for _, q in cases)
-    madd = engine.test_xyzz(2, Pb, Qa)
+    madd = engine.test_xyzz(base + 2, Pb, Qa)
     for i, (p, q) in enumerate(cases):
         assert _xyzz_affine(add, i) == B.add(p, q), i
         assert _xyzz_affine(dbl, i) == B.add(p, p), i

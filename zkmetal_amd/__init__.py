@@ -95,6 +95,9 @@ def lib():
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_set_window_table": (i32, [vp, vp, u32]),
+        "zkmsm_set_batch_window_table": (i32, [vp, vp, u32]),
+        "zkmsm_set_graph_mode": (i32, [vp, i32]),
+        "zkmsm_graph_captures": (u64, [vp]),
         "zkmsm_server_create": (i32, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
         "zkmsm_server_submit": (i32, [vp, vp, u32, ctypes.POINTER(u64)]),
         "zkmsm_server_submit_ref": (i32, [vp, vp, u32, ctypes.POINTER(u64)]),
@@ -286,11 +289,20 @@ class MetalMSM:
         _check(rc, "msm_batch_resident")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
 
-    def set_window_table(self, table):
-        """table[ceil(log2 n)] = window width for that size class (0 = default)."""
+    def set_window_table(self, table, batch=False):
+        """table[ceil(log2 n)] = window width for that size class (0 = default);
+        batch=True sets the table used for MSMs inside multi-MSM calls."""
         arr = np.ascontiguousarray(np.asarray(table, dtype=np.uint32))
-        _check(lib().zkmsm_set_window_table(self.handle, ctypes.c_void_p(arr.ctypes.data), len(arr)),
-               "set_window_table")
+        fn = lib().zkmsm_set_batch_window_table if batch else lib().zkmsm_set_window_table
+        _check(fn(self.handle, ctypes.c_void_p(arr.ctypes.data), len(arr)), "set_window_table")
+
+    def set_graph_mode(self, on):
+        """Capture each call's device launch sequence in a hipGraph and replay
+        it for later calls of the same shape."""
+        _check(lib().zkmsm_set_graph_mode(self.handle, 1 if on else 0), "set_graph_mode")
+
+    def graph_captures(self):
+        return int(lib().zkmsm_graph_captures(self.handle))
 
     def set_window_bits(self, c):
         _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")

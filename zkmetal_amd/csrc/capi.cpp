@@ -327,7 +327,30 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
-    return e->set_window_table(c_by_log2n, len);
+    return e->set_window_table(c_by_log2n, len, false);
+}
+
+ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    e->set_graph_mode(on != 0);
+    return ZKMETAL_SUCCESS;
+}
+
+uint64_t zkmsm_graph_captures(ZkMetalMSMEngine engine) {
+    Engine* e = as_engine(engine);
+    if (!e) return 0;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->graph_captures();
+}
+
+ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
+                                           uint32_t len) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->set_window_table(c_by_log2n, len, true);
 }
 
 // ---------------------------------------------------------------- server
@@ -342,16 +365,12 @@ ZkMetalStatus zkmsm_server_create(ZkMetalMSMEngine srs_engine, const ZkmsmServer
         if (cfg->max_batch) sc.max_batch = cfg->max_batch;
         if (cfg->hbm_budget_bytes) sc.hbm_budget = cfg->hbm_budget_bytes;
     }
-    const void* table;
-    uint32_t n;
-    {
-        std::lock_guard<std::mutex> lk(e->mutex());
-        table = e->srs_table();
-        n = e->srs_size();
-    }
+    std::lock_guard<std::mutex> lk(e->mutex());
+    const void* table = e->srs_table();
+    const uint32_t n = e->srs_size();
     if (!table || n == 0) return ZKMETAL_ERR_INVALID_INPUT;
     zk::Server* s = nullptr;
-    int32_t rc = zk::Server::create(table, n, sc, &s);
+    int32_t rc = zk::Server::create(table, n, sc, &s, e);  // slots take e's window tables
     *out = s;
     return rc;
 }

@@ -1,0 +1,290 @@
+// BN254 G1 XYZZ arithmetic over the 29-bit-limb field (field29.hpp): the
+// same formulas and exception rules as curve.hpp (madd-2008-s, add-2008-s,
+// dbl-2008-s-1; (0,0) affine is the identity, P + P doubles, P + (-P) is the
+// identity), with lazy reduction.  Bounds, in multiples of p:
+//   products < 1.5p (inputs < 8p);  stored X, Y < 4p;  stored ZZ, ZZZ < 2p;
+//   affine table coordinates < p (canonical).
+// A stored value is < 4p < 2^256, so a point is kept in global memory as
+// 256-bit integers (8 x 32-bit words), the same 64 B / 128 B layouts as
+// curve.hpp, and re-sliced into 29-bit limbs on load.
+#pragma once
+#include "curve.hpp"
+#include "field29.hpp"
+
+namespace zk {
+
+struct QAff {
+    Fq x, y;
+};
+
+struct QXyzz {
+    Fq X, Y, ZZ, ZZZ;
+};
+
+// limb i of K*p (folded at compile time inside unrolled loops)
+template <int K>
+__device__ __forceinline__ uint32_t q_kp(int i) {
+    uint64_t c = 0;
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const uint64_t s = (uint64_t)K * q_p(j) + c;
+        if (j == i) r = (uint32_t)s & ZQ_MASK;
+        c = s >> 29;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Fq fq_zero() {
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.v[i] = 0;
+    return r;
+}
+
+// R mod p, R = 2^261
+__device__ __forceinline__ Fq fq_one() {
+    Fq r;
+    r.v[0] = 0x157ccc21u; r.v[1] = 0x141c2758u; r.v[2] = 0x185230d3u; r.v[3] = 0x014c0419u;
+    r.v[4] = 0x0aa36fb9u; r.v[5] = 0x1d4240ceu; r.v[6] = 0x11d54c07u; r.v[7] = 0x052ac7a8u;
+    r.v[8] = 0x000dc836u;
+    return r;
+}
+
+// beta * R mod p (phi(x, y) = (beta x, y))
+__device__ __forceinline__ Fq fq_beta() {
+    Fq r;
+    r.v[0] = 0x0a337995u; r.v[1] = 0x158d1d23u; r.v[2] = 0x189c9b98u; r.v[3] = 0x12fa4e45u;
+    r.v[4] = 0x185faadcu; r.v[5] = 0x0176f16du; r.v[6] = 0x0eed93bau; r.v[7] = 0x14291140u;
+    r.v[8] = 0x000c0afeu;
+    return r;
+}
+
+// a + b, normalised limbs (no reduction: the bound is the sum of the bounds)
+__device__ __forceinline__ Fq fq_add(const Fq& a, const Fq& b) {
+    Fq r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint32_t s = a.v[i] + b.v[i] + c;
+        r.v[i] = s & ZQ_MASK;
+        c = s >> 29;
+    }
+    return r;
+}
+
+// a - b + K*p for b < K*p: in [a - b + (K p - b), a + K p)
+template <int K>
+__device__ __forceinline__ Fq fq_sub(const Fq& a, const Fq& b) {
+    Fq r;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int32_t s = (int32_t)(a.v[i] + q_kp<K>(i)) - (int32_t)b.v[i] + c;
+        r.v[i] = (uint32_t)s & ZQ_MASK;
+        c = s >> 29;  // arithmetic: -1, 0 or 1
+    }
+    return r;
+}
+
+// a >= K*p ? a - K*p : a
+template <int K>
+__device__ __forceinline__ Fq fq_csub(const Fq& a) {
+    Fq d;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int32_t s = (int32_t)a.v[i] - (int32_t)q_kp<K>(i) + c;
+        d.v[i] = (uint32_t)s & ZQ_MASK;
+        c = s >> 29;
+    }
+    const bool keep = c < 0;
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.v[i] = keep ? a.v[i] : d.v[i];
+    return r;
+}
+
+// a == 0 mod p for a < 2p
+__device__ __forceinline__ bool fq_is_zero2(const Fq& a) {
+    uint32_t z = 0, e = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        z |= a.v[i];
+        e |= a.v[i] ^ q_p(i);
+    }
+    return z == 0 || e == 0;
+}
+
+__device__ __forceinline__ Fq fq_select(bool c, const Fq& a, const Fq& b) {
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.v[i] = c ? a.v[i] : b.v[i];
+    return r;
+}
+
+__device__ __forceinline__ QXyzz qxyzz_identity() {
+    QXyzz r;
+    r.X = fq_zero();
+    r.Y = fq_zero();
+    r.ZZ = fq_zero();
+    r.ZZZ = fq_zero();
+    return r;
+}
+
+__device__ __forceinline__ bool qxyzz_is_id(const QXyzz& p) { return fq_is_zero2(p.ZZ); }
+
+__device__ __forceinline__ bool qaff_is_id(const QAff& q) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) x |= q.x.v[i] | q.y.v[i];
+    return x == 0;
+}
+
+__device__ __forceinline__ QXyzz qxyzz_select(bool c, const QXyzz& a, const QXyzz& b) {
+    QXyzz r;
+    r.X = fq_select(c, a.X, b.X);
+    r.Y = fq_select(c, a.Y, b.Y);
+    r.ZZ = fq_select(c, a.ZZ, b.ZZ);
+    r.ZZZ = fq_select(c, a.ZZZ, b.ZZZ);
+    return r;
+}
+
+// doubling core: X, Y < 4p.  Returns X3 < 2.5p, Y3 < 3.5p; V, W < 1.5p.
+__device__ __forceinline__ void qdbl_core(const Fq& X, const Fq& Y, Fq& X3, Fq& Y3, Fq& V, Fq& W) {
+    const Fq U = fq_add(Y, Y);  // < 8p
+    V = fq_mul(U, U);
+    W = fq_mul(U, V);
+    const Fq S = fq_mul(X, V);
+    const Fq xx = fq_mul(X, X);
+    const Fq M = fq_add(fq_add(xx, xx), xx);                    // < 4.5p
+    X3 = fq_csub<2>(fq_sub<3>(fq_mul(M, M), fq_add(S, S)));     // < 4.5p -> < 2.5p
+    Y3 = fq_sub<2>(fq_mul(M, fq_sub<3>(S, X3)), fq_mul(W, Y));  // < 3.5p
+}
+
+// 2*(x, y) for an affine, non-identity point
+__device__ __forceinline__ QXyzz qxyzz_dbl_aff(const Fq& x, const Fq& y) {
+    QXyzz r;
+    qdbl_core(x, y, r.X, r.Y, r.ZZ, r.ZZZ);
+    return r;
+}
+
+// 2*P; identity stays identity (ZZ3 = V*ZZ1 = 0 mod p)
+__device__ __forceinline__ QXyzz qxyzz_dbl(const QXyzz& p) {
+    QXyzz r;
+    Fq V, W;
+    qdbl_core(p.X, p.Y, r.X, r.Y, V, W);
+    r.ZZ = fq_mul(V, p.ZZ);
+    r.ZZZ = fq_mul(W, p.ZZZ);
+    return r;
+}
+
+// acc += (x2, y2); x2, y2 < p.  q_id: the affine point is (0,0).
+__device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y2, bool q_id) {
+    const bool p_id = qxyzz_is_id(acc);
+    const Fq U2 = fq_mul(x2, acc.ZZ);
+    const Fq S2 = fq_mul(y2, acc.ZZZ);
+    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
+    const Fq R = fq_sub<4>(S2, acc.Y);
+    const Fq PP = fq_mul(P, P);
+    const Fq PPP = fq_mul(P, PP);
+    const Fq Q = fq_mul(acc.X, PP);
+    const Fq RR = fq_mul(R, R);
+    const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
+    QXyzz r;
+    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
+    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(acc.Y, PPP));  // < 3.5p
+    r.ZZ = fq_mul(acc.ZZ, PP);
+    r.ZZZ = fq_mul(acc.ZZZ, PPP);
+    if (__builtin_expect(dbl, 0)) r = qxyzz_dbl_aff(x2, y2);
+    // P == -Q falls out of the formulas: P = 0 gives ZZ3 = 0 mod p, the identity.
+    QXyzz q;
+    q.X = x2;
+    q.Y = y2;
+    q.ZZ = fq_one();
+    q.ZZZ = q.ZZ;
+    r = qxyzz_select(p_id, q, r);
+    acc = qxyzz_select(q_id, acc, r);
+}
+
+// acc += q (both XYZZ)
+__device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
+    const bool p_id = qxyzz_is_id(acc);
+    const bool q_id = qxyzz_is_id(q);
+    const Fq U1 = fq_mul(acc.X, q.ZZ);
+    const Fq U2 = fq_mul(q.X, acc.ZZ);
+    const Fq S1 = fq_mul(acc.Y, q.ZZZ);
+    const Fq S2 = fq_mul(q.Y, acc.ZZZ);
+    const Fq P = fq_sub<2>(U2, U1);  // < 3.5p
+    const Fq R = fq_sub<2>(S2, S1);
+    const Fq PP = fq_mul(P, P);
+    const Fq PPP = fq_mul(P, PP);
+    const Fq Q = fq_mul(U1, PP);
+    const Fq RR = fq_mul(R, R);
+    const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
+    QXyzz r;
+    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));
+    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(S1, PPP));
+    r.ZZ = fq_mul(fq_mul(acc.ZZ, q.ZZ), PP);
+    r.ZZZ = fq_mul(fq_mul(acc.ZZZ, q.ZZZ), PPP);
+    if (__builtin_expect(dbl, 0)) r = qxyzz_dbl(acc);
+    r = qxyzz_select(p_id, q, r);
+    acc = qxyzz_select(q_id, acc, r);
+}
+
+// ---- conversions between the two forms (canonical 32-bit form: [0, p),
+// R = 2^256; internal: R = 2^261)
+__device__ __forceinline__ Fq fq_from_fp_canon(const Fp& a) { return fq_csub<1>(fq_from_fp(a)); }
+
+__device__ __forceinline__ Xyzz qxyzz_to_xyzz(const QXyzz& a) {
+    Xyzz r;
+    r.X = fq_to_fp(a.X);
+    r.Y = fq_to_fp(a.Y);
+    r.ZZ = fq_to_fp(a.ZZ);
+    r.ZZZ = fq_to_fp(a.ZZZ);
+    return r;
+}
+
+__device__ __forceinline__ QXyzz qxyzz_from_xyzz(const Xyzz& a) {
+    QXyzz r;
+    r.X = fq_from_fp_canon(fp_canon(a.X));
+    r.Y = fq_from_fp_canon(fp_canon(a.Y));
+    r.ZZ = fq_from_fp_canon(fp_canon(a.ZZ));
+    r.ZZZ = fq_from_fp_canon(fp_canon(a.ZZZ));
+    return r;
+}
+
+// ---- global memory: values as 256-bit integers, as in curve.hpp
+__device__ __forceinline__ QAff load_qaff(const uint4* base, uint64_t idx) {
+    const uint4* p = base + idx * 4;
+    QAff a;
+    a.x = fq_slice(load_fp(p));
+    a.y = fq_slice(load_fp(p + 2));
+    return a;
+}
+
+__device__ __forceinline__ void store_qaff(uint4* base, uint64_t idx, const QAff& a) {
+    uint4* p = base + idx * 4;
+    store_fp(p, fq_pack(a.x));
+    store_fp(p + 2, fq_pack(a.y));
+}
+
+__device__ __forceinline__ QXyzz load_qxyzz(const uint4* base, uint64_t idx) {
+    const uint4* p = base + idx * 8;
+    QXyzz r;
+    r.X = fq_slice(load_fp(p));
+    r.Y = fq_slice(load_fp(p + 2));
+    r.ZZ = fq_slice(load_fp(p + 4));
+    r.ZZZ = fq_slice(load_fp(p + 6));
+    return r;
+}
+
+__device__ __forceinline__ void store_qxyzz(uint4* base, uint64_t idx, const QXyzz& a) {
+    uint4* p = base + idx * 8;
+    store_fp(p, fq_pack(a.X));
+    store_fp(p + 2, fq_pack(a.Y));
+    store_fp(p + 4, fq_pack(a.ZZ));
+    store_fp(p + 6, fq_pack(a.ZZZ));
+}
+
+}  // namespace zk

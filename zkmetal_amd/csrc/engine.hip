@@ -193,6 +193,7 @@ int32_t Engine::create(Engine** out) {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto& ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -272,39 +273,91 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     uint32_t* d_err = (uint32_t*)err_.p;
     hipStream_t st = stream_;
 
-    ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
-    launch_fill_zero(err_.p, 16, st);
-    ZK_TRY(hipEventRecord(ev_[0], st));
-    launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, pl.scalars, digits_.p, d_err, st);
-    ZK_TRY(hipEventRecord(ev_[1], st));
-    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
-                     nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
-                     (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
-    launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
-                     (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
-                     (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
-    launch_fill_zero(buckets_.p, pl.buckets * 128, st);
-    ZK_TRY(hipEventRecord(ev_[2], st));
-    launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
-                      (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
-                      buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
-    ZK_TRY(hipEventRecord(ev_[3], st));
-    // bucket partials that crossed lane boundaries: short runs by their first
-    // lane, long runs (heavy buckets) by one wave each
-    const uint32_t long_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 30);
-    launch_merge_runs((const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
-                      d_err + 1, (uint32_t*)recBk_.p, long_cap, st);
-    ZK_TRY(hipEventRecord(ev_[4], st));
-    launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, buckets_.p, rc_.p,
-                         winsum_.p, st);
-    ZK_TRY(hipEventRecord(ev_[5], st));
-    uint8_t* hw = (uint8_t*)hwin_.p;
     const uint64_t bits_bytes = (uint64_t)nseg * 16 * 128;
-    ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
-    ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 4, hipMemcpyDeviceToHost, st));
-    ZK_TRY(hipMemcpyAsync(hw + bits_bytes + 16, segvalid_.p, (uint64_t)nseg * 4,
-                          hipMemcpyDeviceToHost, st));
-    ZK_TRY(hipEventRecord(ev_[6], st));
+    uint8_t* hw = (uint8_t*)hwin_.p;
+    // the device part of the call: one launch sequence on the engine stream.
+    // Stage events are recorded only outside graph capture.
+    auto enqueue = [&](bool events) -> int32_t {
+        auto mark = [&](int i) -> hipError_t { return events ? hipEventRecord(ev_[i], st) : hipSuccess; };
+        ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
+        launch_fill_zero(err_.p, 16, st);
+        ZK_TRY(mark(0));
+        launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, pl.scalars, digits_.p, d_err, st);
+        ZK_TRY(mark(1));
+        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
+                         nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
+                         (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
+        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
+                         (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
+                         (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
+        launch_fill_zero(buckets_.p, pl.buckets * 128, st);
+        ZK_TRY(mark(2));
+        launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
+                          (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
+                          buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
+        ZK_TRY(mark(3));
+        // bucket partials that crossed lane boundaries: short runs by their first
+        // lane, longer runs (heavy buckets) by lane groups.  A listed run has more
+        // than kRunLimit records, so each list holds at most nrec / (kRunLimit+1).
+        const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
+        launch_merge_runs((const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
+                          d_err + 1, (uint32_t*)recBk_.p, list_cap, st);
+        ZK_TRY(mark(4));
+        launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, buckets_.p, rc_.p,
+                             winsum_.p, st);
+        ZK_TRY(mark(5));
+        ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
+        ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 4, hipMemcpyDeviceToHost, st));
+        ZK_TRY(hipMemcpyAsync(hw + bits_bytes + 16, segvalid_.p, (uint64_t)nseg * 4,
+                              hipMemcpyDeviceToHost, st));
+        return kOk;
+    };
+    if (graph_mode_) {
+        // replay the captured sequence when the call has the same shape and
+        // every buffer it touches is where it was at capture
+        std::vector<uint64_t> key;
+        key.reserve(pl.msms.size() * 3 + 24);
+        for (const MsmDesc& m : pl.msms) {
+            key.push_back(m.n);
+            key.push_back(m.c);
+            key.push_back(m.table0);
+        }
+        for (const void* p : std::initializer_list<const void*>{d_scalars, d_table, desc_.p, hdesc_.p, digits_.p, keysA_.p, valsA_.p,
+                              keysB_.p, valsB_.p, hist_.p, part_.p, segvalid_.p, buckets_.p,
+                              recAk_.p, recAp_.p, recBk_.p, rc_.p, winsum_.p, err_.p, hwin_.p})
+            key.push_back((uint64_t)(uintptr_t)p);
+        if (!graph_exec_ || key != graph_key_) {
+            if (graph_exec_) {
+                (void)hipGraphExecDestroy(graph_exec_);
+                graph_exec_ = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            ZK_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int32_t rc = enqueue(false);
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc != kOk || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                return kGpuError;
+            }
+            const hipError_t ei = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                graph_exec_ = nullptr;
+                (void)hipGetLastError();
+                return kGpuError;
+            }
+            graph_key_ = key;
+            graph_captures_++;
+        }
+        ZK_TRY(hipEventRecord(ev_[0], st));
+        ZK_TRY(hipGraphLaunch(graph_exec_, st));
+        ZK_TRY(hipEventRecord(ev_[kNumStages], st));
+    } else {
+        const int32_t rc = enqueue(true);
+        if (rc != kOk) return rc;
+        ZK_TRY(hipEventRecord(ev_[kNumStages], st));
+    }
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
     uint32_t err;
@@ -313,8 +366,8 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
 
     for (int s = 0; s < kNumStages; s++) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
-        timing_.stage_ms[s] = ms;
+        if (!graph_mode_) (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
+        timing_.stage_ms[s] = ms;  // stages are not split inside a graph
     }
     (void)hipEventElapsedTime(&timing_.device_ms, ev_[0], ev_[kNumStages]);
 
@@ -525,8 +578,11 @@ int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
 
 int32_t Engine::srs_download(uint8_t* points, uint32_t n) {
     if (!points || n > srs_n_) return kInvalidInput;
-    // table entries 2i are the bases themselves
-    ZK_TRY(hipMemcpy2DAsync(points, 64, srs_.p, 128, 64, n, hipMemcpyDeviceToHost, stream_));
+    if (n == 0) return kOk;
+    // table entries 2i are the bases, in the kernels' internal form
+    ZK_ALLOC(tmp_, (uint64_t)n * 64);
+    launch_table_export(srs_.p, n, tmp_.p, stream_);
+    ZK_TRY(hipMemcpyAsync(points, tmp_.p, (uint64_t)n * 64, hipMemcpyDeviceToHost, stream_));
     ZK_TRY(hipStreamSynchronize(stream_));
     return kOk;
 }
@@ -602,19 +658,23 @@ std::vector<uint32_t> Engine::window_bits_for(const std::vector<uint32_t>& ns) c
         if (!c) {
             uint32_t cls = 0;
             while ((1ull << cls) < ns[j]) cls++;  // ceil(log2 n)
-            c = ctab_[std::min<uint32_t>(cls, 32)];
+            cls = std::min<uint32_t>(cls, 32);
+            if (ns.size() > 1) c = ctab_batch_[cls];
+            if (!c) c = ctab_[cls];
         }
         cs[j] = c;  // 0 = cost-model default
     }
     return cs;
 }
 
-int32_t Engine::set_window_table(const uint32_t* c_by_log2n, uint32_t len) {
+int32_t Engine::set_window_table(const uint32_t* c_by_log2n, uint32_t len, bool batch) {
+    uint32_t t[33];
     for (uint32_t i = 0; i <= 32; i++) {
-        uint32_t c = (c_by_log2n && i < len) ? c_by_log2n[i] : 0;
+        const uint32_t c = (c_by_log2n && i < len) ? c_by_log2n[i] : 0;
         if (c != 0 && (c < 2 || c > 16)) return kInvalidInput;
-        ctab_[i] = c;
+        t[i] = c;
     }
+    memcpy(batch ? ctab_batch_ : ctab_, t, sizeof(t));
     return kOk;
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <mutex>
 #include <string>
@@ -116,10 +117,21 @@ class Engine {
     const void* srs_table() const { return srs_.p; }
 
     // per-size-class window widths: c_by_log2n[ceil(log2 n)], 0 = cost model
-    int32_t set_window_table(const uint32_t* c_by_log2n, uint32_t len);
+    // per-size-class window widths for single-MSM calls (batch = false) and
+    // for MSMs inside batched calls (batch = true); 0 = cost-model default
+    int32_t set_window_table(const uint32_t* c_by_log2n, uint32_t len, bool batch = false);
+    void copy_tuning_from(const Engine& o) {
+        memcpy(ctab_, o.ctab_, sizeof(ctab_));
+        memcpy(ctab_batch_, o.ctab_batch_, sizeof(ctab_batch_));
+        c_override_ = o.c_override_;
+    }
     std::vector<uint32_t> window_bits_for(const std::vector<uint32_t>& ns) const;
 
     void set_window_bits(uint32_t c) { c_override_ = c; }
+    // graph mode: the device launch sequence of a call is captured into a
+    // hipGraph and replayed while calls keep the same shape and buffers
+    void set_graph_mode(bool on) { graph_mode_ = on; }
+    uint64_t graph_captures() const { return graph_captures_; }
     const Timing& last_timing() const { return timing_; }
     int device() const { return dev_; }
     hipStream_t stream() const { return stream_; }
@@ -148,6 +160,11 @@ class Engine {
     std::mutex mu_;
     uint32_t c_override_ = 0;
     uint32_t ctab_[33] = {};
+    uint32_t ctab_batch_[33] = {};
+    bool graph_mode_ = false;
+    hipGraphExec_t graph_exec_ = nullptr;
+    std::vector<uint64_t> graph_key_;
+    uint64_t graph_captures_ = 0;
     Timing timing_{};
 
     DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;

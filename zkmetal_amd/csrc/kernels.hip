@@ -21,8 +21,7 @@
 
 #include <algorithm>
 
-#include "curve.hpp"
-#include "field29.hpp"
+#include "curve29.hpp"
 #include "glv.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
@@ -85,36 +84,60 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return off + inc - v;
 }
 
-__device__ __forceinline__ Fp shfl_down_fp(const Fp& a, uint32_t d) {
-    Fp r;
-#pragma unroll
-    for (int i = 0; i < 8; i++) r.v[i] = __shfl_down(a.v[i], d);
-    return r;
-}
-
-__device__ __forceinline__ Xyzz shfl_down_xyzz(const Xyzz& a, uint32_t d) {
-    Xyzz r;
-    r.X = shfl_down_fp(a.X, d);
-    r.Y = shfl_down_fp(a.Y, d);
-    r.ZZ = shfl_down_fp(a.ZZ, d);
-    r.ZZZ = shfl_down_fp(a.ZZZ, d);
-    return r;
-}
-
 // Sum over aligned groups of P lanes (P a power of two <= 64, uniform across
 // the wave); the group's first lane returns the group total.  Cross-lane data
 // moves with ds_bpermute shuffles, no LDS, no barriers.
-__device__ __forceinline__ Xyzz group_sum_xyzz(Xyzz v, uint32_t P) {
+__device__ __forceinline__ Fq shfl_down_fq(const Fq& a, uint32_t d) {
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.v[i] = __shfl_down(a.v[i], d);
+    return r;
+}
+
+__device__ __forceinline__ QXyzz shfl_down_qxyzz(const QXyzz& a, uint32_t d) {
+    QXyzz r;
+    r.X = shfl_down_fq(a.X, d);
+    r.Y = shfl_down_fq(a.Y, d);
+    r.ZZ = shfl_down_fq(a.ZZ, d);
+    r.ZZZ = shfl_down_fq(a.ZZZ, d);
+    return r;
+}
+
+__device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
     const uint32_t lane = __lane_id();
 #pragma unroll 1
     for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
-        Xyzz o = shfl_down_xyzz(v, s);
-        if ((lane & (P - 1)) < s) xyzz_add(v, o);
+        QXyzz o = shfl_down_qxyzz(v, s);
+        if ((lane & (P - 1)) < s) qxyzz_add(v, o);
     }
     return v;
 }
 
 // -------------------------------------------------------- base table / phi
+// table[2i] = P_i, table[2i+1] = phi(P_i) = (beta x, y): affine, coordinates
+// in the 2^261 Montgomery form, canonical, as 256-bit integers.  (0,0) maps
+// to (0,0).
+
+__device__ __forceinline__ void store_table_pair(uint4* table, uint64_t i, const Aff& a) {
+    QAff q;
+    q.x = fq_from_fp_canon(a.x);
+    q.y = fq_from_fp_canon(a.y);
+    store_qaff(table, 2 * i, q);
+    q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
+    store_qaff(table, 2 * i + 1, q);
+}
+
+// table entry 2i back to the 2^256 Montgomery form (64 B per point)
+__global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ table, uint64_t n,
+                                                      uint4* __restrict__ pts) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    QAff q = load_qaff(table, 2 * i);
+    Aff a;
+    a.x = fq_to_fp(q.x);
+    a.y = fq_to_fp(q.y);
+    store_aff(pts, i, a);
+}
 
 __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ pts,
                                                      uint4* __restrict__ table, uint64_t n) {
@@ -123,11 +146,7 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
     Aff a = load_aff(pts, i);
     a.x = fp_canon(a.x);
     a.y = fp_canon(a.y);
-    store_aff(table, 2 * i, a);
-    Aff e;
-    e.x = fp_mul(fp_beta(), a.x);  // (0,0) maps to (0,0)
-    e.y = a.y;
-    store_aff(table, 2 * i + 1, e);
+    store_table_pair(table, i, a);
 }
 
 // --------------------------------------------------- GLV + signed digits
@@ -396,9 +415,9 @@ __global__ __launch_bounds__(256) void k_fill_zero(uint4* __restrict__ p, uint64
 // merge records 2L (first run) and 2L+1 (last run).
 
 __device__ __forceinline__ void put_record(uint32_t* keys, uint4* pts, uint64_t slot,
-                                           uint32_t key, bool valid, const Xyzz& v) {
+                                           uint32_t key, bool valid, const QXyzz& v) {
     keys[slot] = key | (valid ? 0u : ZK_REC_INVALID);
-    if (valid) store_xyzz(pts, slot, v);
+    if (valid) store_qxyzz(pts, slot, v);
 }
 
 __global__ __launch_bounds__(256) void k_accumulate(
@@ -424,31 +443,31 @@ __global__ __launch_bounds__(256) void k_accumulate(
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
-    Xyzz acc = xyzz_identity();
+    QXyzz acc = qxyzz_identity();
     // software pipeline: the next entry's key, value and base are loaded
     // before the current mixed addition, hiding the random 64-byte gather
     uint32_t k = cur, v = vals[b];
-    Aff q = load_aff(tb, v & 0x7fffffffu);
+    QAff q = load_qaff(tb, v & 0x7fffffffu);
     for (uint64_t p = b; p < e; p++) {
         const uint64_t pn = min(p + 1, e - 1);
         const uint32_t kn = keys[pn], vn = vals[pn];
-        const Aff qn = load_aff(tb, vn & 0x7fffffffu);
+        const QAff qn = load_qaff(tb, vn & 0x7fffffffu);
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
             const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
             if (first && head_open) {
                 put_record(rec_keys, rec_pts, 2 * L, gk, true, acc);
             } else {
-                store_xyzz(buckets, gk, acc);
+                store_qxyzz(buckets, gk, acc);
                 if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, acc);
             }
             first = false;
-            acc = xyzz_identity();
+            acc = qxyzz_identity();
             cur = k;
         }
-        const bool q_id = aff_is_id(q);
-        if (v >> 31) q.y = fp_neg(q.y);
-        xyzz_madd(acc, q.x, q.y, q_id);
+        const bool q_id = qaff_is_id(q);
+        if (v >> 31) q.y = fq_sub<1>(fq_zero(), q.y);  // p - y
+        qxyzz_madd(acc, q.x, q.y, q_id);
         k = kn;
         v = vn;
         q = qn;
@@ -457,11 +476,11 @@ __global__ __launch_bounds__(256) void k_accumulate(
     const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
     if (first) {
         const bool open = head_open || tail_open;
-        if (!open) store_xyzz(buckets, gk, acc);
+        if (!open) store_qxyzz(buckets, gk, acc);
         put_record(rec_keys, rec_pts, 2 * L, gk, open, acc);
         put_record(rec_keys, rec_pts, 2 * L + 1, gk, false, acc);
     } else {
-        if (!tail_open) store_xyzz(buckets, gk, acc);
+        if (!tail_open) store_qxyzz(buckets, gk, acc);
         put_record(rec_keys, rec_pts, 2 * L + 1, gk, tail_open, acc);
     }
 }
@@ -471,17 +490,20 @@ __global__ __launch_bounds__(256) void k_accumulate(
 // consecutive lanes; padding of empty lanes carries key 0x7fffffff).
 // k_merge_runs: the lane holding the first record of a run with at most
 // `limit` payload records (and at most 3*limit records in all) sums it and
-// writes dest[key]; a longer run is appended to long_list.  k_merge_long: one
-// 256-lane workgroup per listed run, lanes striding over its records, a
-// shuffle tree per wave and a two-level tree over the four wave partials:
-// depth len/256 + 8 additions.
+// writes dest[key]; a run of at most 64 records is appended to mid_list, a
+// longer one to long_list.  k_merge_mid: eight lanes per listed run (eight
+// runs per wave), each summing every eighth record, then a 3-level shuffle
+// tree.  k_merge_long: one 256-lane workgroup per listed run, a shuffle tree
+// per wave and a two-level tree over the four wave partials: depth len/256 + 8.
 
 __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     uint32_t limit, uint4* __restrict__ dest,
                                                     uint32_t* __restrict__ long_count,
                                                     uint32_t* __restrict__ long_list,
-                                                    uint32_t long_cap) {
+                                                    uint32_t* __restrict__ mid_count,
+                                                    uint32_t* __restrict__ mid_list,
+                                                    uint32_t list_cap) {
     // lane i looks at records 2i and 2i+1 (one accumulation lane's head and
     // tail slots); at most one of them starts a run with a payload
     const uint64_t i2 = 2 * ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -499,21 +521,59 @@ __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__
             j++;
         }
         if (valid > limit || (j - i > 3 * limit && j < nrec && (keys[j] & ZK_REC_KEYMASK) == k)) {
-            const uint32_t slot = atomicAdd(long_count, 1u);
-            if (slot < long_cap) long_list[slot] = (uint32_t)i;
+            // records are sorted by key: the run is longer than 64 records
+            // iff record i + 64 still carries its key
+            const bool is_long = i + 64 < nrec && (keys[i + 64] & ZK_REC_KEYMASK) == k;
+            const uint32_t slot = atomicAdd(is_long ? long_count : mid_count, 1u);
+            if (slot < list_cap) (is_long ? long_list : mid_list)[slot] = (uint32_t)i;
             continue;
         }
         bool has = false;
-        Xyzz acc = xyzz_identity();
+        QXyzz acc = qxyzz_identity();
         for (uint64_t p = i; p < j; p++) {
             if (!(keys[p] & ZK_REC_INVALID)) {
-                Xyzz q = load_xyzz(pts, p);
-                if (has) xyzz_add(acc, q);
+                QXyzz q = load_qxyzz(pts, p);
+                if (has) qxyzz_add(acc, q);
                 else acc = q;
                 has = true;
             }
         }
-        if (has) store_xyzz(dest, k, acc);
+        if (has) store_qxyzz(dest, k, acc);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ keys,
+                                                   const uint4* __restrict__ pts, uint64_t nrec,
+                                                   const uint32_t* __restrict__ mid_count,
+                                                   const uint32_t* __restrict__ mid_list,
+                                                   uint32_t list_cap, uint4* __restrict__ dest) {
+    const uint32_t n = min(*mid_count, list_cap);
+    const uint32_t lane = __lane_id(), sub = lane & 7u;
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    for (uint64_t base = wave * 8; base < n; base += nwaves * 8) {  // uniform per wave
+        const uint64_t run = base + (lane >> 3);
+        bool has = false;
+        QXyzz acc = qxyzz_identity();
+        uint32_t k = 0;
+        if (run < n) {
+            const uint64_t start = mid_list[run];
+            k = keys[start] & ZK_REC_KEYMASK;
+            for (uint64_t p = start + sub; p < start + 64 && p < nrec; p += 8) {
+                const uint32_t kr = keys[p];
+                if ((kr & ZK_REC_KEYMASK) != k) break;
+                if (!(kr & ZK_REC_INVALID)) {
+                    QXyzz q = load_qxyzz(pts, p);
+                    if (has) qxyzz_add(acc, q);
+                    else acc = q;
+                    has = true;
+                }
+            }
+        }
+        acc = group_sum(acc, 8);
+        // the group's first lane stores if any lane of the group had a payload
+        const uint64_t hb = __ballot(has);
+        if (run < n && sub == 0 && ((hb >> (lane & ~7u)) & 0xffull)) store_qxyzz(dest, k, acc);
     }
 }
 
@@ -530,34 +590,34 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
         const uint64_t start = long_list[run];
         const uint32_t k = keys[start] & ZK_REC_KEYMASK;
         bool has = false;
-        Xyzz acc = xyzz_identity();
+        QXyzz acc = qxyzz_identity();
         for (uint64_t p = start + t; p < nrec; p += 256) {
             const uint32_t kr = keys[p];
             if ((kr & ZK_REC_KEYMASK) != k) break;
             if (!(kr & ZK_REC_INVALID)) {
-                Xyzz q = load_xyzz(pts, p);
-                if (has) xyzz_add(acc, q);
+                QXyzz q = load_qxyzz(pts, p);
+                if (has) qxyzz_add(acc, q);
                 else acc = q;
                 has = true;
             }
         }
-        acc = group_sum_xyzz(acc, 64);  // identity lanes add nothing
+        acc = group_sum(acc, 64);  // identity lanes add nothing
         const bool wave_has = __ballot(has) != 0;
         if (lane == 0) {
-            store_xyzz(&part[wv][0], 0, acc);
+            store_qxyzz(&part[wv][0], 0, acc);
             part_has[wv] = wave_has ? 1u : 0u;
         }
         __syncthreads();
         if (lane == 0 && (wv & 1) == 0) {
-            Xyzz o = load_xyzz(&part[wv + 1][0], 0);
-            xyzz_add(acc, o);
-            store_xyzz(&part[wv][0], 0, acc);
+            QXyzz o = load_qxyzz(&part[wv + 1][0], 0);
+            qxyzz_add(acc, o);
+            store_qxyzz(&part[wv][0], 0, acc);
         }
         __syncthreads();
         if (t == 0) {
-            Xyzz o = load_xyzz(&part[2][0], 0);
-            xyzz_add(acc, o);
-            if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_xyzz(dest, k, acc);
+            QXyzz o = load_qxyzz(&part[2][0], 0);
+            qxyzz_add(acc, o);
+            if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_qxyzz(dest, k, acc);
         }
         __syncthreads();  // LDS partials are reused by the next run
     }
@@ -609,18 +669,18 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     const uint32_t nitem = rows ? nrow : ncol;
     const uint32_t len = rows ? ncol : nrow;  // buckets per item
     const uint32_t per = len / P;
-    Xyzz acc = xyzz_identity();
+    QXyzz acc = qxyzz_identity();
     if (item < nitem) {
         for (uint32_t e = 0; e < per; e++) {
             const uint32_t x = part * per + e;
             const uint64_t k = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
-            Xyzz q = load_xyzz(buckets, sg.bkt0 + k);
+            QXyzz q = load_qxyzz(buckets, sg.bkt0 + k);
             if (e == 0) acc = q;
-            else xyzz_add(acc, q);
+            else qxyzz_add(acc, q);
         }
     }
-    acc = group_sum_xyzz(acc, P);
-    if (item < nitem && part == 0) store_xyzz(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
+    acc = group_sum(acc, P);
+    if (item < nitem && part == 0) store_qxyzz(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
 }
 
 // index of the i-th value with bit b set: insert a 1 at bit position b
@@ -652,17 +712,17 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
     } else {
         base = R0; b = j - 1 - g; count = 1u << (h - 1); masked = true;
     }
-    Xyzz acc = xyzz_identity();
+    QXyzz acc = qxyzz_identity();
     bool first = true;
     for (uint32_t i = lane; i < count; i += 64) {
         const uint32_t idx = masked ? with_bit(i, b) : i;
-        Xyzz q = load_xyzz(rc, base + idx);
+        QXyzz q = load_qxyzz(rc, base + idx);
         if (first) acc = q;
-        else xyzz_add(acc, q);
+        else qxyzz_add(acc, q);
         first = false;
     }
-    acc = group_sum_xyzz(acc, count >= 64 ? 64u : count);  // count is a power of 2
-    if (lane == 0) store_xyzz(bits, item, acc);
+    acc = group_sum(acc, count >= 64 ? 64u : count);  // count is a power of 2
+    if (lane == 0) store_xyzz(bits, item, qxyzz_to_xyzz(acc));  // host form
 }
 
 // ------------------------------------------------ synthetic base table
@@ -693,11 +753,7 @@ __global__ __launch_bounds__(256) void k_gen_bases(const uint4* __restrict__ ks,
         a.x = fp_mul(acc.X, fp_mul(t, acc.ZZZ));
         a.y = fp_mul(acc.Y, fp_mul(t, acc.ZZ));
     }
-    store_aff(table, 2 * i, a);
-    Aff e;
-    e.x = fp_mul(fp_beta(), a.x);
-    e.y = a.y;
-    store_aff(table, 2 * i + 1, e);
+    store_table_pair(table, i, a);
 }
 
 // ------------------------------------------------------ test entry points
@@ -720,21 +776,37 @@ __global__ __launch_bounds__(64) void k_test_fp(int op, const uint4* __restrict_
     store_fp(out + 2 * i, r);
 }
 
-// op 0: P + Q (XYZZ add), 1: 2P, 2: P + affine(Q.X, Q.Y) with (0,0) as identity
+// Points in and out in the 2^256 Montgomery form.  op 0: P + Q (XYZZ add),
+// 1: 2P, 2: P + affine(Q.X, Q.Y) with (0,0) as identity -- on the 29-bit
+// arithmetic of the MSM kernels; ops 3, 4, 5: the same on the 32-bit
+// arithmetic (used by the base generator).
 __global__ __launch_bounds__(64) void k_test_xyzz(int op, const uint4* __restrict__ p, const uint4* __restrict__ q,
                             uint4* __restrict__ out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Xyzz a = load_xyzz(p, i), b = load_xyzz(q, i);
-    if (op == 0) xyzz_add(a, b);
-    else if (op == 1) a = xyzz_dbl(a);
-    else {
-        Aff qa;
-        qa.x = b.X;
-        qa.y = b.Y;
-        xyzz_madd(a, qa.x, qa.y, aff_is_id(qa));
+    if (op >= 3) {
+        if (op == 3) xyzz_add(a, b);
+        else if (op == 4) a = xyzz_dbl(a);
+        else {
+            Aff qa;
+            qa.x = b.X;
+            qa.y = b.Y;
+            xyzz_madd(a, qa.x, qa.y, aff_is_id(qa));
+        }
+        store_xyzz(out, i, a);
+        return;
     }
-    store_xyzz(out, i, a);
+    QXyzz qa = qxyzz_from_xyzz(a), qb = qxyzz_from_xyzz(b);
+    if (op == 0) qxyzz_add(qa, qb);
+    else if (op == 1) qa = qxyzz_dbl(qa);
+    else {
+        QAff t;
+        t.x = qb.X;
+        t.y = qb.Y;
+        qxyzz_madd(qa, t.x, t.y, qaff_is_id(t));
+    }
+    store_xyzz(out, i, qxyzz_to_xyzz(qa));
 }
 
 __global__ __launch_bounds__(64) void k_test_glv(const uint4* __restrict__ scalars, uint4* __restrict__ out,
@@ -871,6 +943,12 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) {
     return (unsigned)((n + bs - 1) / bs);
 }
 
+void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_table_export, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                       (const uint4*)table, n, (uint4*)pts);
+}
+
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_build_table, dim3(blocks_for(n, 256)), dim3(256), 0, st,
@@ -921,16 +999,21 @@ void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t 
 }
 
 void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
-                       void* dest, uint32_t* long_count, uint32_t* long_list, uint32_t long_cap,
+                       void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                        hipStream_t st) {
-    if (!nrec) return;
+    if (!nrec || !list_cap) return;
+    uint32_t* long_list = lists;
+    uint32_t* mid_list = lists + list_cap;
     hipLaunchKernelGGL(k_merge_runs, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
-                       (const uint4*)pts, nrec, limit, (uint4*)dest, long_count, long_list,
-                       long_cap);
-    if (!long_cap) return;
-    hipLaunchKernelGGL(k_merge_long, dim3(std::min<uint32_t>(long_cap, 2048)), dim3(256), 0, st, keys,
-                       (const uint4*)pts, nrec, (const uint32_t*)long_count,
-                       (const uint32_t*)long_list, long_cap, (uint4*)dest);
+                       (const uint4*)pts, nrec, limit, (uint4*)dest, counts, long_list,
+                       counts + 1, mid_list, list_cap);
+    // grid-stride over the lists (their lengths are known on the device only)
+    hipLaunchKernelGGL(k_merge_mid, dim3(std::min<uint32_t>(blocks_for(list_cap, 32), 1024)), dim3(256),
+                       0, st, keys, (const uint4*)pts, nrec, (const uint32_t*)(counts + 1),
+                       (const uint32_t*)mid_list, list_cap, (uint4*)dest);
+    hipLaunchKernelGGL(k_merge_long, dim3(std::min<uint32_t>(list_cap, 2048)), dim3(256), 0, st, keys,
+                       (const uint4*)pts, nrec, (const uint32_t*)counts,
+                       (const uint32_t*)long_list, list_cap, (uint4*)dest);
 }
 
 void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,

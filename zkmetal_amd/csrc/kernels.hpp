@@ -7,6 +7,7 @@
 
 namespace zk {
 
+void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);
 void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,
                       const SegDesc* segs, const void* scalars, uint64_t total, void* digits,
@@ -21,8 +22,10 @@ void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t 
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
+// counts[0] / counts[1]: long / mid run counts (zeroed by the caller);
+// lists: 2 * list_cap words (long list, then mid list)
 void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
-                       void* dest, uint32_t* long_count, uint32_t* long_list, uint32_t long_cap,
+                       void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                        hipStream_t st);
 void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
                           const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves,

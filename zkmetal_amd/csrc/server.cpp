@@ -19,7 +19,7 @@ uint64_t Server::footprint(uint32_t n) {
 }
 
 int32_t Server::create(const void* table, uint32_t table_n, const ServerConfig& cfg,
-                       Server** out) {
+                       Server** out, const Engine* tuning) {
     if (!out || !table || table_n == 0 || cfg.slots == 0 || cfg.max_batch == 0) return kInvalidInput;
     *out = nullptr;
     Server* s = new Server();
@@ -33,6 +33,7 @@ int32_t Server::create(const void* table, uint32_t table_n, const ServerConfig& 
             delete s;
             return rc;
         }
+        if (tuning) e->copy_tuning_from(*tuning);
         s->engines_.push_back(e);
     }
     for (uint32_t i = 0; i < cfg.slots; i++) s->threads_.emplace_back(&Server::worker, s, i);

@@ -39,8 +39,9 @@ struct ServerStats {
 
 class Server {
   public:
+    // slot engines take their window tables from `tuning` when given
     static int32_t create(const void* table, uint32_t table_n, const ServerConfig& cfg,
-                          Server** out);
+                          Server** out, const Engine* tuning = nullptr);
     ~Server();
 
     // copies the scalars (or, with copy == false, reads them in place: the
