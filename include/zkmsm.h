@@ -27,6 +27,19 @@ ZkMetalStatus zkmsm_srs_upload(ZkMetalMSMEngine engine, const uint8_t* points, u
  * computed on the device (benchmark / test inputs). */
 ZkMetalStatus zkmsm_srs_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n);
 /* Copy the first n bases back (64 B each). */
+/* Resident table from an SRS file image in the reference's layout
+ * (Sources/zkMetal/KZG/SRSFileFormat.swift:57-160: "ptau", version 1,
+ * sections {u32 type, u64 length}; section 1 header with curve id 1 = BN254;
+ * section 2 G1 powers, 64 B per point, x and y 32-byte big-endian standard
+ * form, (0,0) = identity).  The first max_points powers (0 = all) are
+ * converted and checked on the device: a coordinate >= p or a point off the
+ * curve fails the load with ZKMETAL_ERR_INVALID_INPUT (and leaves no table).
+ * *n_loaded (nullable) receives the number of bases. */
+ZkMetalStatus zkmsm_srs_load_ptau(ZkMetalMSMEngine engine, const uint8_t* data, uint64_t len,
+                                  uint32_t max_points, uint32_t* n_loaded);
+/* Same from a file path (memory-mapped). */
+ZkMetalStatus zkmsm_srs_load_ptau_file(ZkMetalMSMEngine engine, const char* path,
+                                       uint32_t max_points, uint32_t* n_loaded);
 ZkMetalStatus zkmsm_srs_download(ZkMetalMSMEngine engine, uint8_t* points, uint32_t n);
 uint32_t zkmsm_srs_size(ZkMetalMSMEngine engine);
 
@@ -41,6 +54,21 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
                                               const void* device_scalars,
                                               const uint32_t* counts, uint32_t n_msms,
                                               uint8_t* results);
+
+/* KZG commitments over the resident table (KZGEngine.commit / batchCommit,
+ * Sources/zkMetal/KZG/KZGEngine.swift:131-179): polynomial p has counts[p]
+ * coefficients, each an Fr element in Montgomery form (32 B, 8 x u32 LE --
+ * the reference's Fr limbs), concatenated; commitment p = sum_j coeff_j *
+ * SRS[j], 96 B X | Y | Z per polynomial as for zkmetal_bn254_msm.  The
+ * Montgomery-to-standard conversion (the reference does it on the host,
+ * bn254_fr_batch_to_limbs, Sources/NeonFieldOps/bn254_msm.c:1537-1552) runs
+ * inside the decomposition kernel.  Polynomials of different lengths share
+ * one call. */
+ZkMetalStatus zkmsm_kzg_commit(ZkMetalMSMEngine engine, const uint8_t* coeffs_mont,
+                               const uint32_t* counts, uint32_t n_polys, uint8_t* results);
+/* Same, with the coefficients already in device memory (e.g. from a GPU NTT). */
+ZkMetalStatus zkmsm_kzg_commit_device(ZkMetalMSMEngine engine, const void* device_coeffs_mont,
+                                      const uint32_t* counts, uint32_t n_polys, uint8_t* results);
 
 /* Plain MSMs over bases and scalars that are both already in device memory
  * (the zkmetal_bn254_msm_batch contract without the host copies): the

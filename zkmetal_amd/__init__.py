@@ -97,6 +97,10 @@ def lib():
         "zkmsm_set_window_table": (i32, [vp, vp, u32]),
         "zkmsm_set_batch_window_table": (i32, [vp, vp, u32]),
         "zkmsm_set_graph_mode": (i32, [vp, i32]),
+        "zkmsm_kzg_commit": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_srs_load_ptau": (i32, [vp, vp, u64, u32, ctypes.POINTER(u32)]),
+        "zkmsm_srs_load_ptau_file": (i32, [vp, ctypes.c_char_p, u32, ctypes.POINTER(u32)]),
+        "zkmsm_kzg_commit_device": (i32, [vp, vp, vp, u32, vp]),
         "zkmsm_graph_captures": (u64, [vp]),
         "zkmsm_server_create": (i32, [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
         "zkmsm_server_submit": (i32, [vp, vp, u32, ctypes.POINTER(u64)]),
@@ -287,6 +291,34 @@ class MetalMSM:
             rc = lib().zkmsm_msm_batch_resident(self.handle, sp, ctypes.c_void_p(counts.ctypes.data),
                                                 len(counts), ctypes.c_void_p(out.ctypes.data))
         _check(rc, "msm_batch_resident")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def srs_load_ptau(self, data=None, path=None, max_points=0):
+        """Resident table from an SRS file (reference .ptau layout): bytes or a
+        path; returns the number of bases loaded."""
+        n = ctypes.c_uint32()
+        if path is not None:
+            rc = lib().zkmsm_srs_load_ptau_file(self.handle, os.fsencode(path), max_points, ctypes.byref(n))
+        else:
+            sp, k = _buf(data)
+            rc = lib().zkmsm_srs_load_ptau(self.handle, sp, len(data), max_points, ctypes.byref(n))
+        _check(rc, "srs_load_ptau")
+        return n.value
+
+    def kzg_commit(self, coeffs_mont, counts):
+        """KZG commitments over the resident SRS (KZGEngine.batchCommit): each
+        polynomial's coefficients are Fr elements in Montgomery form (32 B LE),
+        concatenated; one 96-byte X|Y|Z result per polynomial."""
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        if isinstance(coeffs_mont, DeviceBuffer):
+            rc = lib().zkmsm_kzg_commit_device(self.handle, coeffs_mont.ptr, ctypes.c_void_p(counts.ctypes.data),
+                                               len(counts), ctypes.c_void_p(out.ctypes.data))
+        else:
+            sp, k = _buf(coeffs_mont)
+            rc = lib().zkmsm_kzg_commit(self.handle, sp, ctypes.c_void_p(counts.ctypes.data),
+                                        len(counts), ctypes.c_void_p(out.ctypes.data))
+        _check(rc, "kzg_commit")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
 
     def set_window_table(self, table, batch=False):

@@ -1,5 +1,9 @@
 // extern "C" boundary of libzkmetal_amd.so: the zkMetal MSM ABI
 // (include/zkmetal.h) and this library's extensions (include/zkmsm.h).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -194,6 +198,53 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->msm_resident(device_scalars, true, counts, n_msms, results);
+}
+
+ZkMetalStatus zkmsm_srs_load_ptau(ZkMetalMSMEngine engine, const uint8_t* data, uint64_t len,
+                                  uint32_t max_points, uint32_t* n_loaded) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_load_ptau(data, len, max_points, n_loaded);
+}
+
+ZkMetalStatus zkmsm_srs_load_ptau_file(ZkMetalMSMEngine engine, const char* path,
+                                       uint32_t max_points, uint32_t* n_loaded) {
+    Engine* e = as_engine(engine);
+    if (!e || !path) return ZKMETAL_ERR_INVALID_INPUT;
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) return ZKMETAL_ERR_INVALID_INPUT;
+    struct stat sb;
+    if (fstat(fd, &sb) != 0 || sb.st_size < 12) {
+        close(fd);
+        return ZKMETAL_ERR_INVALID_INPUT;
+    }
+    void* map = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (map == MAP_FAILED) return ZKMETAL_ERR_INVALID_INPUT;
+    ZkMetalStatus rc;
+    {
+        std::lock_guard<std::mutex> lk(e->mutex());
+        rc = e->srs_load_ptau((const uint8_t*)map, (uint64_t)sb.st_size, max_points, n_loaded);
+    }
+    munmap(map, (size_t)sb.st_size);
+    return rc;
+}
+
+ZkMetalStatus zkmsm_kzg_commit(ZkMetalMSMEngine engine, const uint8_t* coeffs_mont,
+                               const uint32_t* counts, uint32_t n_polys, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->kzg_commit(coeffs_mont, false, counts, n_polys, results);
+}
+
+ZkMetalStatus zkmsm_kzg_commit_device(ZkMetalMSMEngine engine, const void* device_coeffs_mont,
+                                      const uint32_t* counts, uint32_t n_polys, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->kzg_commit(device_coeffs_mont, true, counts, n_polys, results);
 }
 
 ZkMetalStatus zkmsm_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,

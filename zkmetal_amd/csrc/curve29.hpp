@@ -207,6 +207,40 @@ __device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y
     acc = qxyzz_select(q_id, acc, r);
 }
 
+// Bucket-accumulation step acc += (x2, y2), (x2, y2) an affine point other
+// than (0,0).  The caller keeps "acc is the identity" in `id` instead of
+// testing ZZ each step; the rare cases acc == +-q take a branch instead of
+// being selected on every step.  acc is meaningful only while !id.
+__device__ __forceinline__ void qxyzz_madd_step(QXyzz& acc, bool& id, const Fq& x2, const Fq& y2) {
+    if (id) {
+        acc.X = x2;
+        acc.Y = y2;
+        acc.ZZ = fq_one();
+        acc.ZZZ = acc.ZZ;
+        id = false;
+        return;
+    }
+    const Fq U2 = fq_mul(x2, acc.ZZ);
+    const Fq S2 = fq_mul(y2, acc.ZZZ);
+    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
+    const Fq R = fq_sub<4>(S2, acc.Y);
+    const Fq PP = fq_mul(P, P);
+    const Fq PPP = fq_mul(P, PP);
+    const Fq Q = fq_mul(acc.X, PP);
+    const Fq RR = fq_mul(R, R);
+    if (__builtin_expect(fq_is_zero2(PP), 0)) {
+        // same x: acc == q doubles, acc == -q cancels to the identity
+        if (fq_is_zero2(RR)) acc = qxyzz_dbl_aff(x2, y2);
+        else id = true;
+        return;
+    }
+    const Fq X3 = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));  // < 3.5p
+    acc.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, X3)), fq_mul(acc.Y, PPP));  // < 3.5p
+    acc.X = X3;
+    acc.ZZ = fq_mul(acc.ZZ, PP);
+    acc.ZZZ = fq_mul(acc.ZZZ, PPP);
+}
+
 // acc += q (both XYZZ)
 __device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
     const bool p_id = qxyzz_is_id(acc);

@@ -93,7 +93,7 @@ uint32_t default_window_bits(uint64_t n) {
 }
 
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
-              Plan& pl) {
+              Plan& pl, uint32_t form) {
     pl = Plan();
     pl.msms.resize(nmsm);
     pl.msm_scalar0.resize(nmsm + 1);
@@ -108,6 +108,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         m.c = c;
         m.nwin = nwin_for(c);
         m.seg0 = (uint32_t)pl.segs.size();
+        m.form = form;
         pl.msm_scalar0[j] = pl.scalars;
         pl.scalars += ns[j];
         for (uint32_t w = 0; w < m.nwin; w++) {
@@ -529,6 +530,60 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     return kOk;
 }
 
+// SRS file (Sources/zkMetal/KZG/SRSFileFormat.swift:92-160): "ptau", u32
+// version 1, u32 section count, then sections {u32 type, u64 length, data}.
+// Section 1: u32 curve id (1 = BN254), u32 power, u32 ceremony power;
+// section 2: G1 powers, 64 B per point (x, y big-endian standard form).
+// Other sections (G2 powers, contributions) are skipped.
+int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_points,
+                              uint32_t* n_loaded) {
+    if (!data || len < 12) return kInvalidInput;
+    auto rd32 = [&](uint64_t off) {
+        return (uint32_t)data[off] | ((uint32_t)data[off + 1] << 8) | ((uint32_t)data[off + 2] << 16) |
+               ((uint32_t)data[off + 3] << 24);
+    };
+    if (memcmp(data, "ptau", 4) != 0 || rd32(4) != 1) return kInvalidInput;
+    const uint32_t nsec = rd32(8);
+    uint64_t off = 12;
+    const uint8_t* g1 = nullptr;
+    uint64_t g1_len = 0;
+    bool bn254 = false;
+    for (uint32_t s = 0; s < nsec; s++) {
+        if (off + 12 > len) return kInvalidInput;
+        const uint32_t type = rd32(off);
+        const uint64_t slen = (uint64_t)rd32(off + 4) | ((uint64_t)rd32(off + 8) << 32);
+        off += 12;
+        if (slen > len - off) return kInvalidInput;
+        if (type == 1) {
+            if (slen < 12) return kInvalidInput;
+            bn254 = rd32(off) == 1;
+        } else if (type == 2) {
+            g1 = data + off;
+            g1_len = slen;
+        }
+        off += slen;
+    }
+    if (!bn254 || !g1) return kInvalidInput;
+    uint64_t n = g1_len / 64;
+    if (max_points) n = std::min<uint64_t>(n, max_points);
+    if (n == 0 || n >= (1ull << 31)) return kInvalidInput;
+    srs_n_ = 0;
+    ZK_ALLOC(pts_in_, n * 64);
+    ZK_ALLOC(srs_, n * 128);
+    ZK_ALLOC(err_, 16);
+    ZK_TRY(hipMemcpyAsync(pts_in_.p, g1, n * 64, hipMemcpyHostToDevice, stream_));
+    launch_fill_zero(err_.p, 16, stream_);
+    launch_import_be(pts_in_.p, n, srs_.p, (uint32_t*)err_.p, stream_);
+    uint32_t bad = 0;
+    ZK_TRY(hipMemcpyAsync(&bad, err_.p, 4, hipMemcpyDeviceToHost, stream_));
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
+    if (bad) return kInvalidInput;  // not canonical or not on the curve
+    srs_n_ = (uint32_t)n;
+    if (n_loaded) *n_loaded = srs_n_;
+    return kOk;
+}
+
 // b * 2^(8w) * G for w < 32, b < 256, affine Montgomery (entry 0 = (0,0))
 int32_t Engine::ensure_gtab() {
     if (gtab_.p) return kOk;
@@ -593,6 +648,19 @@ int32_t Engine::msm_resident(const void* scalars, bool on_device, const uint32_t
     return msm_table(srs_.p, srs_n_, scalars, on_device, counts, nmsm, out96);
 }
 
+int32_t Engine::kzg_commit(const void* coeffs, bool on_device, const uint32_t* counts,
+                           uint32_t npoly, uint8_t* out96) {
+    if (srs_n_ == 0 || !coeffs || !counts) return kInvalidInput;
+    if (on_device) return msm_table_parts(srs_.p, srs_n_, nullptr, coeffs, counts, npoly, out96, 1);
+    std::vector<const uint8_t*> parts(npoly);
+    uint64_t off = 0;
+    for (uint32_t j = 0; j < npoly; j++) {
+        parts[j] = (const uint8_t*)coeffs + off * 32;
+        off += counts[j];
+    }
+    return msm_table_parts(srs_.p, srs_n_, parts.data(), nullptr, counts, npoly, out96, 1);
+}
+
 int32_t Engine::msm_table(const void* table, uint32_t table_n, const void* scalars,
                           bool on_device, const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
     if (!scalars || !counts) return kInvalidInput;
@@ -609,7 +677,8 @@ int32_t Engine::msm_table(const void* table, uint32_t table_n, const void* scala
 
 int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
                                 const uint8_t* const* host_parts, const void* d_scalars,
-                                const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
+                                const uint32_t* counts, uint32_t nmsm, uint8_t* out96,
+                                uint32_t form) {
     if (!table || !counts || !out96 || nmsm == 0) return kInvalidInput;
     if (!host_parts && !d_scalars) return kInvalidInput;
     std::vector<uint32_t> ns, idx;
@@ -630,7 +699,7 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form);
     if (rc != kOk) return rc;
     const void* d_sc = d_scalars;
     if (host_parts) {

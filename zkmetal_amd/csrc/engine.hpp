@@ -80,7 +80,7 @@ uint32_t default_window_bits(uint64_t n);
 
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
-              const uint32_t* cs /* nullable */, Plan& pl);
+              const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0);
 
 class Engine {
   public:
@@ -100,6 +100,8 @@ class Engine {
     int32_t srs_upload(const uint8_t* points, uint32_t n);
     int32_t srs_generate(uint64_t seed, uint32_t n);
     int32_t srs_download(uint8_t* points, uint32_t n);
+    // G1 powers of an SRS file image (see engine.hip); max_points 0 = all
+    int32_t srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_points, uint32_t* n_loaded);
     uint32_t srs_size() const { return srs_n_; }
 
     // MSMs over prefixes of the resident table; scalars on host or device
@@ -113,7 +115,12 @@ class Engine {
     // host_parts is null, one device buffer holding all MSMs' scalars
     int32_t msm_table_parts(const void* table, uint32_t table_n,
                             const uint8_t* const* host_parts, const void* d_scalars,
-                            const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
+                            const uint32_t* counts, uint32_t nmsm, uint8_t* out96,
+                            uint32_t form = 0);
+    // KZG commitments over the resident table: coefficients are Fr elements in
+    // Montgomery form (converted to standard form on the device)
+    int32_t kzg_commit(const void* coeffs, bool on_device, const uint32_t* counts, uint32_t npoly,
+                       uint8_t* out96);
     const void* srs_table() const { return srs_.p; }
 
     // per-size-class window widths: c_by_log2n[ceil(log2 n)], 0 = cost model

@@ -28,6 +28,44 @@ __device__ __forceinline__ uint32_t r_limb(int i) {
     }
 }
 
+// Fr element in Montgomery form (x * 2^256 mod r, the reference's Fr limbs)
+// -> standard form x, for any 256-bit input: Montgomery reduction REDC(k),
+// the same value as the reference's fr_mul(k, 1)
+// (Sources/NeonFieldOps/bn254_msm.c:1537-1552).  The REDC result is below
+// r + 1, so one conditional subtraction makes it canonical.
+#define ZK_RINV32 0xefffffffu  // -r^-1 mod 2^32
+__device__ __forceinline__ void fr_from_mont(uint32_t k[8]) {
+    uint32_t t[17];
+#pragma unroll
+    for (int i = 0; i < 8; i++) t[i] = k[i];
+#pragma unroll
+    for (int i = 8; i < 17; i++) t[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t m = t[i] * ZK_RINV32;  // t + m r == 0 mod 2^(32(i+1))
+        uint64_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint64_t s = (uint64_t)m * r_limb(j) + t[i + j] + c;
+            t[i + j] = (uint32_t)s;
+            c = s >> 32;
+        }
+#pragma unroll
+        for (int j = i + 8; j < 17; j++) {
+            const uint64_t s = (uint64_t)t[j] + c;
+            t[j] = (uint32_t)s;
+            c = s >> 32;
+        }
+    }
+    uint32_t d[8];
+    unsigned br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[i] = __builtin_subc(t[8 + i], r_limb(i), br, &br);
+    br = (t[16] == 0) ? br : 0u;  // t[16] is 0: the result is < r + 1 < 2^256
+#pragma unroll
+    for (int i = 0; i < 8; i++) k[i] = br ? t[8 + i] : d[i];
+}
+
 // out[0..na+nb) = a[0..na) * b[0..nb)
 template <int NA, int NB>
 __device__ __forceinline__ void mul_wide(const uint32_t* a, const uint32_t* b, uint32_t* out) {

@@ -127,6 +127,64 @@ __device__ __forceinline__ void store_table_pair(uint4* table, uint64_t i, const
     store_qaff(table, 2 * i + 1, q);
 }
 
+// SRS import from an SRS file's G1 section (Sources/zkMetal/KZG/
+// SRSFileFormat.swift:92-160, TrustedSetup.swift:58-70,686-696): point i is
+// x then y, 32-byte big-endian standard-form integers; (0,0) is the identity.
+// Each point is checked canonical (x, y < p) and on the curve y^2 = x^3 + 3;
+// `bad` counts the points that are not.  Writes the table pair of every point.
+__device__ __forceinline__ Fp fp_from_be(const uint8_t* b) {
+    Fp r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint8_t* w = b + 28 - 4 * j;
+        r.v[j] = ((uint32_t)w[0] << 24) | ((uint32_t)w[1] << 16) | ((uint32_t)w[2] << 8) | w[3];
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool fp_lt_p(const Fp& a) {
+    unsigned br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) (void)__builtin_subc(a.v[i], p_limb(i), br, &br);
+    return br != 0;
+}
+
+__device__ __forceinline__ bool fq_eq(const Fq& a, const Fq& b) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) d |= a.v[i] ^ b.v[i];
+    return d == 0;
+}
+
+__global__ __launch_bounds__(256) void k_import_be(const uint8_t* __restrict__ raw, uint64_t n,
+                                                   uint4* __restrict__ table,
+                                                   uint32_t* __restrict__ bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Fp x = fp_from_be(raw + 64 * i), y = fp_from_be(raw + 64 * i + 32);
+    Fq k2;  // 2^522 mod p: a Montgomery product with it maps x to x * 2^261
+    k2.v[0] = 0x059bac10u; k2.v[1] = 0x0d1503a3u; k2.v[2] = 0x018016b8u; k2.v[3] = 0x10ab0ca8u;
+    k2.v[4] = 0x02632639u; k2.v[5] = 0x02c0169fu; k2.v[6] = 0x169bfd53u; k2.v[7] = 0x11869d4cu;
+    k2.v[8] = 0x002a11a6u;
+    QAff q;
+    q.x = fq_csub<1>(fq_mul(fq_slice(x), k2));
+    q.y = fq_csub<1>(fq_mul(fq_slice(y), k2));
+    bool ok = fp_lt_p(x) && fp_lt_p(y);
+    if (!qaff_is_id(q)) {
+        Fq b3;  // 3 * 2^261 mod p
+        b3.v[0] = 0x00766463u; b3.v[1] = 0x1c54760au; b3.v[2] = 0x08f6927au; b3.v[3] = 0x03e40c4du;
+        b3.v[4] = 0x1fea4f2bu; b3.v[5] = 0x17c6c26au; b3.v[6] = 0x157fe417u; b3.v[7] = 0x0f8056f9u;
+        b3.v[8] = 0x002958a2u;
+        const Fq lhs = fq_csub<1>(fq_mul(q.y, q.y));                                // < p
+        const Fq rhs = fq_csub<1>(fq_csub<2>(fq_add(fq_mul(fq_mul(q.x, q.x), q.x), b3)));  // < 2.5p -> < p
+        ok = ok && fq_eq(lhs, rhs);
+    }
+    if (!ok) atomicAdd(bad, 1u);
+    store_qaff(table, 2 * i, q);
+    q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
+    store_qaff(table, 2 * i + 1, q);
+}
+
 // table entry 2i back to the 2^256 Montgomery form (64 B per point)
 __global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ table, uint64_t n,
                                                       uint4* __restrict__ pts) {
@@ -164,6 +222,7 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
     uint64_t i = g - m.scalar0;
     uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
     uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    if (m.form == 1) fr_from_mont(k);
     GlvSplit sp = glv_split(k);
     const uint32_t c = m.c, half = 1u << (c - 1), full = 1u << c;
     uint32_t carry1 = 0, carry2 = 0;
@@ -443,7 +502,11 @@ __global__ __launch_bounds__(256) void k_accumulate(
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
-    QXyzz acc = qxyzz_identity();
+    // the run's sum; `id` marks it as the identity (run start, or a
+    // cancellation), materialised only when it is stored
+    QXyzz acc;
+    bool id = true;
+    auto value = [&]() { return id ? qxyzz_identity() : acc; };
     // software pipeline: the next entry's key, value and base are loaded
     // before the current mixed addition, hiding the random 64-byte gather
     uint32_t k = cur, v = vals[b];
@@ -455,33 +518,36 @@ __global__ __launch_bounds__(256) void k_accumulate(
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
             const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
+            const QXyzz run = value();
             if (first && head_open) {
-                put_record(rec_keys, rec_pts, 2 * L, gk, true, acc);
+                put_record(rec_keys, rec_pts, 2 * L, gk, true, run);
             } else {
-                store_qxyzz(buckets, gk, acc);
-                if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, acc);
+                store_qxyzz(buckets, gk, run);
+                if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, run);
             }
             first = false;
-            acc = qxyzz_identity();
+            id = true;
             cur = k;
         }
-        const bool q_id = qaff_is_id(q);
-        if (v >> 31) q.y = fq_sub<1>(fq_zero(), q.y);  // p - y
-        qxyzz_madd(acc, q.x, q.y, q_id);
+        if (!qaff_is_id(q)) {  // a (0,0) base adds nothing
+            if (v >> 31) q.y = fq_sub<1>(fq_zero(), q.y);  // p - y
+            qxyzz_madd_step(acc, id, q.x, q.y);
+        }
         k = kn;
         v = vn;
         q = qn;
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
     const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
+    const QXyzz last = value();
     if (first) {
         const bool open = head_open || tail_open;
-        if (!open) store_qxyzz(buckets, gk, acc);
-        put_record(rec_keys, rec_pts, 2 * L, gk, open, acc);
-        put_record(rec_keys, rec_pts, 2 * L + 1, gk, false, acc);
+        if (!open) store_qxyzz(buckets, gk, last);
+        put_record(rec_keys, rec_pts, 2 * L, gk, open, last);
+        put_record(rec_keys, rec_pts, 2 * L + 1, gk, false, last);
     } else {
-        if (!tail_open) store_qxyzz(buckets, gk, acc);
-        put_record(rec_keys, rec_pts, 2 * L + 1, gk, tail_open, acc);
+        if (!tail_open) store_qxyzz(buckets, gk, last);
+        put_record(rec_keys, rec_pts, 2 * L + 1, gk, tail_open, last);
     }
 }
 
@@ -941,6 +1007,12 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 static inline unsigned blocks_for(uint64_t n, unsigned bs) {
     return (unsigned)((n + bs - 1) / bs);
+}
+
+void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_import_be, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                       (const uint8_t*)raw, n, (uint4*)table, bad);
 }
 
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st) {

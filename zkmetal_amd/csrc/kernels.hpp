@@ -7,6 +7,7 @@
 
 namespace zk {
 
+void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);
 void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,

@@ -19,6 +19,8 @@ struct MsmDesc {
     uint32_t c;        // window width in bits (<= 16)
     uint32_t nwin;     // ceil(128 / c)
     uint32_t seg0;     // first segment
+    uint32_t form;     // scalar encoding: 0 standard form, 1 Fr Montgomery form
+    uint32_t reserved;
 };
 
 struct SegDesc {

@@ -45,10 +45,20 @@ def _time(fn, reps):
     return best
 
 
+def _pick(res, which, default, margin):
+    """fastest candidate, but the cost-model default unless beaten by > margin"""
+    best = min(res, key=lambda c: res[c][which])
+    if default in res and res[best][which] > res[default][which] * (1.0 - margin):
+        return default
+    return best
+
+
 def tune(engine=None, min_logn=10, max_logn=22, reps=3, spread=2, dist=0, seed=0x7E57,
-         batch_points=1 << 21, log=None):
+         batch_points=1 << 21, margin=0.03, log=None):
     """Sweep c per size class; returns {"plain": {logn: c}, "batch": {...},
-    "ms": {...}} and leaves the engine with the tuned tables applied."""
+    "ms": {...}} and leaves the engine with the tuned tables applied.  A
+    candidate replaces the cost-model default only when it is faster by more
+    than `margin` (run-to-run noise is a few percent)."""
     own = engine is None
     eng = engine or zk.MetalMSM()
     nmax = 1 << max_logn
@@ -69,8 +79,9 @@ def tune(engine=None, min_logn=10, max_logn=22, reps=3, spread=2, dist=0, seed=0
                 res[c] = (alone, many)
                 if log:
                     log(f"logn {logn} c {c}: alone {alone:.3f} ms, in batch of {k} {many:.3f} ms/MSM")
-            out["plain"][logn] = min(res, key=lambda c: res[c][0])
-            out["batch"][logn] = min(res, key=lambda c: res[c][1])
+            c0 = zk.default_window_bits(n)
+            out["plain"][logn] = _pick(res, 0, c0, margin)
+            out["batch"][logn] = _pick(res, 1, c0, margin)
             out["ms"][logn] = {str(c): [round(a, 4), round(b, 4)] for c, (a, b) in res.items()}
     finally:
         eng.set_window_bits(0)
@@ -114,9 +125,10 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--spread", type=int, default=2)
     ap.add_argument("--dist", type=int, default=0)
+    ap.add_argument("--margin", type=float, default=0.03)
     args = ap.parse_args()
     res = tune(min_logn=args.min_logn, max_logn=args.max_logn, reps=args.reps, spread=args.spread,
-               dist=args.dist, log=lambda s: print(s, flush=True))
+               dist=args.dist, margin=args.margin, log=lambda s: print(s, flush=True))
     print(json.dumps({"plain": res["plain"], "batch": res["batch"]}), flush=True)
     if args.out:
         save(res, args.out)
