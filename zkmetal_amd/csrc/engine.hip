@@ -93,7 +93,7 @@ uint32_t default_window_bits(uint64_t n) {
 }
 
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
-              Plan& pl, uint32_t form) {
+              Plan& pl, uint32_t form, uint64_t lane_cap) {
     pl = Plan();
     pl.msms.resize(nmsm);
     pl.msm_scalar0.resize(nmsm + 1);
@@ -134,13 +134,19 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     // 32-bit positions / bucket ids on the device
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
-    // accumulation: ~2^17 lanes (two waves per SIMD on 256 CUs), >= 4 entries
+    // accumulation: at most lane_cap lanes (one resident round of waves --
+    // every segment rounds its lane count up, hence the - nseg), >= 4 entries
     // each, and (up to 32) at least half a bucket's mean entry count, so that
     // a bucket's partials span few lanes and merge without a long-run pass
     const uint64_t half_bucket = pl.entries / (2 * std::max<uint64_t>(1, pl.buckets));
+    const uint64_t cap = lane_cap > 2ull * nseg + 64 ? lane_cap - nseg : std::max<uint64_t>(lane_cap / 2, 64);
     pl.T = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(4, std::min<uint64_t>(32, half_bucket)),
-                                        (pl.entries + (1u << 17) - 1) >> 17);
-    pl.G = 0;
+                                        (pl.entries + cap - 1) / cap);
+    // bucket reduction: per buckets per lane before the shuffle tree; 8 keeps
+    // the tree shallow for small calls, larger batches (throughput-bound) use
+    // up to 32 so that fewer additions go to tree levels
+    pl.G = 8;
+    while (pl.G < 32 && (pl.buckets >> 17) >= 2 * pl.G) pl.G *= 2;
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
     pl.seg_rcw0.resize(nseg + 1);
@@ -155,7 +161,8 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         // bucket reduction: 2^h rows of 2^g buckets and 2^g columns of 2^h,
         // max(1, len/8) lanes per row / column, rows and columns wave-aligned
         const uint32_t cb = 31 - __builtin_clz(sg.nbk), g = cb / 2, h = cb - g;
-        const uint64_t pr = (1u << g) >= 8 ? (1u << g) / 8 : 1, pc = (1u << h) >= 8 ? (1u << h) / 8 : 1;
+        const uint64_t per = pl.G;
+        const uint64_t pr = (1u << g) >= per ? (1u << g) / per : 1, pc = (1u << h) >= per ? (1u << h) / per : 1;
         pl.rc_waves += (((uint64_t)1 << h) * pr + 63) / 64 + (((uint64_t)1 << g) * pc + 63) / 64;
         pl.groups += (1u << h) + (1u << g);
         for (uint32_t t = 0; t < sg.ntiles; t++) pl.tile_seg.push_back(s);
@@ -182,6 +189,7 @@ int32_t Engine::create(Engine** out) {
         delete e;
         return kGpuError;
     }
+    e->acc_cap_ = accumulate_capacity(e->dev_);
     for (auto& ev : e->ev_) {
         if (hipEventCreate(&ev) != hipSuccess) {
             delete e;
@@ -304,7 +312,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         launch_merge_runs((const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
                           d_err + 1, (uint32_t*)recBk_.p, list_cap, st);
         ZK_TRY(mark(4));
-        launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, buckets_.p, rc_.p,
+        launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, buckets_.p, rc_.p,
                              winsum_.p, st);
         ZK_TRY(mark(5));
         ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
@@ -473,7 +481,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * 64);
     ZK_ALLOC(ptable_, total * 128);
@@ -507,7 +515,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
     if (rc != kOk) return rc;
     ZK_ALLOC(ptable_, total * 128);
     launch_build_table(d_points, ptable_.p, total, stream_);
@@ -699,7 +707,7 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_);
     if (rc != kOk) return rc;
     const void* d_sc = d_scalars;
     if (host_parts) {

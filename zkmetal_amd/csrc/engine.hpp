@@ -72,7 +72,8 @@ struct Plan {
     std::vector<uint32_t> tile_seg;
     std::vector<uint32_t> chunk_seg;
     uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0, rc_waves = 0;
-    uint32_t tiles = 0, chunks = 0, T = 0, G = 0;
+    uint32_t tiles = 0, chunks = 0, T = 0;
+    uint32_t G = 8;  // buckets per bucket-reduction lane (a power of 2, 8..32)
 };
 
 // window width for an MSM of n points (cost model; overridable)
@@ -80,7 +81,8 @@ uint32_t default_window_bits(uint64_t n);
 
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
-              const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0);
+              const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0,
+              uint64_t lane_cap = 1u << 17);
 
 class Engine {
   public:
@@ -169,6 +171,7 @@ class Engine {
     uint32_t ctab_[33] = {};
     uint32_t ctab_batch_[33] = {};
     bool graph_mode_ = false;
+    uint64_t acc_cap_ = 1u << 17;  // resident k_accumulate lanes on this device
     hipGraphExec_t graph_exec_ = nullptr;
     std::vector<uint64_t> graph_key_;
     uint64_t graph_captures_ = 0;

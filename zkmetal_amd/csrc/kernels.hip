@@ -103,6 +103,38 @@ __device__ __forceinline__ QXyzz shfl_down_qxyzz(const QXyzz& a, uint32_t d) {
     return r;
 }
 
+// Lane-sum then group tree with ONE addition site: steps 0..nload-1 add the
+// lane's own elements (load(e) returns the e-th, or the identity), the next
+// log2(P) steps add the partner's partial (shuffle tree over aligned groups of
+// P lanes).  nload and P must be wave-uniform.  Keeping a single inlined
+// addition halves the code of the latency-bound reduction kernels.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d));
+    return v;
+}
+
+template <class Load>
+__device__ __forceinline__ QXyzz sum_then_tree(uint32_t nload, uint32_t P, Load load) {
+    const uint32_t lane = __lane_id();
+    uint32_t levels = 0;
+    while ((1u << levels) < P) levels++;
+    QXyzz acc = qxyzz_identity();
+#pragma unroll 1
+    for (uint32_t st = 0; st < nload + levels; st++) {
+        QXyzz q;
+        if (st < nload) {
+            q = load(st);
+        } else {
+            const uint32_t s = P >> (st - nload + 1);
+            q = shfl_down_qxyzz(acc, s);
+            if ((lane & (P - 1)) >= s) q = qxyzz_identity();
+        }
+        qxyzz_add(acc, q);
+    }
+    return acc;
+}
+
 __device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
     const uint32_t lane = __lane_id();
 #pragma unroll 1
@@ -560,7 +592,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
 // longer one to long_list.  k_merge_mid: eight lanes per listed run (eight
 // runs per wave), each summing every eighth record, then a 3-level shuffle
 // tree.  k_merge_long: one 256-lane workgroup per listed run, a shuffle tree
-// per wave and a two-level tree over the four wave partials: depth len/256 + 8.
+// per wave, then one lane adds the four wave partials: depth len/256 + 9.
 
 __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
@@ -619,24 +651,23 @@ __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ 
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     for (uint64_t base = wave * 8; base < n; base += nwaves * 8) {  // uniform per wave
         const uint64_t run = base + (lane >> 3);
+        uint64_t start = 0;
+        uint32_t k = 0, cnt = 0;
         bool has = false;
-        QXyzz acc = qxyzz_identity();
-        uint32_t k = 0;
         if (run < n) {
-            const uint64_t start = mid_list[run];
+            start = mid_list[run];
             k = keys[start] & ZK_REC_KEYMASK;
             for (uint64_t p = start + sub; p < start + 64 && p < nrec; p += 8) {
                 const uint32_t kr = keys[p];
                 if ((kr & ZK_REC_KEYMASK) != k) break;
-                if (!(kr & ZK_REC_INVALID)) {
-                    QXyzz q = load_qxyzz(pts, p);
-                    if (has) qxyzz_add(acc, q);
-                    else acc = q;
-                    has = true;
-                }
+                cnt++;
+                has |= !(kr & ZK_REC_INVALID);
             }
         }
-        acc = group_sum(acc, 8);
+        const QXyzz acc = sum_then_tree(wave_max(cnt), 8, [&](uint32_t e) {
+            const uint64_t p = start + sub + 8ull * e;
+            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
+        });
         // the group's first lane stores if any lane of the group had a payload
         const uint64_t hb = __ballot(has);
         if (run < n && sub == 0 && ((hb >> (lane & ~7u)) & 0xffull)) store_qxyzz(dest, k, acc);
@@ -655,34 +686,30 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
     for (uint32_t run = blockIdx.x; run < n; run += gridDim.x) {  // uniform per workgroup
         const uint64_t start = long_list[run];
         const uint32_t k = keys[start] & ZK_REC_KEYMASK;
+        uint32_t cnt = 0;
         bool has = false;
-        QXyzz acc = qxyzz_identity();
         for (uint64_t p = start + t; p < nrec; p += 256) {
             const uint32_t kr = keys[p];
             if ((kr & ZK_REC_KEYMASK) != k) break;
-            if (!(kr & ZK_REC_INVALID)) {
-                QXyzz q = load_qxyzz(pts, p);
-                if (has) qxyzz_add(acc, q);
-                else acc = q;
-                has = true;
-            }
+            cnt++;
+            has |= !(kr & ZK_REC_INVALID);
         }
-        acc = group_sum(acc, 64);  // identity lanes add nothing
+        QXyzz acc = sum_then_tree(wave_max(cnt), 64, [&](uint32_t e) {
+            const uint64_t p = start + t + 256ull * e;
+            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
+        });
         const bool wave_has = __ballot(has) != 0;
         if (lane == 0) {
             store_qxyzz(&part[wv][0], 0, acc);
             part_has[wv] = wave_has ? 1u : 0u;
         }
         __syncthreads();
-        if (lane == 0 && (wv & 1) == 0) {
-            QXyzz o = load_qxyzz(&part[wv + 1][0], 0);
-            qxyzz_add(acc, o);
-            store_qxyzz(&part[wv][0], 0, acc);
-        }
-        __syncthreads();
         if (t == 0) {
-            QXyzz o = load_qxyzz(&part[2][0], 0);
-            qxyzz_add(acc, o);
+#pragma unroll 1
+            for (uint32_t w = 1; w < 4; w++) {
+                const QXyzz o = load_qxyzz(&part[w][0], 0);
+                qxyzz_add(acc, o);
+            }
             if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_qxyzz(dest, k, acc);
         }
         __syncthreads();  // LDS partials are reused by the next run
@@ -696,9 +723,10 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
 //   U = sum_lo C_lo,  Pl_b = sum_{lo: bit b} C_lo,  Ph_b = sum_{hi: bit b} R_hi,
 // with row sums R_hi = sum_lo B_{hi,lo} and column sums C_lo = sum_hi B_{hi,lo}
 // (two additions per bucket in all).  k_br_rowcol: a row of 2^g buckets is
-// summed by P_r = max(1, 2^g/8) lanes (8 buckets each, then a log2(P_r)
-// shuffle tree), a column of 2^h buckets by P_c = max(1, 2^h/8) lanes; rows
-// and columns of a segment start on wave boundaries so P is wave-uniform.
+// summed by P_r = max(1, 2^g/per) lanes (`per` buckets each, then a
+// log2(P_r) shuffle tree), a column of 2^h buckets by P_c = max(1, 2^h/per)
+// lanes; rows and columns of a segment start on wave boundaries so P is
+// wave-uniform.  per = 8..32 is chosen by the planner from the call's size.
 // k_br_bits: one wave per output point, bits[s*16 + j]: j = 0 -> U,
 // 1..g -> Pl_{j-1}, g+1..c-1 -> Ph_{j-1-g}.  The host applies the powers of 2.
 
@@ -708,12 +736,12 @@ __device__ __forceinline__ void seg_shape(uint32_t nbk, uint32_t& g, uint32_t& h
     h = cb - g;
 }
 
-__device__ __forceinline__ uint32_t lanes_per(uint32_t len) { return len >= 8 ? len / 8 : 1; }
+__device__ __forceinline__ uint32_t lanes_per(uint32_t len, uint32_t per) { return len >= per ? len / per : 1; }
 
 __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
                                                    const uint64_t* __restrict__ seg_wave0,
                                                    const uint64_t* __restrict__ seg_rc0,
-                                                   uint32_t nseg, uint64_t waves,
+                                                   uint32_t nseg, uint64_t waves, uint32_t per,
                                                    const uint4* __restrict__ buckets,
                                                    uint4* __restrict__ rc) {
     const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -724,7 +752,7 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     uint32_t g, h;
     seg_shape(sg.nbk, g, h);
     const uint32_t nrow = 1u << h, ncol = 1u << g;
-    const uint32_t Pr = lanes_per(ncol), Pc = lanes_per(nrow);
+    const uint32_t Pr = lanes_per(ncol, per), Pc = lanes_per(nrow, per);
     const uint32_t row_waves = (nrow * Pr + 63) / 64;
     uint32_t w = (uint32_t)(wave - seg_wave0[s]);
     const bool rows = w < row_waves;
@@ -734,18 +762,12 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     const uint32_t item = li / P, part = li % P;
     const uint32_t nitem = rows ? nrow : ncol;
     const uint32_t len = rows ? ncol : nrow;  // buckets per item
-    const uint32_t per = len / P;
-    QXyzz acc = qxyzz_identity();
-    if (item < nitem) {
-        for (uint32_t e = 0; e < per; e++) {
-            const uint32_t x = part * per + e;
-            const uint64_t k = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
-            QXyzz q = load_qxyzz(buckets, sg.bkt0 + k);
-            if (e == 0) acc = q;
-            else qxyzz_add(acc, q);
-        }
-    }
-    acc = group_sum(acc, P);
+    const uint32_t cnt = len / P;  // buckets this lane sums
+    const QXyzz acc = sum_then_tree(cnt, P, [&](uint32_t e) {
+        const uint32_t x = part * cnt + e;
+        const uint64_t kk = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
+        return item < nitem ? load_qxyzz(buckets, sg.bkt0 + kk) : qxyzz_identity();
+    });
     if (item < nitem && part == 0) store_qxyzz(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
 }
 
@@ -778,16 +800,12 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
     } else {
         base = R0; b = j - 1 - g; count = 1u << (h - 1); masked = true;
     }
-    QXyzz acc = qxyzz_identity();
-    bool first = true;
-    for (uint32_t i = lane; i < count; i += 64) {
-        const uint32_t idx = masked ? with_bit(i, b) : i;
-        QXyzz q = load_qxyzz(rc, base + idx);
-        if (first) acc = q;
-        else qxyzz_add(acc, q);
-        first = false;
-    }
-    acc = group_sum(acc, count >= 64 ? 64u : count);  // count is a power of 2
+    // count is a power of 2: lanes take elements lane, lane+64, ...
+    const uint32_t P = count >= 64 ? 64u : count;
+    const QXyzz acc = sum_then_tree((count + 63) / 64, P, [&](uint32_t e) {
+        const uint32_t i = lane + 64 * e;
+        return i < count ? load_qxyzz(rc, base + (masked ? with_bit(i, b) : i)) : qxyzz_identity();
+    });
     if (lane == 0) store_xyzz(bits, item, qxyzz_to_xyzz(acc));  // host form
 }
 
@@ -1060,6 +1078,17 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, (uint4*)p, n16);
 }
 
+uint64_t accumulate_capacity(int device) {
+    int cus = 0, blocks = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate, 256, 0) != hipSuccess ||
+        cus <= 0 || blocks <= 0) {
+        (void)hipGetLastError();
+        return 1u << 17;
+    }
+    return (uint64_t)cus * (uint64_t)blocks * 256u;
+}
+
 void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
@@ -1089,11 +1118,11 @@ void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uin
 }
 
 void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
-                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves,
+                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
                           const void* buckets, void* rc, void* bits, hipStream_t st) {
     if (!rc_waves) return;
     hipLaunchKernelGGL(k_br_rowcol, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
-                       seg_wave0, seg_rc0, nseg, rc_waves, (const uint4*)buckets, (uint4*)rc);
+                       seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
     const uint64_t bit_items = (uint64_t)nseg * 16;
     hipLaunchKernelGGL(k_br_bits, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
                        seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);

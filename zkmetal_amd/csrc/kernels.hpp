@@ -19,6 +19,10 @@ void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_s
                       uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
                       uint32_t* out_vals, hipStream_t st);
 void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
+// lanes of k_accumulate resident at once on the device (CUs x workgroups per
+// CU at its register use x 256): a lane count just above it leaves a second,
+// nearly empty round of waves
+uint64_t accumulate_capacity(int device);
 void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
@@ -29,7 +33,7 @@ void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uin
                        void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                        hipStream_t st);
 void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
-                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves,
+                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
                           const void* buckets, void* rc, void* bits, hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
