@@ -284,6 +284,22 @@ def serve(args, eng):
     wall = time.perf_counter() - wall0
     st = srv.stats()
     srv.close()
+    # saturation: the first `burst` requests of the same stream submitted at
+    # once (the server never idles), against the same MSMs as back-to-back
+    # plain calls on one engine
+    burst = min(args.burst, nreq)
+    srv = zk.Server(eng, slots=args.slots, max_batch=args.max_batch,
+                    hbm_budget_bytes=args.hbm_budget_gib << 30)
+    t0 = time.perf_counter()
+    tk = [srv.submit(views[i], copy=False) for i in range(burst)]
+    burst_res = [srv.wait(t)[0] for t in tk]
+    burst_s = time.perf_counter() - t0
+    bst = srv.stats()
+    srv.close()
+    t0 = time.perf_counter()
+    seq_res = [eng.msm_batch_resident(views[i], [sizes[i]])[0] for i in range(burst)]
+    seq_s = time.perf_counter() - t0
+    burst_same = burst_res == seq_res
     # bit-exact: a sample of requests recomputed alone equals the served result
     check = sorted(rnd.choice(nreq, size=min(12, nreq), replace=False).tolist())
     same = all(eng.msm_batch_resident(views[i], [sizes[i]])[0] == results[i][0] for i in check)
@@ -317,7 +333,13 @@ def serve(args, eng):
                   "batches": st["batches"], "mean_batch": round(st["mean_batch"], 3),
                   "hbm_peak_gib": round(st["hbm_peak_bytes"] / 2**30, 3),
                   "points_per_s": round(sum(sizes) / wall, 1),
-                  "results_identical_sample": bool(same), "checked": len(check)},
+                  "results_identical_sample": bool(same), "checked": len(check),
+                  "saturation": {"requests": burst, "served_msms_per_s": round(burst / burst_s, 2),
+                                 "back_to_back_msms_per_s": round(burst / seq_s, 2),
+                                 "speedup_vs_back_to_back": round(seq_s / burst_s, 3),
+                                 "mean_batch": round(bst["mean_batch"], 3),
+                                 "p50_ms": round(bst["p50_ms"], 3), "p99_ms": round(bst["p99_ms"], 3),
+                                 "results_identical": bool(burst_same)}},
     }
 
 
@@ -334,6 +356,7 @@ def main():
     ap.add_argument("--slots", type=int, default=2, help="serve: engine slots")
     ap.add_argument("--max-batch", type=int, default=64, help="serve: requests per batch")
     ap.add_argument("--hbm-budget-gib", type=int, default=16, help="serve: admission budget")
+    ap.add_argument("--burst", type=int, default=300, help="serve: requests in the saturation phase")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-2-20", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))

@@ -68,6 +68,42 @@ def fq_mul(a, b):
     return t
 
 
+def fq_sqr(a):
+    """the squaring variant: doubled cross products once, then the squares"""
+    assert all(v < (1 << 29) for v in a)
+    assert val(a) * val(a) < P * R // 2
+    a2 = [2 * v for v in a]
+    m = [0] * 9
+    t = [0] * 9
+    acc = 0
+
+    def mac(x, y):
+        nonlocal acc
+        acc += x * y
+        assert acc < U64, "column overflow"
+
+    for k in range(17):
+        lo = max(0, k - 8)
+        for i in range(lo, 9):
+            if 2 * i < k and k - i <= 8:
+                mac(a2[i], a[k - i])
+        if k % 2 == 0 and k // 2 <= 8:
+            mac(a[k // 2], a[k // 2])
+        for j in range(lo, min(k, 9)):
+            mac(m[j], PL[k - j])
+        if k < 9:
+            m[k] = ((acc & 0xFFFFFFFF) * PINV) & MASK
+            mac(m[k], PL[0])
+            assert acc & MASK == 0
+            acc >>= 29
+        else:
+            t[k - 9] = acc & MASK
+            acc >>= 29
+    t[8] = acc
+    assert t == fq_mul(a, a)
+    return t
+
+
 def fq_add(a, b):
     r, c = [], 0
     for i in range(9):
@@ -120,13 +156,13 @@ def lt(a, bound):
 def dbl_core(X, Y):
     U = fq_add(Y, Y)
     lt(U, 8)
-    V = fq_mul(U, U)
+    V = fq_sqr(U)
     W = fq_mul(U, V)
     S = fq_mul(X, V)
-    xx = fq_mul(X, X)
+    xx = fq_sqr(X)
     M = fq_add(fq_add(xx, xx), xx)
     lt(M, 4.5)
-    X3 = fq_csub(2, fq_sub(3, fq_mul(M, M), fq_add(S, S)))
+    X3 = fq_csub(2, fq_sub(3, fq_sqr(M), fq_add(S, S)))
     lt(X3, 2.5)
     Y3 = fq_sub(2, fq_mul(M, fq_sub(3, S, X3)), fq_mul(W, Y))
     lt(Y3, 3.5)
@@ -143,10 +179,10 @@ def madd(acc, x2, y2, q_id):
     S2 = fq_mul(y2, ZZZ1)
     Pp = fq_sub(4, U2, X1)
     Rr = fq_sub(4, S2, Y1)
-    PP = fq_mul(Pp, Pp)
+    PP = fq_sqr(Pp)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(X1, PP)
-    RR = fq_mul(Rr, Rr)
+    RR = fq_sqr(Rr)
     dbl = iz2(PP) and iz2(RR) and not p_id and not q_id
     X3 = fq_csub(3, fq_sub(5, RR, fq_add(fq_add(Q, Q), PPP)))
     lt(X3, 3.5)
@@ -171,10 +207,10 @@ def add(acc, q):
     S2 = fq_mul(Y2, ZZZ1)
     Pp = fq_sub(2, U2, U1)
     Rr = fq_sub(2, S2, S1)
-    PP = fq_mul(Pp, Pp)
+    PP = fq_sqr(Pp)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(U1, PP)
-    RR = fq_mul(Rr, Rr)
+    RR = fq_sqr(Rr)
     dbl = iz2(PP) and iz2(RR) and not p_id and not q_id
     X3 = fq_csub(3, fq_sub(5, RR, fq_add(fq_add(Q, Q), PPP)))
     Y3 = fq_sub(2, fq_mul(Rr, fq_sub(4, Q, X3)), fq_mul(S1, PPP))
@@ -217,6 +253,8 @@ def test_mul_extremes():
     # all-ones limbs (limbs < 2^30 as after a carry-free add) at a safe value
     a = [MASK] * 8 + [0]
     fq_mul(a, worst(8))
+    for a in (worst(8), worst(5), worst(1), limbs(0), limbs(P - 1), [MASK] * 8 + [0]):
+        fq_sqr(a)
 
 
 def test_group_law_and_bounds():

@@ -48,8 +48,10 @@ def test_fq29_mul_matches_montgomery(engine):
     A = b"".join(fe(B.to_mont(x)) for x in a)
     Bb = b"".join(fe(B.to_mont(x)) for x in b)
     out = engine.test_fp(6, A, Bb)
+    sq = engine.test_fp(7, A, A)
     for i, (x, y) in enumerate(zip(a, b)):
         assert rd(out, i) == B.to_mont(x * y % B.P), i
+        assert rd(sq, i) == B.to_mont(x * x % B.P), i
 
 
 def test_fp_inv_neg_canon(engine):

@@ -153,12 +153,12 @@ __device__ __forceinline__ QXyzz qxyzz_select(bool c, const QXyzz& a, const QXyz
 // doubling core: X, Y < 4p.  Returns X3 < 2.5p, Y3 < 3.5p; V, W < 1.5p.
 __device__ __forceinline__ void qdbl_core(const Fq& X, const Fq& Y, Fq& X3, Fq& Y3, Fq& V, Fq& W) {
     const Fq U = fq_add(Y, Y);  // < 8p
-    V = fq_mul(U, U);
+    V = fq_sqr(U);
     W = fq_mul(U, V);
     const Fq S = fq_mul(X, V);
-    const Fq xx = fq_mul(X, X);
+    const Fq xx = fq_sqr(X);
     const Fq M = fq_add(fq_add(xx, xx), xx);                    // < 4.5p
-    X3 = fq_csub<2>(fq_sub<3>(fq_mul(M, M), fq_add(S, S)));     // < 4.5p -> < 2.5p
+    X3 = fq_csub<2>(fq_sub<3>(fq_sqr(M), fq_add(S, S)));        // < 4.5p -> < 2.5p
     Y3 = fq_sub<2>(fq_mul(M, fq_sub<3>(S, X3)), fq_mul(W, Y));  // < 3.5p
 }
 
@@ -186,10 +186,10 @@ __device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y
     const Fq S2 = fq_mul(y2, acc.ZZZ);
     const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
     const Fq R = fq_sub<4>(S2, acc.Y);
-    const Fq PP = fq_mul(P, P);
+    const Fq PP = fq_sqr(P);
     const Fq PPP = fq_mul(P, PP);
     const Fq Q = fq_mul(acc.X, PP);
-    const Fq RR = fq_mul(R, R);
+    const Fq RR = fq_sqr(R);
     const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
     QXyzz r;
     r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
@@ -207,40 +207,6 @@ __device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y
     acc = qxyzz_select(q_id, acc, r);
 }
 
-// Bucket-accumulation step acc += (x2, y2), (x2, y2) an affine point other
-// than (0,0).  The caller keeps "acc is the identity" in `id` instead of
-// testing ZZ each step; the rare cases acc == +-q take a branch instead of
-// being selected on every step.  acc is meaningful only while !id.
-__device__ __forceinline__ void qxyzz_madd_step(QXyzz& acc, bool& id, const Fq& x2, const Fq& y2) {
-    if (id) {
-        acc.X = x2;
-        acc.Y = y2;
-        acc.ZZ = fq_one();
-        acc.ZZZ = acc.ZZ;
-        id = false;
-        return;
-    }
-    const Fq U2 = fq_mul(x2, acc.ZZ);
-    const Fq S2 = fq_mul(y2, acc.ZZZ);
-    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
-    const Fq R = fq_sub<4>(S2, acc.Y);
-    const Fq PP = fq_mul(P, P);
-    const Fq PPP = fq_mul(P, PP);
-    const Fq Q = fq_mul(acc.X, PP);
-    const Fq RR = fq_mul(R, R);
-    if (__builtin_expect(fq_is_zero2(PP), 0)) {
-        // same x: acc == q doubles, acc == -q cancels to the identity
-        if (fq_is_zero2(RR)) acc = qxyzz_dbl_aff(x2, y2);
-        else id = true;
-        return;
-    }
-    const Fq X3 = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));  // < 3.5p
-    acc.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, X3)), fq_mul(acc.Y, PPP));  // < 3.5p
-    acc.X = X3;
-    acc.ZZ = fq_mul(acc.ZZ, PP);
-    acc.ZZZ = fq_mul(acc.ZZZ, PPP);
-}
-
 // acc += q (both XYZZ)
 __device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
     const bool p_id = qxyzz_is_id(acc);
@@ -251,10 +217,10 @@ __device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
     const Fq S2 = fq_mul(q.Y, acc.ZZZ);
     const Fq P = fq_sub<2>(U2, U1);  // < 3.5p
     const Fq R = fq_sub<2>(S2, S1);
-    const Fq PP = fq_mul(P, P);
+    const Fq PP = fq_sqr(P);
     const Fq PPP = fq_mul(P, PP);
     const Fq Q = fq_mul(U1, PP);
-    const Fq RR = fq_mul(R, R);
+    const Fq RR = fq_sqr(R);
     const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
     QXyzz r;
     r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));
