@@ -105,7 +105,9 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
 /* Per-size-class window widths: c_by_log2n[ceil(log2 n)] for 0 <= index < len
- * (<= 33 entries); 0 keeps the cost-model default for that class.  The plain
+ * (<= 33 entries); 0 keeps the cost-model default for that class.  An engine
+ * starts from the tables measured on MI355X (zkmetal_amd/csrc/tuned_gfx950.hpp);
+ * these calls replace them.  The plain
  * table applies to calls with one MSM, the batch table to each MSM of a call
  * with several (falling back to the plain table where it holds 0).  Tables
  * come from the autotuning sweep in zkmetal_amd/tune.py; a request server

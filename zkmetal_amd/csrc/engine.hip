@@ -18,6 +18,7 @@
 
 #include "gen.hpp"
 #include "kernels.hpp"
+#include "tuned_gfx950.hpp"
 
 namespace zk {
 
@@ -190,6 +191,8 @@ int32_t Engine::create(Engine** out) {
         return kGpuError;
     }
     e->acc_cap_ = accumulate_capacity(e->dev_);
+    memcpy(e->ctab_, kTunedPlain, sizeof(e->ctab_));
+    memcpy(e->ctab_batch_, kTunedBatch, sizeof(e->ctab_batch_));
     for (auto& ev : e->ev_) {
         if (hipEventCreate(&ev) != hipSuccess) {
             delete e;
@@ -383,28 +386,27 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     // host tail: Horner over windows, then affine normalisation
     auto t_tail0 = std::chrono::steady_clock::now();
     std::vector<zkh::Xyzz> res(nmsm);
+    // result = sum_w 2^(c w) (U_w + sum_{t < c-1} 2^t Q_{w,t}): one Horner over
+    // the bit positions b = c w + t (Q_{w,t} = Pl_t for t < g, else Ph_{t-g};
+    // U_w at t = 0), ~127 doublings and W c additions per MSM
     auto tail = [&](uint32_t jb, uint32_t je) {
         for (uint32_t j = jb; j < je; j++) {
             const MsmDesc& m = pl.msms[j];
-            const uint32_t cb = m.c - 1, g = cb / 2;
+            const uint32_t c = m.c, cb = c - 1, g = cb / 2;
             zkh::Xyzz acc = zkh::identity();
-            for (int w = (int)m.nwin - 1; w >= 0; w--) {
-                if (w != (int)m.nwin - 1)
-                    for (uint32_t b = 0; b < m.c; b++) acc = zkh::xdbl(acc);
-                // window sum: U + sum_{t < c-1} 2^t Q_t, Q_t = Pl_t (t < g) or Ph_{t-g}
+            for (int b = (int)(c * (m.nwin - 1) + cb - 1); b >= 0; b--) {
+                if (!zkh::is_id(acc)) acc = zkh::xdbl(acc);
+                const uint32_t w = (uint32_t)b / c, t = (uint32_t)b % c;
+                if (t >= cb) continue;  // bit c-1 of a window carries no plane
                 const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * 128;
-                zkh::Xyzz sw = zkh::identity();
-                for (int t = (int)cb - 1; t >= 0; t--) {
-                    sw = zkh::xdbl(sw);
-                    const uint32_t slot = (uint32_t)t < g ? 1 + t : 1 + g + (t - g);
-                    zkh::Xyzz q;
-                    memcpy(&q, bs + (uint64_t)slot * 128, 128);
-                    sw = zkh::xadd(sw, q);
+                const uint32_t slot = t < g ? 1 + t : 1 + g + (t - g);
+                zkh::Xyzz q;
+                memcpy(&q, bs + (uint64_t)slot * 128, 128);
+                acc = zkh::xadd(acc, q);
+                if (t == 0) {
+                    memcpy(&q, bs, 128);  // U_w
+                    acc = zkh::xadd(acc, q);
                 }
-                zkh::Xyzz u;
-                memcpy(&u, bs, 128);
-                sw = zkh::xadd(sw, u);
-                acc = zkh::xadd(acc, sw);
             }
             res[j] = acc;
         }

@@ -65,17 +65,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
-// 256-thread block: exclusive prefix of v, block total in *total
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds4,
+// Block of NW waves: exclusive prefix of v, block total in *total (lds: NW words)
+template <int NW = 4>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds,
                                                          uint32_t* total) {
     const uint32_t inc = wave_inclusive_scan(v);
     const uint32_t w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 63u) lds4[w] = inc;
+    if ((threadIdx.x & 63u) == 63u) lds[w] = inc;
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < 4; i++) {
-        uint32_t x = lds4[i];
+    for (uint32_t i = 0; i < (uint32_t)NW; i++) {
+        uint32_t x = lds[i];
         off += (i < w) ? x : 0u;
         tot += x;
     }
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SegDesc* __res
                                                             const uint32_t* __restrict__ tile_seg,
                                                             const void* __restrict__ in, int pass,
                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t cnt[kSortWaves][256];
     const uint32_t t = blockIdx.x;
     const SegDesc sg = segs[tile_seg[t]];
     const uint32_t tin = t - sg.tile0;
@@ -301,20 +302,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SegDesc* __res
     const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
     const uint32_t shift = pass ? 8u : 0u;
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
-#pragma unroll
-    for (int i = 0; i < 4; i++) cnt[i][threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < kSortWaves * 256u; i += kSortThreads) (&cnt[0][0])[i] = 0;
     __syncthreads();
     for (int r = 0; r < kSortItems; r++) {
-        const uint32_t pos = w * 1024u + r * 64u + lane;
+        const uint32_t pos = w * (64u * kSortItems) + r * 64u + lane;
         const bool act = pos < len;
         const uint32_t digit = act ? (sort_key(in, pass, sg.ent0 + start + pos) >> shift) & 0xffu : 0u;
         const uint64_t peers = match_digit8(digit, act);
         if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] += (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    const uint32_t d = threadIdx.x;
-    hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin] =
-        cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
+    if (threadIdx.x < 256) {
+        const uint32_t d = threadIdx.x;
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kSortWaves; i++) s += cnt[i][d];
+        hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin] = s;
+    }
 }
 
 // Exclusive scan of each segment's 256*ntiles counts into global destinations
@@ -415,10 +419,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, int pass,
     const uint32_t* __restrict__ hist, uint16_t* __restrict__ out_keys,
     uint32_t* __restrict__ out_vals) {
-    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t cnt[kSortWaves][256];
     __shared__ uint32_t dstart[256];
     __shared__ uint32_t goff[256];
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t ldsw[kSortWaves];
     __shared__ uint16_t lkey[kSortTile];
     __shared__ uint32_t lval[kSortTile];
     const uint32_t t = blockIdx.x;
@@ -428,13 +432,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
     const uint32_t shift = pass ? 8u : 0u;
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
-#pragma unroll
-    for (int i = 0; i < 4; i++) cnt[i][threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < kSortWaves * 256u; i += kSortThreads) (&cnt[0][0])[i] = 0;
     __syncthreads();
     uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        const uint32_t pos = w * 1024u + r * 64u + lane;
+        const uint32_t pos = w * (64u * kSortItems) + r * 64u + lane;
         const bool act = pos < len;
         key[r] = ZK_KEY_NONE;
         val[r] = 0;
@@ -457,21 +460,31 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
     __syncthreads();
     {
+        // digit totals by threads 0..255; per-wave starts within each digit
         const uint32_t d = threadIdx.x;
-        const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+        uint32_t s = 0;
+        if (d < 256) {
+#pragma unroll
+            for (int i = 0; i < kSortWaves; i++) s += cnt[i][d];
+        }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(c0 + c1 + c2 + c3, lds4, &tot);
-        dstart[d] = ex;
-        cnt[0][d] = ex;
-        cnt[1][d] = ex + c0;
-        cnt[2][d] = ex + c0 + c1;
-        cnt[3][d] = ex + c0 + c1 + c2;
-        goff[d] = hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin];
+        const uint32_t ex = block_exclusive_scan<kSortWaves>(s, ldsw, &tot);
+        if (d < 256) {
+            dstart[d] = ex;
+            uint32_t run = ex;
+#pragma unroll
+            for (int i = 0; i < kSortWaves; i++) {
+                const uint32_t c = cnt[i][d];
+                cnt[i][d] = run;
+                run += c;
+            }
+            goff[d] = hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        const uint32_t pos = w * 1024u + r * 64u + lane;
+        const uint32_t pos = w * (64u * kSortItems) + r * 64u + lane;
         if (pos < len) {
             const uint32_t j = cnt[w][(key[r] >> shift) & 0xffu] + rank[r];
             lkey[j] = (uint16_t)key[r];
