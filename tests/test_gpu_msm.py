@@ -130,6 +130,30 @@ def test_repeated_and_cancelling_bases(engine, srs):
     assert result_affine(engine.msm(pts2, sc2)) is None
 
 
+def test_identity_bases_scattered(engine, srs):
+    """(0,0) bases anywhere: a whole 64-point bitmap word, single ones on word
+    boundaries and at the last index; in one MSM and in a batch whose parts
+    start at offsets that are not multiples of 64."""
+    bases, ks = srs
+    n = 5000
+    rnd = random.Random(77)
+    ident = set(range(128, 192)) | {0, 63, 64, 65, 1000, n - 1}
+    ident |= {rnd.randrange(n) for _ in range(200)}
+    pts = bytearray(bases[:64 * n])
+    for i in ident:
+        pts[64 * i:64 * i + 64] = bytes(64)
+    sc = zk.gen_scalars(0x1D, n, 0)
+    ints = scalars_to_ints(sc)
+    kk = [0 if i in ident else ks[i] for i in range(n)]
+    assert result_affine(engine.msm(bytes(pts), sc)) == closed_form(kk, ints)
+    counts = [37, 1000, n - 1037]
+    res = engine.msm_batch(bytes(pts), sc.tobytes(), counts)
+    off = 0
+    for j, c in enumerate(counts):
+        assert result_affine(res[j]) == closed_form(kk[off:off + c], ints[off:off + c]), j
+        off += c
+
+
 def test_resident_batch_mixed_sizes(engine, srs):
     bases, ks = srs
     rnd = random.Random(12)

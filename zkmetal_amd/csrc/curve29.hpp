@@ -4,9 +4,9 @@
 // identity), with lazy reduction.  Bounds, in multiples of p:
 //   products < 1.5p (inputs < 8p);  stored X, Y < 4p;  stored ZZ, ZZZ < 2p;
 //   affine table coordinates < p (canonical).
-// A stored value is < 4p < 2^256, so a point is kept in global memory as
-// 256-bit integers (8 x 32-bit words), the same 64 B / 128 B layouts as
-// curve.hpp, and re-sliced into 29-bit limbs on load.
+// A stored XYZZ value is < 4p < 2^256, so XYZZ points are kept in global
+// memory as 256-bit integers (8 x 32-bit words, 128 B as in curve.hpp) and
+// re-sliced into 29-bit limbs on load; base-table entries keep their limbs.
 #pragma once
 #include "curve.hpp"
 #include "field29.hpp"
@@ -254,19 +254,44 @@ __device__ __forceinline__ QXyzz qxyzz_from_xyzz(const Xyzz& a) {
     return r;
 }
 
-// ---- global memory: values as 256-bit integers, as in curve.hpp
-__device__ __forceinline__ QAff load_qaff(const uint4* base, uint64_t idx) {
-    const uint4* p = base + idx * 4;
+// ---- global memory.  Base-table entries: one 128-byte line holding the
+// limbs of x, y and p - y (words 0-8, 9-17, 18-26; 27-31 zero), canonical,
+// so a gather needs no re-slicing and a negative digit just takes p - y.
+// XYZZ points: 256-bit integers (< 4p < 2^256), 128 B, as in curve.hpp.
+// Table entry, 128 B: words 0-7 x[0..7], 8-15 y[0..7], 16-23 (p-y)[0..7],
+// 24-25 (x[8], y[8]), 26-27 (x[8], (p-y)[8]), 28-31 zero.  A load for either
+// sign is two 32-byte pieces and one 8-byte piece of the same 128-byte line.
+__device__ __forceinline__ QAff load_qtab(const uint4* base, uint64_t idx, bool neg) {
+    const uint4* p = base + idx * 8;
+    const uint4* py = p + (neg ? 4 : 2);
+    const uint4 x0 = p[0], x1 = p[1], y0 = py[0], y1 = py[1];
+    const uint2 h = reinterpret_cast<const uint2*>(p)[neg ? 13 : 12];
     QAff a;
-    a.x = fq_slice(load_fp(p));
-    a.y = fq_slice(load_fp(p + 2));
+    a.x.v[0] = x0.x; a.x.v[1] = x0.y; a.x.v[2] = x0.z; a.x.v[3] = x0.w;
+    a.x.v[4] = x1.x; a.x.v[5] = x1.y; a.x.v[6] = x1.z; a.x.v[7] = x1.w;
+    a.y.v[0] = y0.x; a.y.v[1] = y0.y; a.y.v[2] = y0.z; a.y.v[3] = y0.w;
+    a.y.v[4] = y1.x; a.y.v[5] = y1.y; a.y.v[6] = y1.z; a.y.v[7] = y1.w;
+    a.x.v[8] = h.x;
+    a.y.v[8] = h.y;
     return a;
 }
 
-__device__ __forceinline__ void store_qaff(uint4* base, uint64_t idx, const QAff& a) {
-    uint4* p = base + idx * 4;
-    store_fp(p, fq_pack(a.x));
-    store_fp(p + 2, fq_pack(a.y));
+__device__ __forceinline__ void store_qtab(uint4* base, uint64_t idx, const QAff& a) {
+    uint32_t w[32];
+    const Fq ny = fq_csub<1>(fq_sub<1>(fq_zero(), a.y));  // p - y; 0 stays 0
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        w[i] = a.x.v[i];
+        w[8 + i] = a.y.v[i];
+        w[16 + i] = ny.v[i];
+    }
+    w[24] = a.x.v[8]; w[25] = a.y.v[8];
+    w[26] = a.x.v[8]; w[27] = ny.v[8];
+#pragma unroll
+    for (int i = 28; i < 32; i++) w[i] = 0;
+    uint4* p = base + idx * 8;
+#pragma unroll
+    for (int i = 0; i < 8; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 __device__ __forceinline__ QXyzz load_qxyzz(const uint4* base, uint64_t idx) {

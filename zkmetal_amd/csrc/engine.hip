@@ -238,7 +238,8 @@ static DescLayout layout_for(const Plan& pl) {
 
 static constexpr uint32_t kRunLimit = 8;
 
-int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96) {
+int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
+                    uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
     const uint32_t nmsm = (uint32_t)pl.msms.size();
     const uint32_t nseg = (uint32_t)pl.segs.size();
@@ -294,7 +295,8 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
         launch_fill_zero(err_.p, 16, st);
         ZK_TRY(mark(0));
-        launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, pl.scalars, digits_.p, d_err, st);
+        launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, d_table, table_n, pl.scalars,
+                         digits_.p, d_err, st);
         ZK_TRY(mark(1));
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
                          nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
@@ -334,6 +336,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
             key.push_back(m.c);
             key.push_back(m.table0);
         }
+        key.push_back(table_n);  // places the table's identity bitmap
         for (const void* p : std::initializer_list<const void*>{d_scalars, d_table, desc_.p, hdesc_.p, digits_.p, keysA_.p, valsA_.p,
                               keysB_.p, valsB_.p, hist_.p, part_.p, segvalid_.p, buckets_.p,
                               recAk_.p, recAp_.p, recBk_.p, rc_.p, winsum_.p, err_.p, hwin_.p})
@@ -486,13 +489,13 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * 64);
-    ZK_ALLOC(ptable_, total * 128);
+    ZK_ALLOC(ptable_, table_bytes(total));
     ZK_ALLOC(scalars_, total * 32);
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * 64, hipMemcpyHostToDevice, stream_));
     ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
     launch_build_table(pts_in_.p, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
-    rc = run(pl, scalars_.p, ptable_.p, res.data());
+    rc = run(pl, scalars_.p, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
     for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
     return kOk;
@@ -519,10 +522,10 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     Plan pl;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
     if (rc != kOk) return rc;
-    ZK_ALLOC(ptable_, total * 128);
+    ZK_ALLOC(ptable_, table_bytes(total));
     launch_build_table(d_points, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
-    rc = run(pl, d_scalars, ptable_.p, res.data());
+    rc = run(pl, d_scalars, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
     for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
     return kOk;
@@ -531,7 +534,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
 int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     if (!points || n == 0) return kInvalidInput;
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
-    ZK_ALLOC(srs_, (uint64_t)n * 128);
+    ZK_ALLOC(srs_, table_bytes(n));
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
     launch_build_table(pts_in_.p, srs_.p, n, stream_);
     ZK_TRY(hipStreamSynchronize(stream_));
@@ -579,7 +582,7 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
     if (n == 0 || n >= (1ull << 31)) return kInvalidInput;
     srs_n_ = 0;
     ZK_ALLOC(pts_in_, n * 64);
-    ZK_ALLOC(srs_, n * 128);
+    ZK_ALLOC(srs_, table_bytes(n));
     ZK_ALLOC(err_, 16);
     ZK_TRY(hipMemcpyAsync(pts_in_.p, g1, n * 64, hipMemcpyHostToDevice, stream_));
     launch_fill_zero(err_.p, 16, stream_);
@@ -632,7 +635,7 @@ int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
     std::vector<uint8_t> ks((uint64_t)n * 32);
     zkh::gen_scalars(seed, n, 0, ks.data());
     ZK_ALLOC(tmp_, (uint64_t)n * 32);
-    ZK_ALLOC(srs_, (uint64_t)n * 128);
+    ZK_ALLOC(srs_, table_bytes(n));
     ZK_TRY(hipMemcpyAsync(tmp_.p, ks.data(), ks.size(), hipMemcpyHostToDevice, stream_));
     launch_gen_bases(tmp_.p, gtab_.p, n, srs_.p, stream_);
     ZK_TRY(hipStreamSynchronize(stream_));
@@ -724,7 +727,7 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
         d_sc = scalars_.p;
     }
     std::vector<uint8_t> res(ns.size() * 96);
-    rc = run(pl, d_sc, table, res.data());
+    rc = run(pl, d_sc, table, table_n, res.data());
     if (rc != kOk) return rc;
     for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
     return kOk;

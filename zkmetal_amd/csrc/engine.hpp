@@ -158,7 +158,8 @@ class Engine {
     int32_t probe_copy(uint64_t bytes, double* ms_out);
 
     // run a planned batch: scalars and table already on the device
-    int32_t run(const Plan& pl, const void* d_scalars, const void* d_table, uint8_t* out96);
+    int32_t run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
+                uint8_t* out96);
 
   private:
     Engine() = default;

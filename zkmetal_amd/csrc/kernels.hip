@@ -43,6 +43,12 @@ __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n,
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
+// Sort key of a bucket index x < 2^15: low 7 bits in byte 0, the rest in
+// byte 1.  A real key's low byte is then <= 0x7f, so the first radix pass
+// puts every "no bucket" entry (0xFFFF) in bin 0xFF, alone, and can drop them.
+__device__ __forceinline__ uint32_t key_encode(uint32_t x) { return (x & 0x7fu) | ((x >> 7) << 8); }
+__device__ __forceinline__ uint32_t key_decode(uint32_t k) { return (k & 0x7fu) | ((k >> 8) << 7); }
+
 // mask of active lanes whose 8-bit digit equals this lane's
 __device__ __forceinline__ uint64_t match_digit8(uint32_t digit, bool active) {
     uint64_t peers = __ballot(active);
@@ -148,16 +154,29 @@ __device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
 
 // -------------------------------------------------------- base table / phi
 // table[2i] = P_i, table[2i+1] = phi(P_i) = (beta x, y): affine, coordinates
-// in the 2^261 Montgomery form, canonical, as 256-bit integers.  (0,0) maps
-// to (0,0).
+// in the 2^261 Montgomery form, canonical, as 29-bit limbs with p - y beside y
+// (store_qtab, 128 B per entry).  (0,0) maps to (0,0).  After the n pairs
+// (n * 256 bytes) comes a bitmap, bit i set when P_i is the identity: the
+// decomposition gives such a point no digits, so the accumulation never adds
+// an identity base.
 
-__device__ __forceinline__ void store_table_pair(uint4* table, uint64_t i, const Aff& a) {
+// one 64-bit bitmap word per wave; the wave covers points [64k, 64k + 64)
+// (256-thread blocks over consecutive points) and lane 0 is active whenever
+// any lane is, since lanes past n leave before this call
+__device__ __forceinline__ void store_id_bit(uint64_t* idmask, uint64_t i, bool is_id) {
+    const uint64_t m = __ballot(is_id);
+    if (__lane_id() == 0) idmask[i >> 6] = m;
+}
+
+__device__ __forceinline__ void store_table_pair(uint4* table, uint64_t* idmask, uint64_t i,
+                                                 const Aff& a) {
     QAff q;
     q.x = fq_from_fp_canon(a.x);
     q.y = fq_from_fp_canon(a.y);
-    store_qaff(table, 2 * i, q);
+    store_id_bit(idmask, i, qaff_is_id(q));
+    store_qtab(table, 2 * i, q);
     q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
-    store_qaff(table, 2 * i + 1, q);
+    store_qtab(table, 2 * i + 1, q);
 }
 
 // SRS import from an SRS file's G1 section (Sources/zkMetal/KZG/
@@ -191,6 +210,7 @@ __device__ __forceinline__ bool fq_eq(const Fq& a, const Fq& b) {
 
 __global__ __launch_bounds__(256) void k_import_be(const uint8_t* __restrict__ raw, uint64_t n,
                                                    uint4* __restrict__ table,
+                                                   uint64_t* __restrict__ idmask,
                                                    uint32_t* __restrict__ bad) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -213,9 +233,10 @@ __global__ __launch_bounds__(256) void k_import_be(const uint8_t* __restrict__ r
         ok = ok && fq_eq(lhs, rhs);
     }
     if (!ok) atomicAdd(bad, 1u);
-    store_qaff(table, 2 * i, q);
+    store_id_bit(idmask, i, qaff_is_id(q));
+    store_qtab(table, 2 * i, q);
     q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
-    store_qaff(table, 2 * i + 1, q);
+    store_qtab(table, 2 * i + 1, q);
 }
 
 // table entry 2i back to the 2^256 Montgomery form (64 B per point)
@@ -223,7 +244,7 @@ __global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ 
                                                       uint4* __restrict__ pts) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    QAff q = load_qaff(table, 2 * i);
+    QAff q = load_qtab(table, 2 * i, false);
     Aff a;
     a.x = fq_to_fp(q.x);
     a.y = fq_to_fp(q.y);
@@ -231,13 +252,14 @@ __global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ pts,
-                                                     uint4* __restrict__ table, uint64_t n) {
+                                                     uint4* __restrict__ table,
+                                                     uint64_t* __restrict__ idmask, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Aff a = load_aff(pts, i);
     a.x = fp_canon(a.x);
     a.y = fp_canon(a.y);
-    store_table_pair(table, i, a);
+    store_table_pair(table, idmask, i, a);
 }
 
 // --------------------------------------------------- GLV + signed digits
@@ -246,6 +268,7 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
                                                    const uint64_t* __restrict__ msm_scalar0,
                                                    uint32_t nmsm, const SegDesc* __restrict__ segs,
                                                    const uint4* __restrict__ scalars,
+                                                   const uint64_t* __restrict__ idmask,
                                                    uint64_t total, uint2* __restrict__ digits,
                                                    uint32_t* __restrict__ err) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,6 +279,11 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
     uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
     uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
     if (m.form == 1) fr_from_mont(k);
+    const uint64_t pt = (m.table0 >> 1) + i;  // the base's point index in the table
+    if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) k[j] = 0;  // identity base: no digits
+    }
     GlvSplit sp = glv_split(k);
     const uint32_t c = m.c, half = 1u << (c - 1), full = 1u << c;
     uint32_t carry1 = 0, carry2 = 0;
@@ -267,8 +295,8 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
         carry2 = d2 > half;
         if (carry1) { d1 = full - d1; neg1 ^= 1u; }
         if (carry2) { d2 = full - d2; neg2 ^= 1u; }
-        uint32_t w1 = d1 ? ((d1 - 1u) | (neg1 << 31)) : ZK_KEY_NONE;
-        uint32_t w2 = d2 ? ((d2 - 1u) | (neg2 << 31)) : ZK_KEY_NONE;
+        uint32_t w1 = d1 ? (key_encode(d1 - 1u) | (neg1 << 31)) : ZK_KEY_NONE;
+        uint32_t w2 = d2 ? (key_encode(d2 - 1u) | (neg2 << 31)) : ZK_KEY_NONE;
         const uint64_t e = segs[m.seg0 + w].ent0 + 2 * i;
         digits[e >> 1] = make_uint2(w1, w2);
     }
@@ -290,26 +318,35 @@ __device__ __forceinline__ uint32_t sort_key(const void* in, int pass, uint64_t 
     return reinterpret_cast<const uint16_t*>(in)[pos];
 }
 
+// Entries of a tile: pass 0 sorts all 2n of the segment; pass 1 only the
+// seg_valid[s] entries with a bucket (pass 0 put them first).
+__device__ __forceinline__ uint32_t tile_len(const SegDesc& sg, uint32_t start, int pass,
+                                             const uint32_t* seg_valid, uint32_t s) {
+    const uint32_t n = pass ? seg_valid[s] : sg.len;
+    return start < n ? min((uint32_t)kSortTile, n - start) : 0u;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SegDesc* __restrict__ segs,
                                                             const uint32_t* __restrict__ tile_seg,
                                                             const void* __restrict__ in, int pass,
+                                                            const uint32_t* __restrict__ seg_valid,
                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[kSortWaves][256];
     const uint32_t t = blockIdx.x;
-    const SegDesc sg = segs[tile_seg[t]];
+    const uint32_t s = tile_seg[t];
+    const SegDesc sg = segs[s];
     const uint32_t tin = t - sg.tile0;
     const uint32_t start = tin * kSortTile;
-    const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
+    const uint32_t len = tile_len(sg, start, pass, seg_valid, s);
     const uint32_t shift = pass ? 8u : 0u;
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
     for (uint32_t i = threadIdx.x; i < kSortWaves * 256u; i += kSortThreads) (&cnt[0][0])[i] = 0;
     __syncthreads();
+    // counts only (no ranks): one LDS atomic per entry into the wave's own
+    // counters is far cheaper than matching equal digits with ballots
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t pos = w * (64u * kSortItems) + r * 64u + lane;
-        const bool act = pos < len;
-        const uint32_t digit = act ? (sort_key(in, pass, sg.ent0 + start + pos) >> shift) & 0xffu : 0u;
-        const uint64_t peers = match_digit8(digit, act);
-        if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] += (uint32_t)__popcll(peers);
+        if (pos < len) atomicAdd(&cnt[w][(sort_key(in, pass, sg.ent0 + start + pos) >> shift) & 0xffu], 1u);
     }
     __syncthreads();
     if (threadIdx.x < 256) {
@@ -401,7 +438,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const SegDesc* __restrict__ s
         uint32_t o[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            if (pass == 1 && i0 + i == none_bin) seg_valid[s] = acc;
+            if (pass == 0 && i0 + i == none_bin) seg_valid[s] = acc;
             o[i] = (uint32_t)sg.ent0 + acc;
             acc += v[i];
         }
@@ -411,14 +448,15 @@ __global__ __launch_bounds__(256) void k_scan_down(const SegDesc* __restrict__ s
     }
 }
 
-// Scatter: rank every entry of the tile (wave-private counters, as in
-// k_sort_hist), sort the tile in LDS, then write each digit's run to its
-// global destination with consecutive threads writing consecutive entries.
+// Scatter: rank every entry of the tile (wave-private counters; stable ranks
+// from ballot matching in the second pass), sort the tile in LDS, then write
+// each digit's run to its global destination with consecutive threads
+// writing consecutive entries.
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const SegDesc* __restrict__ segs, const uint32_t* __restrict__ tile_seg,
     const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, int pass,
-    const uint32_t* __restrict__ hist, uint16_t* __restrict__ out_keys,
-    uint32_t* __restrict__ out_vals) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ seg_valid,
+    uint16_t* __restrict__ out_keys, uint32_t* __restrict__ out_vals) {
     __shared__ uint32_t cnt[kSortWaves][256];
     __shared__ uint32_t dstart[256];
     __shared__ uint32_t goff[256];
@@ -426,10 +464,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __shared__ uint16_t lkey[kSortTile];
     __shared__ uint32_t lval[kSortTile];
     const uint32_t t = blockIdx.x;
-    const SegDesc sg = segs[tile_seg[t]];
+    const uint32_t s = tile_seg[t];
+    const SegDesc sg = segs[s];
     const uint32_t tin = t - sg.tile0;
     const uint32_t start = tin * kSortTile;
-    const uint32_t len = min((uint32_t)kSortTile, sg.len - start);
+    const uint32_t len = tile_len(sg, start, pass, seg_valid, s);
     const uint32_t shift = pass ? 8u : 0u;
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
     for (uint32_t i = threadIdx.x; i < kSortWaves * 256u; i += kSortThreads) (&cnt[0][0])[i] = 0;
@@ -453,10 +492,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
             }
         }
         const uint32_t digit = (key[r] >> shift) & 0xffu;
-        const uint64_t peers = match_digit8(digit, act);
-        const uint32_t base = cnt[w][digit];
-        rank[r] = base + (uint32_t)__popcll(peers & lanemask_lt());
-        if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] = base + (uint32_t)__popcll(peers);
+        if (pass == 0) {
+            // the first pass need not be stable (the second, stable pass fixes
+            // the final order): ranks from LDS atomics, no digit matching
+            rank[r] = act ? atomicAdd(&cnt[w][digit], 1u) : 0u;
+        } else {
+            const uint64_t peers = match_digit8(digit, act);
+            const uint32_t base = cnt[w][digit];
+            rank[r] = base + (uint32_t)__popcll(peers & lanemask_lt());
+            if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] = base + (uint32_t)__popcll(peers);
+        }
     }
     __syncthreads();
     {
@@ -496,7 +541,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         const uint32_t k = lkey[j];
         const uint32_t d = (k >> shift) & 0xffu;
         const uint32_t g = goff[d] + (j - dstart[d]);
-        if (pass == 0 || k != ZK_KEY_NONE) {
+        if (k != ZK_KEY_NONE) {  // entries without a bucket are not kept
             out_keys[g] = (uint16_t)k;
             out_vals[g] = lval[j];
         }
@@ -543,22 +588,22 @@ __global__ __launch_bounds__(256) void k_accumulate(
         rec_keys[2 * L + 1] = ZK_REC_INVALID | ZK_REC_KEYMASK;
         return;
     }
-    const uint4* tb = table + msms[sg.msm].table0 * 4;
+    const uint4* tb = table + msms[sg.msm].table0 * 8;  // 128-byte entries
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
     QXyzz acc = qxyzz_identity();
     // software pipeline: the next entry's key, value and base are loaded
-    // before the current mixed addition, hiding the random 64-byte gather
+    // before the current mixed addition, hiding the random 128-byte gather
     uint32_t k = cur, v = vals[b];
-    QAff q = load_qaff(tb, v & 0x7fffffffu);
+    QAff q = load_qtab(tb, v & 0x7fffffffu, v >> 31);
     for (uint64_t p = b; p < e; p++) {
         const uint64_t pn = min(p + 1, e - 1);
         const uint32_t kn = keys[pn], vn = vals[pn];
-        const QAff qn = load_qaff(tb, vn & 0x7fffffffu);
+        const QAff qn = load_qtab(tb, vn & 0x7fffffffu, vn >> 31);
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
-            const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
+            const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
             if (first && head_open) {
                 put_record(rec_keys, rec_pts, 2 * L, gk, true, acc);
             } else {
@@ -572,15 +617,15 @@ __global__ __launch_bounds__(256) void k_accumulate(
         // exceptional cases (identity, doubling) by selects inside the
         // addition: a branchy form costs ~13% more VALU instructions per step
         // (register moves at the joins), measured with SQ_INSTS_VALU
-        const bool q_id = qaff_is_id(q);
-        if (v >> 31) q.y = fq_sub<1>(fq_zero(), q.y);  // p - y
-        qxyzz_madd(acc, q.x, q.y, q_id);
+        // y is already negated for a negative digit; identity bases have no
+        // digits (table bitmap), so q is never the identity
+        qxyzz_madd(acc, q.x, q.y, false);
         k = kn;
         v = vn;
         q = qn;
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
-    const uint32_t gk = (uint32_t)(sg.bkt0 + cur);
+    const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
     if (first) {
         const bool open = head_open || tail_open;
         if (!open) store_qxyzz(buckets, gk, acc);
@@ -824,7 +869,8 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
 
 __global__ __launch_bounds__(256) void k_gen_bases(const uint4* __restrict__ ks,
                                                    const uint4* __restrict__ gtab, uint64_t n,
-                                                   uint4* __restrict__ table) {
+                                                   uint4* __restrict__ table,
+                                                   uint64_t* __restrict__ idmask) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint4 s0 = ks[2 * i], s1 = ks[2 * i + 1];
@@ -846,7 +892,7 @@ __global__ __launch_bounds__(256) void k_gen_bases(const uint4* __restrict__ ks,
         a.x = fp_mul(acc.X, fp_mul(t, acc.ZZZ));
         a.y = fp_mul(acc.Y, fp_mul(t, acc.ZZ));
     }
-    store_table_pair(table, i, a);
+    store_table_pair(table, idmask, i, a);
 }
 
 // ------------------------------------------------------ test entry points
@@ -1037,10 +1083,14 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) {
     return (unsigned)((n + bs - 1) / bs);
 }
 
+static inline uint64_t* table_idmask(void* table, uint64_t n) {
+    return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) + n * 256);
+}
+
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_import_be, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint8_t*)raw, n, (uint4*)table, bad);
+                       (const uint8_t*)raw, n, (uint4*)table, table_idmask(table, n), bad);
 }
 
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st) {
@@ -1052,15 +1102,18 @@ void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t s
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_build_table, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint4*)pts, (uint4*)table, n);
+                       (const uint4*)pts, (uint4*)table, table_idmask(table, n), n);
 }
 
 void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,
-                      const SegDesc* segs, const void* scalars, uint64_t total, void* digits,
-                      uint32_t* err, hipStream_t st) {
+                      const SegDesc* segs, const void* scalars, const void* table,
+                      uint64_t table_n, uint64_t total, void* digits, uint32_t* err,
+                      hipStream_t st) {
     if (!total) return;
     hipLaunchKernelGGL(k_decompose, dim3(blocks_for(total, 256)), dim3(256), 0, st, msms,
-                       msm_scalar0, nmsm, segs, (const uint4*)scalars, total, (uint2*)digits, err);
+                       msm_scalar0, nmsm, segs, (const uint4*)scalars,
+                       (const uint64_t*)table_idmask(const_cast<void*>(table), table_n), total,
+                       (uint2*)digits, err);
 }
 
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
@@ -1070,14 +1123,15 @@ void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_s
                       uint32_t* out_vals, hipStream_t st) {
     if (!ntiles) return;
     hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
-                       in_keys, pass, hist);
+                       in_keys, pass, (const uint32_t*)seg_valid, hist);
     hipLaunchKernelGGL(k_scan_up, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg,
                        (const uint32_t*)hist, part);
     hipLaunchKernelGGL(k_scan_spine, dim3(nseg), dim3(256), 0, st, segs, part);
     hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg, hist,
                        (const uint32_t*)part, pass, seg_valid);
     hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
-                       in_keys, in_vals, pass, (const uint32_t*)hist, out_keys, out_vals);
+                       in_keys, in_vals, pass, (const uint32_t*)hist, (const uint32_t*)seg_valid,
+                       out_keys, out_vals);
 }
 
 void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st) {
@@ -1142,7 +1196,8 @@ void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_gen_bases, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint4*)ks, (const uint4*)gtab, n, (uint4*)table);
+                       (const uint4*)ks, (const uint4*)gtab, n, (uint4*)table,
+                       table_idmask(table, n));
 }
 
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,

@@ -7,12 +7,17 @@
 
 namespace zk {
 
+// A table of n points takes table_bytes(n): n pairs of 128-byte entries, then
+// a bitmap with bit i set when point i is the identity (written by the table
+// builders, read by the decomposition).
+inline uint64_t table_bytes(uint64_t n) { return n * 256 + (n + 63) / 64 * 8; }
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);
 void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,
-                      const SegDesc* segs, const void* scalars, uint64_t total, void* digits,
-                      uint32_t* err, hipStream_t st);
+                      const SegDesc* segs, const void* scalars, const void* table,
+                      uint64_t table_n, uint64_t total, void* digits, uint32_t* err,
+                      hipStream_t st);
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
                       uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
                       const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,

@@ -3,7 +3,7 @@
 // A launch processes a batch of MSMs.  Each MSM j has n_j scalars, window
 // width c_j and W_j = ceil(128 / c_j) signed windows; its GLV half-scalars
 // address a base table laid out as [P_0, phi(P_0), P_1, phi(P_1), ...]
-// (64-byte affine Montgomery points), so "entry" 2i+h of an MSM is base i
+// (affine, 128 bytes per entry), so "entry" 2i+h of an MSM is base i
 // (h = 0) or its endomorphism image (h = 1).  Every (MSM, window) pair is a
 // segment: 2 n_j digit entries that are sorted by bucket, accumulated into
 // 2^(c_j - 1) buckets and reduced to one window sum.
