@@ -10,6 +10,7 @@
 #pragma once
 #include "curve.hpp"
 #include "field29.hpp"
+#include "plan.hpp"
 
 namespace zk {
 
@@ -262,6 +263,7 @@ __device__ __forceinline__ QXyzz qxyzz_from_xyzz(const Xyzz& a) {
 // 24-25 (x[8], y[8]), 26-27 (x[8], (p-y)[8]), 28-31 zero.  A load for either
 // sign is two 32-byte pieces and one 8-byte piece of the same 128-byte line.
 __device__ __forceinline__ QAff load_qtab(const uint4* base, uint64_t idx, bool neg) {
+    static_assert(kTableEntryBytes == 128, "entry layout");
     const uint4* p = base + idx * 8;
     const uint4* py = p + (neg ? 4 : 2);
     const uint4 x0 = p[0], x1 = p[1], y0 = py[0], y1 = py[1];

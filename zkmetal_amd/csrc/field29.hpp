@@ -46,6 +46,14 @@ __device__ __forceinline__ void qmac(uint64_t& acc, uint32_t x, uint32_t y) {
     acc += (uint64_t)x * y;
 }
 
+// An empty asm on the column carry (no instruction is emitted).  Without it
+// the compiler's reassociation sums each column's products from zero, apart
+// from the carry, and adds the carry with a separate 64-bit add plus register
+// moves; with it the carry opens the next column's multiply-add chain.
+// Measured in the bucket accumulation: 7% fewer VALU instructions per mixed
+// addition, 3% less time.
+__device__ __forceinline__ void carry_fence(uint64_t& acc) { asm("" : "+v"(acc)); }
+
 // a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
 // a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
 __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
@@ -63,6 +71,7 @@ __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
         m[k] = ((uint32_t)acc * ZQ_PINV) & ZQ_MASK;
         qmac(acc, m[k], q_p(0));
         acc >>= 29;
+        carry_fence(acc);
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
@@ -73,6 +82,7 @@ __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
         }
         t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
         acc >>= 29;
+        carry_fence(acc);
     }
     t.v[8] = (uint32_t)acc;
     return t;
@@ -98,6 +108,7 @@ __device__ __forceinline__ Fq fq_sqr(const Fq& a) {
         m[k] = ((uint32_t)acc * ZQ_PINV) & ZQ_MASK;
         qmac(acc, m[k], q_p(0));
         acc >>= 29;
+        carry_fence(acc);
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
@@ -108,6 +119,7 @@ __device__ __forceinline__ Fq fq_sqr(const Fq& a) {
         for (int j = k - 8; j < 9; j++) qmac(acc, m[j], q_p(k - j));
         t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
         acc >>= 29;
+        carry_fence(acc);
     }
     t.v[8] = (uint32_t)acc;
     return t;

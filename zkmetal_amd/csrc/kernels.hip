@@ -155,8 +155,8 @@ __device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
 // -------------------------------------------------------- base table / phi
 // table[2i] = P_i, table[2i+1] = phi(P_i) = (beta x, y): affine, coordinates
 // in the 2^261 Montgomery form, canonical, as 29-bit limbs with p - y beside y
-// (store_qtab, 128 B per entry).  (0,0) maps to (0,0).  After the n pairs
-// (n * 256 bytes) comes a bitmap, bit i set when P_i is the identity: the
+// (store_qtab, kTableEntryBytes per entry).  (0,0) maps to (0,0).  After the
+// n pairs comes a bitmap, bit i set when P_i is the identity: the
 // decomposition gives such a point no digits, so the accumulation never adds
 // an identity base.
 
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
         rec_keys[2 * L + 1] = ZK_REC_INVALID | ZK_REC_KEYMASK;
         return;
     }
-    const uint4* tb = table + msms[sg.msm].table0 * 8;  // 128-byte entries
+    const uint4* tb = table + msms[sg.msm].table0 * (kTableEntryBytes / 16);
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
@@ -1084,7 +1084,7 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) {
 }
 
 static inline uint64_t* table_idmask(void* table, uint64_t n) {
-    return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) + n * 256);
+    return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) + n * 2 * kTableEntryBytes);
 }
 
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st) {

@@ -10,7 +10,7 @@ namespace zk {
 // A table of n points takes table_bytes(n): n pairs of 128-byte entries, then
 // a bitmap with bit i set when point i is the identity (written by the table
 // builders, read by the decomposition).
-inline uint64_t table_bytes(uint64_t n) { return n * 256 + (n + 63) / 64 * 8; }
+inline uint64_t table_bytes(uint64_t n) { return n * 2 * kTableEntryBytes + (n + 63) / 64 * 8; }
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);

@@ -12,6 +12,9 @@
 
 namespace zk {
 
+// bytes per base-table entry (one affine point in the kernels' field form)
+constexpr uint64_t kTableEntryBytes = 128;
+
 struct MsmDesc {
     uint64_t scalar0;  // first scalar of this MSM in the batch scalar buffer
     uint64_t table0;   // first table entry (affine point index)
