@@ -106,7 +106,8 @@ __device__ __forceinline__ Fq fq_csub(const Fq& a) {
     return r;
 }
 
-// a == 0 mod p for a < 2p
+// a == 0 mod p for a < 2p.  The empty asm keeps the two OR-reductions whole:
+// without it the compiler splits "z == 0" into one compare per limb.
 __device__ __forceinline__ bool fq_is_zero2(const Fq& a) {
     uint32_t z = 0, e = 0;
 #pragma unroll
@@ -114,6 +115,7 @@ __device__ __forceinline__ bool fq_is_zero2(const Fq& a) {
         z |= a.v[i];
         e |= a.v[i] ^ q_p(i);
     }
+    asm("" : "+v"(z), "+v"(e));
     return z == 0 || e == 0;
 }
 
@@ -206,6 +208,58 @@ __device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y
     q.ZZZ = q.ZZ;
     r = qxyzz_select(p_id, q, r);
     acc = qxyzz_select(q_id, acc, r);
+}
+
+// Placeholder for an accumulator that starts a bucket run: the value is never
+// used (the run's first addition selects the point itself), but ZZ = ZZZ = 1
+// and X = 0 keep P = x2 - X away from 0, so the first addition does not look
+// exceptional.
+__device__ __forceinline__ QXyzz qxyzz_run_start() {
+    QXyzz r;
+    r.X = fq_zero();
+    r.Y = fq_zero();
+    r.ZZ = fq_one();
+    r.ZZZ = r.ZZ;
+    return r;
+}
+
+// acc += (x2, y2) in the bucket-accumulation loop.  acc_id says acc is the
+// identity (a run start, or a cancellation earlier in the run); (x2, y2) is
+// never the identity there.  Doubling (acc == q) and cancellation (acc == -q)
+// both have PP = 0 mod p; PP < 1.5p is then 0 or p, so its low limb is 0 or
+// p_0.  That one-limb test screens every step; the exact tests run only in
+// lanes that pass it, and a wave branches to them only if one of its lanes
+// does.  After a cancellation acc holds ZZ = 0 mod p, the stored form of the
+// identity.
+__device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
+                                               const Fq& y2) {
+    const Fq U2 = fq_mul(x2, acc.ZZ);
+    const Fq S2 = fq_mul(y2, acc.ZZZ);
+    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
+    const Fq R = fq_sub<4>(S2, acc.Y);
+    const Fq PP = fq_sqr(P);
+    const Fq PPP = fq_mul(P, PP);
+    const Fq Q = fq_mul(acc.X, PP);
+    const Fq RR = fq_sqr(R);
+    QXyzz r;
+    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
+    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(acc.Y, PPP));  // < 3.5p
+    r.ZZ = fq_mul(acc.ZZ, PP);
+    r.ZZZ = fq_mul(acc.ZZZ, PPP);
+    bool cancel = false;
+    if (__builtin_expect(PP.v[0] == 0u || PP.v[0] == q_p(0), 0)) {
+        const bool z = !acc_id && fq_is_zero2(PP);
+        const bool d = z && fq_is_zero2(RR);
+        cancel = z && !d;
+        if (d) r = qxyzz_dbl_aff(x2, y2);
+    }
+    QXyzz q;
+    q.X = x2;
+    q.Y = y2;
+    q.ZZ = fq_one();
+    q.ZZZ = q.ZZ;
+    acc = qxyzz_select(acc_id, q, r);
+    acc_id = cancel;
 }
 
 // acc += q (both XYZZ)

@@ -592,7 +592,8 @@ __global__ __launch_bounds__(256) void k_accumulate(
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
-    QXyzz acc = qxyzz_identity();
+    QXyzz acc = qxyzz_run_start();
+    bool acc_id = true;
     // software pipeline: the next entry's key, value and base are loaded
     // before the current mixed addition, hiding the random 128-byte gather
     uint32_t k = cur, v = vals[b];
@@ -611,15 +612,13 @@ __global__ __launch_bounds__(256) void k_accumulate(
                 if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, acc);
             }
             first = false;
-            acc = qxyzz_identity();
+            acc = qxyzz_run_start();
+            acc_id = true;
             cur = k;
         }
-        // exceptional cases (identity, doubling) by selects inside the
-        // addition: a branchy form costs ~13% more VALU instructions per step
-        // (register moves at the joins), measured with SQ_INSTS_VALU
         // y is already negated for a negative digit; identity bases have no
         // digits (table bitmap), so q is never the identity
-        qxyzz_madd(acc, q.x, q.y, false);
+        qxyzz_madd_run(acc, acc_id, q.x, q.y);
         k = kn;
         v = vn;
         q = qn;
