@@ -104,6 +104,62 @@ def fq_sqr(a):
     return t
 
 
+def kpr(k):
+    """k*p with 2^29 borrowed into limbs 0..7 (q_kpr)"""
+    c = kp(k)
+    r = [c[0] + (1 << 29)] + [c[i] + (1 << 29) - 1 for i in range(1, 8)] + [c[8] - 1]
+    assert val(r) == k * P
+    return r
+
+
+def fq_mul_sub(k, a, b, c, sqr=False):
+    """fq_mul_sub<K> / fq_sqr_sub<K>: a*b/R - c + k*p, the limbs of k*p - c
+    entering the high columns (k = 0: the plain product)."""
+    if sqr:
+        assert a is b and all(v < (1 << 29) for v in a)
+    assert all(v < (1 << 30) for v in a + b)
+    assert val(a) * val(b) < P * R // 2
+    d = [0] * 9
+    if k:
+        K = kpr(k)
+        assert all(v < (1 << 29) for v in c)
+        d = [K[i] - c[i] for i in range(9)]
+        assert all(0 <= x < (1 << 32) for x in d), "negative limb of k*p - c"
+    m = [0] * 9
+    t = [0] * 9
+    acc = 0
+
+    def mac(x, y):
+        nonlocal acc
+        acc += x * y
+        assert acc < U64, "column overflow"
+
+    for kk in range(9):
+        for j in range(kk):
+            mac(a[j], b[kk - j])
+            mac(m[j], PL[kk - j])
+        mac(a[kk], b[0])
+        m[kk] = ((acc & 0xFFFFFFFF) * PINV) & MASK
+        mac(m[kk], PL[0])
+        assert acc & MASK == 0
+        acc >>= 29
+    for kk in range(9, 17):
+        acc += d[kk - 9]
+        for j in range(kk - 8, 9):
+            mac(a[j], b[kk - j])
+            mac(m[j], PL[kk - j])
+        t[kk - 9] = acc & MASK
+        acc >>= 29
+    t[8] = acc + d[8]
+    assert t[8] < (1 << 29)
+    r = val(t)
+    cv = val(c) if k else 0
+    prod = val(a) * val(b)
+    assert (r + cv - k * P) % P == prod * pow(R, -1, P) % P
+    assert r < prod // R + P + 1 - cv + k * P
+    return t
+
+
 def fq_add(a, b):
     r, c = [], 0
     for i in range(9):
@@ -197,6 +253,41 @@ def madd(acc, x2, y2, q_id):
     return r
 
 
+RUN_START = (limbs(0), limbs(0), ONE, ONE)
+
+
+def madd_run(acc, acc_id, x2, y2):
+    """qxyzz_madd_run: the accumulation-loop addition, identity as a flag,
+    exceptional cases screened on limb 0 of PP"""
+    X1, Y1, ZZ1, ZZZ1 = acc
+    lt(X1, 4), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
+    Pp = fq_mul_sub(5, x2, ZZ1, X1)
+    Rr = fq_mul_sub(5, y2, ZZZ1, Y1)
+    lt(Pp, 6.5), lt(Rr, 6.5)
+    PP = fq_sqr(Pp)
+    PPP = fq_mul(Pp, PP)
+    Q = fq_mul(X1, PP)
+    T = fq_add(fq_add(Q, Q), PPP)
+    lt(T, 4.5)
+    X3 = fq_csub(3, fq_mul_sub(5, Rr, Rr, T, sqr=True))
+    lt(X3, 3.5)
+    Y3 = fq_mul_sub(2, Rr, fq_sub(4, Q, X3), fq_mul(Y1, PPP))
+    lt(Y3, 3.5)
+    r = (X3, Y3, fq_mul(ZZ1, PP), fq_mul(ZZZ1, PPP))
+    cancel = False
+    if PP[0] in (0, PL[0]):  # the one-limb screen
+        z = not acc_id and iz2(PP)
+        d = z and iz2(fq_sqr(Rr))
+        cancel = z and not d
+        if d:
+            r = dbl_core(x2, y2)
+    else:
+        assert not iz2(PP), "screen missed PP = 0 mod p"
+    if acc_id:
+        r = (x2, y2, ONE, ONE)
+    return r, cancel
+
+
 def add(acc, q):
     X1, Y1, ZZ1, ZZZ1 = acc
     X2, Y2, ZZ2, ZZZ2 = q
@@ -241,8 +332,8 @@ def affine(pt):
 
 
 def worst(k):
-    """a value just below k*p with normalised limbs"""
-    return limbs(k * P - 1)
+    """a value just below k*p with normalised limbs (k may be a half)"""
+    return limbs(int(2 * k) * P // 2 - 1)
 
 
 def test_mul_extremes():
@@ -292,6 +383,50 @@ def test_group_law_and_bounds():
     assert affine(add((limbs(0),) * 4, x)) == pts[2]
 
 
+def test_fused_subtraction_extremes():
+    """fq_mul_sub / fq_sqr_sub at the largest subtrahends the kernels pass"""
+    for k, cb in ((5, 4), (5, 4.5), (2, 1.5)):
+        for a in (worst(8), worst(6.5), worst(1), limbs(0)):
+            for b in (worst(2), worst(6.5), limbs(1)):
+                for c in (worst(cb), limbs(0), limbs(P - 1)):
+                    if val(a) * val(b) < P * R // 2:
+                        fq_mul_sub(k, a, b, c)
+            fq_mul_sub(k, a, a, worst(cb), sqr=True)
+    # a subtrahend past the stated bound is caught (planted on the CPU)
+    try:
+        fq_mul_sub(2, worst(1), worst(1), worst(2))
+    except AssertionError:
+        pass
+    else:
+        raise AssertionError("kpr check did not fire")
+
+
+def test_run_addition_law():
+    """The accumulation loop's form: runs from RUN_START, doubling and
+    cancellation inside a run, bounds at every step."""
+    rnd = random.Random(5)
+    pts = [B.mul(B.G, rnd.randrange(1, B.R_ORDER)) for _ in range(30)]
+    acc, aid, ref = RUN_START, True, None
+    for i, pt in enumerate(pts):
+        q = pt if i % 4 else B.neg(pt)
+        acc, aid = madd_run(acc, aid, to_int(q[0]), to_int(q[1]))
+        ref = B.add(ref, q)
+        assert not aid and affine(acc) == ref
+    # doubling: acc == q
+    a1, f1 = madd_run(RUN_START, True, to_int(pts[0][0]), to_int(pts[0][1]))
+    a2, f2 = madd_run(a1, f1, to_int(pts[0][0]), to_int(pts[0][1]))
+    assert not f2 and affine(a2) == B.add(pts[0], pts[0])
+    # cancellation: acc == -q, then the run goes on from the identity
+    ny = to_int(B.neg(pts[0])[1])
+    a3, f3 = madd_run(a1, f1, to_int(pts[0][0]), ny)
+    assert f3 and affine(a3) is None
+    a4, f4 = madd_run(a3, f3, to_int(pts[1][0]), to_int(pts[1][1]))
+    assert not f4 and affine(a4) == pts[1]
+    # the generator has x = 1: P = x2 - 0 is not 0 from RUN_START either way
+    g1, _ = madd_run(RUN_START, True, to_int(B.G[0]), to_int(B.G[1]))
+    assert affine(g1) == B.G
+
+
 def test_bounds_at_extremes():
     """Accumulators at the top of their stated bounds still give the right
     group element (the formulas only see residues)."""
@@ -314,3 +449,5 @@ def test_bounds_at_extremes():
         assert affine(out) == B.add(pt, q)
         out2 = add(acc, acc)
         assert affine(out2) == B.add(pt, pt)
+        out3, f = madd_run(acc, False, to_int(q[0]), to_int(q[1]))
+        assert not f and affine(out3) == B.add(pt, q)

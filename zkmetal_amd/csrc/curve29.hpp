@@ -22,20 +22,6 @@ struct QXyzz {
     Fq X, Y, ZZ, ZZZ;
 };
 
-// limb i of K*p (folded at compile time inside unrolled loops)
-template <int K>
-__device__ __forceinline__ uint32_t q_kp(int i) {
-    uint64_t c = 0;
-    uint32_t r = 0;
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-        const uint64_t s = (uint64_t)K * q_p(j) + c;
-        if (j == i) r = (uint32_t)s & ZQ_MASK;
-        c = s >> 29;
-    }
-    return r;
-}
-
 __device__ __forceinline__ Fq fq_zero() {
     Fq r;
 #pragma unroll
@@ -233,23 +219,20 @@ __device__ __forceinline__ QXyzz qxyzz_run_start() {
 // identity.
 __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
                                                const Fq& y2) {
-    const Fq U2 = fq_mul(x2, acc.ZZ);
-    const Fq S2 = fq_mul(y2, acc.ZZZ);
-    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
-    const Fq R = fq_sub<4>(S2, acc.Y);
+    const Fq P = fq_mul_sub<5>(x2, acc.ZZ, acc.X);   // U2 - X1 + 5p < 6.5p
+    const Fq R = fq_mul_sub<5>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 5p < 6.5p
     const Fq PP = fq_sqr(P);
     const Fq PPP = fq_mul(P, PP);
     const Fq Q = fq_mul(acc.X, PP);
-    const Fq RR = fq_sqr(R);
     QXyzz r;
-    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
-    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(acc.Y, PPP));  // < 3.5p
+    r.X = fq_csub<3>(fq_sqr_sub<5>(R, fq_add(fq_add(Q, Q), PPP)));  // < 6.5p -> < 3.5p
+    r.Y = fq_mul_sub<2>(R, fq_sub<4>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
     r.ZZ = fq_mul(acc.ZZ, PP);
     r.ZZZ = fq_mul(acc.ZZZ, PPP);
     bool cancel = false;
     if (__builtin_expect(PP.v[0] == 0u || PP.v[0] == q_p(0), 0)) {
         const bool z = !acc_id && fq_is_zero2(PP);
-        const bool d = z && fq_is_zero2(RR);
+        const bool d = z && fq_is_zero2(fq_sqr(R));  // R^2 < 1.5p
         cancel = z && !d;
         if (d) r = qxyzz_dbl_aff(x2, y2);
     }

@@ -54,9 +54,38 @@ __device__ __forceinline__ void qmac(uint64_t& acc, uint32_t x, uint32_t y) {
 // addition, 3% less time.
 __device__ __forceinline__ void carry_fence(uint64_t& acc) { asm("" : "+v"(acc)); }
 
-// a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
-// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
-__device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
+// limb i of K*p, normalised (folded at compile time inside unrolled loops)
+template <int K>
+__device__ __forceinline__ uint32_t q_kp(int i) {
+    uint64_t c = 0;
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const uint64_t s = (uint64_t)K * q_p(j) + c;
+        if (j == i) r = (uint32_t)s & ZQ_MASK;
+        c = s >> 29;
+    }
+    return r;
+}
+
+// K*p with 2^29 borrowed into every limb below the top one: limb 0 is
+// kp_0 + 2^29, limbs 1..7 kp_i + 2^29 - 1, limb 8 kp_8 - 1 (same value).  For
+// a normalised c with c_8 < kp_8 every limb of (this - c) is >= 0, so K*p - c
+// needs no borrow propagation.
+template <int K>
+__device__ __forceinline__ uint32_t q_kpr(int i) {
+    const uint32_t c = q_kp<K>(i);
+    return i == 0 ? c + (1u << 29) : (i < 8 ? c + (1u << 29) - 1u : c - 1u);
+}
+
+// a * b / 2^261 - c + K*p mod p (K > 0), or a * b / 2^261 (K == 0):
+// result < ab/2^261 + p + K*p with normalised limbs.  The limbs of K*p - c
+// (q_kpr) enter the high columns of the product, where they cost one
+// subtraction each instead of a separate normalised subtraction.
+// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p);
+// c (K > 0): normalised limbs, c_8 < kp_8 (e.g. c < (K - 1/2) p).
+template <int K>
+__device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) {
     uint32_t m[9];
     Fq t;
     uint64_t acc = 0;
@@ -75,6 +104,7 @@ __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
+        if (K > 0) acc += q_kpr<K>(k - 9) - c.v[k - 9];
 #pragma unroll
         for (int j = k - 8; j < 9; j++) {
             qmac(acc, a.v[j], b.v[k - j]);
@@ -84,15 +114,21 @@ __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) {
         acc >>= 29;
         carry_fence(acc);
     }
-    t.v[8] = (uint32_t)acc;
+    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K>(8) - c.v[8] : 0u);
     return t;
 }
+
+// a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
+// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
+__device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) { return fq_mul_sub<0>(a, b, a); }
 
 // a^2 / 2^261 mod p: the symmetric products a_i a_j (i != j) once, from the
 // doubled limb 2 a_i (< 2^30), plus the squares: 45 + 81 multiply-adds.
 // Column bound: 4 doubled products < 2^59, one square < 2^58, 9 m*p products
 // < 2^58 and the carry: < 18 * 2^58 < 2^63.  a: normalised limbs, value < 8p.
-__device__ __forceinline__ Fq fq_sqr(const Fq& a) {
+// fq_sqr_sub<K>(a, c) = a^2 / 2^261 - c + K*p, as fq_mul_sub.
+template <int K>
+__device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
     uint32_t a2[9], m[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) a2[i] = a.v[i] << 1;
@@ -112,6 +148,7 @@ __device__ __forceinline__ Fq fq_sqr(const Fq& a) {
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
+        if (K > 0) acc += q_kpr<K>(k - 9) - c.v[k - 9];
 #pragma unroll
         for (int i = k - 8; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
         if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
@@ -121,9 +158,11 @@ __device__ __forceinline__ Fq fq_sqr(const Fq& a) {
         acc >>= 29;
         carry_fence(acc);
     }
-    t.v[8] = (uint32_t)acc;
+    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K>(8) - c.v[8] : 0u);
     return t;
 }
+
+__device__ __forceinline__ Fq fq_sqr(const Fq& a) { return fq_sqr_sub<0>(a, a); }
 
 // 256-bit integer (8 x 32) <-> 9 x 29 bit slices of the same integer
 __device__ __forceinline__ Fq fq_slice(const Fp& a) {
