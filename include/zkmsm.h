@@ -70,6 +70,21 @@ ZkMetalStatus zkmsm_kzg_commit(ZkMetalMSMEngine engine, const uint8_t* coeffs_mo
 ZkMetalStatus zkmsm_kzg_commit_device(ZkMetalMSMEngine engine, const void* device_coeffs_mont,
                                       const uint32_t* counts, uint32_t n_polys, uint8_t* results);
 
+/* k MSMs sharing one base set (MultiMSM.swift:21-60 multiMSM): points n x 64 B,
+ * scalar_sets k x n x 32 B (set j at offset j*n*32), results k x 96 B.  The
+ * base table is built once; the k MSMs run as one batched launch. */
+ZkMetalStatus zkmsm_multi_msm(ZkMetalMSMEngine engine, const uint8_t* points, uint32_t n,
+                              const uint8_t* scalar_sets, uint32_t k, uint8_t* results);
+
+/* Partitioned MSM (MultiMSM.swift:90-124 innerProductMSM): results[j] =
+ * MSM(points[partitions[j] .. partitions[j+1]), same range of scalars);
+ * partitions has n_parts + 1 entries, non-decreasing, first 0, last n.  An
+ * empty partition writes 96 zero bytes (the identity). */
+ZkMetalStatus zkmsm_inner_product_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                      const uint8_t* scalars, uint32_t n,
+                                      const uint32_t* partitions, uint32_t n_parts,
+                                      uint8_t* results);
+
 /* Plain MSMs over bases and scalars that are both already in device memory
  * (the zkmetal_bn254_msm_batch contract without the host copies): the
  * [P, phi(P)] table is built inside the call. */

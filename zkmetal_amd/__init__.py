@@ -110,6 +110,8 @@ def lib():
         "zkmsm_server_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
         "zkmsm_server_destroy": (None, [vp]),
         "zkmsm_server_footprint": (u64, [u32]),
+        "zkmsm_multi_msm": (i32, [vp, vp, u32, vp, u32, vp]),
+        "zkmsm_inner_product_msm": (i32, [vp, vp, vp, u32, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -256,6 +258,34 @@ class MetalMSM:
         _check(lib().zkmetal_bn254_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
                                              len(counts), ctypes.c_void_p(out.ctypes.data)), "msm_batch")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def multi_msm(self, points, scalar_sets):
+        """zkmsm_multi_msm: k scalar sets (k x n x 32 B) over one base set of n
+        points -> list of k 96-byte results."""
+        n = _count(points, 64)
+        sp, k2 = _buf(scalar_sets)
+        total = _count(scalar_sets, 32)
+        if n == 0 or total % n:
+            raise MSMError(ERR_INVALID_INPUT, "multi_msm: scalar sets must be k x n")
+        k = total // n
+        pp, k1 = _buf(points)
+        out = np.zeros(k * 96, dtype=np.uint8)
+        _check(lib().zkmsm_multi_msm(self.handle, pp, n, sp, k, ctypes.c_void_p(out.ctypes.data)),
+               "multi_msm")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(k)]
+
+    def inner_product_msm(self, points, scalars, partitions):
+        """zkmsm_inner_product_msm: one MSM per range [partitions[j], partitions[j+1])."""
+        n = _count(points, 64)
+        parts = np.ascontiguousarray(np.asarray(partitions, dtype=np.uint32))
+        nparts = max(len(parts) - 1, 0)
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = np.zeros(max(nparts, 1) * 96, dtype=np.uint8)
+        _check(lib().zkmsm_inner_product_msm(self.handle, pp, sp, n, ctypes.c_void_p(parts.ctypes.data),
+                                             nparts, ctypes.c_void_p(out.ctypes.data)),
+               "inner_product_msm")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(nparts)]
 
     def msm_batch_device(self, dpoints, dscalars, counts):
         """Plain MSMs with bases and scalars in DeviceBuffers."""

@@ -137,6 +137,33 @@ ZkMetalStatus zkmetal_bn254_msm_batch_auto(const uint8_t* all_points, const uint
     return batch_impl(e, all_points, all_scalars, counts, n_msms, results);
 }
 
+ZkMetalStatus zkmsm_multi_msm(ZkMetalMSMEngine engine, const uint8_t* points, uint32_t n,
+                              const uint8_t* scalar_sets, uint32_t k, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->multi_msm_host(points, n, scalar_sets, k, results);
+}
+
+ZkMetalStatus zkmsm_inner_product_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                      const uint8_t* scalars, uint32_t n,
+                                      const uint32_t* partitions, uint32_t n_parts,
+                                      uint8_t* results) {
+    // MultiMSM.swift:90-124: boundaries sorted, first 0, last n; an empty
+    // partition gives the identity (96 zero bytes, as a zero count does in
+    // the batch call)
+    Engine* e = as_engine(engine);
+    if (!e || !partitions || n_parts == 0 || partitions[0] != 0 || partitions[n_parts] != n)
+        return ZKMETAL_ERR_INVALID_INPUT;
+    std::vector<uint32_t> counts(n_parts);
+    for (uint32_t j = 0; j < n_parts; j++) {
+        if (partitions[j + 1] < partitions[j]) return ZKMETAL_ERR_INVALID_INPUT;
+        counts[j] = partitions[j + 1] - partitions[j];
+    }
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_batch_host(points, scalars, counts.data(), n_parts, results);
+}
+
 void zkmetal_set_shader_dir(const char* path) {
     std::lock_guard<std::mutex> lk(g_dir_mu);
     g_shader_dir = path ? path : "";

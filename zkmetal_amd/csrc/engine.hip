@@ -501,6 +501,22 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     return kOk;
 }
 
+// MultiMSM.swift:21-60 (k MSMs sharing one point set).  The reference runs
+// the k MSMs one after another over the uploaded points; here the base table
+// is built once and every scalar set is an MSM over its full prefix, all in
+// one batched launch sequence.
+int32_t Engine::multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t* scalars,
+                               uint32_t k, uint8_t* out96) {
+    if (!points || !scalars || !out96 || n == 0 || k == 0) return kInvalidInput;
+    if ((uint64_t)n * k >= (1ull << 30)) return kInvalidInput;
+    ZK_ALLOC(pts_in_, (uint64_t)n * 64);
+    ZK_ALLOC(ptable_, table_bytes(n));
+    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
+    launch_build_table(pts_in_.p, ptable_.p, n, stream_);
+    std::vector<uint32_t> counts(k, n);
+    return msm_table(ptable_.p, n, scalars, false, counts.data(), k, out96);
+}
+
 int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
                            uint32_t nmsm, uint8_t* out96) {
     if (!d_points || !d_scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;

@@ -94,6 +94,11 @@ class Engine {
     int32_t msm_batch_host(const uint8_t* points, const uint8_t* scalars,
                            const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
 
+    // k MSMs over one base set of n points (scalar set j at scalars + j*n*32):
+    // the table is built once and the k MSMs run as one batched launch
+    int32_t multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t* scalars, uint32_t k,
+                           uint8_t* out96);
+
     // bases and scalars already in device memory (plain MSMs, phi table built per call)
     int32_t msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
                        uint32_t nmsm, uint8_t* out96);

@@ -251,15 +251,32 @@ __global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ 
     store_aff(pts, i, a);
 }
 
+// The block's 256 table pairs are staged in LDS (a row per point, padded by
+// 16 B) and written out as consecutive 16-byte pieces across the block, so
+// every store instruction of a wave covers 1 KB of contiguous table.
 __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ pts,
                                                      uint4* __restrict__ table,
                                                      uint64_t* __restrict__ idmask, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Aff a = load_aff(pts, i);
-    a.x = fp_canon(a.x);
-    a.y = fp_canon(a.y);
-    store_table_pair(table, idmask, i, a);
+    constexpr uint32_t kPair = 2 * kTableEntryBytes / 16;  // uint4 per point
+    constexpr uint32_t kRow = kPair + 1;
+    __shared__ uint4 stage[256 * kRow];
+    const uint64_t i0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t i = i0 + threadIdx.x;
+    if (i < n) {
+        Aff a = load_aff(pts, i);
+        QAff q;
+        q.x = fq_from_fp_canon(fp_canon(a.x));
+        q.y = fq_from_fp_canon(fp_canon(a.y));
+        store_id_bit(idmask, i, qaff_is_id(q));
+        uint4* row = stage + threadIdx.x * kRow;
+        store_qtab(row, 0, q);
+        q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
+        store_qtab(row, 1, q);
+    }
+    __syncthreads();
+    const uint32_t words = (uint32_t)min<uint64_t>(256, n - i0) * kPair;
+    uint4* dst = table + i0 * kPair;
+    for (uint32_t w = threadIdx.x; w < words; w += 256) dst[w] = stage[(w / kPair) * kRow + w % kPair];
 }
 
 // --------------------------------------------------- GLV + signed digits
