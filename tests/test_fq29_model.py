@@ -104,25 +104,36 @@ def fq_sqr(a):
     return t
 
 
-def kpr(k):
-    """k*p with 2^29 borrowed into limbs 0..7 (q_kpr)"""
+def kpr(k, b=1):
+    """k*p with b * 2^29 borrowed into limbs 0..7 (q_kpr<K, B>)"""
     c = kp(k)
-    r = [c[0] + (1 << 29)] + [c[i] + (1 << 29) - 1 for i in range(1, 8)] + [c[8] - 1]
+    r = [c[0] + b * (1 << 29)] + [c[i] + b * (1 << 29) - b for i in range(1, 8)] + [c[8] - b]
     assert val(r) == k * P
     return r
 
 
-def fq_mul_sub(k, a, b, c, sqr=False):
-    """fq_mul_sub<K> / fq_sqr_sub<K>: a*b/R - c + k*p, the limbs of k*p - c
-    entering the high columns (k = 0: the plain product)."""
+def sub_lazy(k, q, c, b=1):
+    """fq_sub_lazy<K, B>: q - c + k*p limb by limb, no carries"""
+    K = kpr(k, b)
+    r = [q[i] + K[i] - c[i] for i in range(9)]
+    assert all(0 <= v < (1 << 32) for v in r)
+    assert val(r) == val(q) - val(c) + k * P
+    return r
+
+
+def fq_mul_sub(k, a, b, c, sqr=False, borrow=1):
+    """fq_mul_sub<K, B> / fq_sqr_sub<K, B>: a*b/R - c + k*p, the limbs of
+    k*p - c entering the high columns (k = 0: the plain product)."""
     if sqr:
         assert a is b and all(v < (1 << 29) for v in a)
-    assert all(v < (1 << 30) for v in a + b)
+    # limbs < 2^30, or one operand normalised and the other < 1.5 * 2^30
+    assert (all(v < (1 << 30) for v in a + b)
+            or (all(v < (1 << 29) for v in a) and all(v < 3 << 29 for v in b)))
     assert val(a) * val(b) < P * R // 2
     d = [0] * 9
     if k:
-        K = kpr(k)
-        assert all(v < (1 << 29) for v in c)
+        K = kpr(k, borrow)
+        assert all(v <= borrow * MASK for v in c)
         d = [K[i] - c[i] for i in range(9)]
         assert all(0 <= x < (1 << 32) for x in d), "negative limb of k*p - c"
     m = [0] * 9
@@ -267,11 +278,11 @@ def madd_run(acc, acc_id, x2, y2):
     PP = fq_sqr(Pp)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(X1, PP)
-    T = fq_add(fq_add(Q, Q), PPP)
+    T = [2 * Q[i] + PPP[i] for i in range(9)]  # lazy: no carries
     lt(T, 4.5)
-    X3 = fq_csub(3, fq_mul_sub(5, Rr, Rr, T, sqr=True))
+    X3 = fq_csub(3, fq_mul_sub(5, Rr, Rr, T, sqr=True, borrow=3))
     lt(X3, 3.5)
-    Y3 = fq_mul_sub(2, Rr, fq_sub(4, Q, X3), fq_mul(Y1, PPP))
+    Y3 = fq_mul_sub(2, Rr, sub_lazy(4, Q, X3), fq_mul(Y1, PPP))
     lt(Y3, 3.5)
     r = (X3, Y3, fq_mul(ZZ1, PP), fq_mul(ZZZ1, PPP))
     cancel = False
@@ -392,6 +403,17 @@ def test_fused_subtraction_extremes():
                     if val(a) * val(b) < P * R // 2:
                         fq_mul_sub(k, a, b, c)
             fq_mul_sub(k, a, a, worst(cb), sqr=True)
+    # the lazy forms: T = 2Q + PPP with all-ones limbs at its largest value,
+    # and Q - X3 + 4p as the wide multiplicand against a normalised R
+    qq, ppp = worst(1.5), worst(1.5)
+    T = [2 * qq[i] + ppp[i] for i in range(9)]
+    T_ones = [3 * MASK] * 8 + [T[8]]
+    for r_ in (worst(6.5), limbs(P - 1)):
+        fq_mul_sub(5, r_, r_, T, sqr=True, borrow=3)
+        fq_mul_sub(5, r_, r_, T_ones, sqr=True, borrow=3)
+        for x3 in (worst(3.5), limbs(0)):
+            t = sub_lazy(4, [MASK] * 8 + [qq[8]], x3)
+            fq_mul_sub(2, r_, t, worst(1.5))
     # a subtrahend past the stated bound is caught (planted on the CPU)
     try:
         fq_mul_sub(2, worst(1), worst(1), worst(2))

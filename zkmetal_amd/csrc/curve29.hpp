@@ -225,8 +225,11 @@ __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const F
     const Fq PPP = fq_mul(P, PP);
     const Fq Q = fq_mul(acc.X, PP);
     QXyzz r;
-    r.X = fq_csub<3>(fq_sqr_sub<5>(R, fq_add(fq_add(Q, Q), PPP)));  // < 6.5p -> < 3.5p
-    r.Y = fq_mul_sub<2>(R, fq_sub<4>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
+    Fq T;  // 2Q + PPP < 4.5p, limbs sums of three normalised limbs
+#pragma unroll
+    for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
+    r.X = fq_csub<3>(fq_sqr_sub<5, 3>(R, T));                            // < 6.5p -> < 3.5p
+    r.Y = fq_mul_sub<2>(R, fq_sub_lazy<4>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
     r.ZZ = fq_mul(acc.ZZ, PP);
     r.ZZZ = fq_mul(acc.ZZZ, PPP);
     bool cancel = false;
@@ -333,22 +336,41 @@ __device__ __forceinline__ void store_qtab(uint4* base, uint64_t idx, const QAff
     for (int i = 0; i < 8; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
+// Buckets, merge records and reduction partials: the 36 limbs as they are
+// (X, Y, ZZ, ZZZ, 9 words each; kBucketBytes = 144 per point), so a store
+// needs no repacking into 256-bit words.  All-zero bytes are the identity.
 __device__ __forceinline__ QXyzz load_qxyzz(const uint4* base, uint64_t idx) {
-    const uint4* p = base + idx * 8;
+    static_assert(kBucketBytes == 144, "bucket layout");
+    const uint4* p = base + idx * 9;
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint4 v = p[i];
+        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+    }
     QXyzz r;
-    r.X = fq_slice(load_fp(p));
-    r.Y = fq_slice(load_fp(p + 2));
-    r.ZZ = fq_slice(load_fp(p + 4));
-    r.ZZZ = fq_slice(load_fp(p + 6));
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        r.X.v[i] = w[i];
+        r.Y.v[i] = w[9 + i];
+        r.ZZ.v[i] = w[18 + i];
+        r.ZZZ.v[i] = w[27 + i];
+    }
     return r;
 }
 
 __device__ __forceinline__ void store_qxyzz(uint4* base, uint64_t idx, const QXyzz& a) {
-    uint4* p = base + idx * 8;
-    store_fp(p, fq_pack(a.X));
-    store_fp(p + 2, fq_pack(a.Y));
-    store_fp(p + 4, fq_pack(a.ZZ));
-    store_fp(p + 6, fq_pack(a.ZZZ));
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        w[i] = a.X.v[i];
+        w[9 + i] = a.Y.v[i];
+        w[18 + i] = a.ZZ.v[i];
+        w[27 + i] = a.ZZZ.v[i];
+    }
+    uint4* p = base + idx * 9;
+#pragma unroll
+    for (int i = 0; i < 9; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 }  // namespace zk

@@ -256,11 +256,11 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_ALLOC(hist_, (uint64_t)pl.tiles * 256 * 4);
     ZK_ALLOC(part_, (uint64_t)pl.chunks * 4);
     ZK_ALLOC(segvalid_, (uint64_t)nseg * 4);
-    ZK_ALLOC(buckets_, pl.buckets * 128);
+    ZK_ALLOC(buckets_, pl.buckets * kBucketBytes);
     ZK_ALLOC(recAk_, rec_cap * 4);
-    ZK_ALLOC(recAp_, rec_cap * 128);
+    ZK_ALLOC(recAp_, rec_cap * kBucketBytes);
     ZK_ALLOC(recBk_, rec_cap * 4);
-    ZK_ALLOC(rc_, pl.groups * 128);
+    ZK_ALLOC(rc_, pl.groups * kBucketBytes);
     ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * 128);
     ZK_ALLOC(err_, 16);
     ZK_ALLOC(hwin_, (uint64_t)nseg * (16 * 128 + 4) + 16);
@@ -304,7 +304,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
                          (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
                          (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
-        launch_fill_zero(buckets_.p, pl.buckets * 128, st);
+        launch_fill_zero(buckets_.p, pl.buckets * kBucketBytes, st);
         ZK_TRY(mark(2));
         launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
                           (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,

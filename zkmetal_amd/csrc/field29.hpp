@@ -68,23 +68,37 @@ __device__ __forceinline__ uint32_t q_kp(int i) {
     return r;
 }
 
-// K*p with 2^29 borrowed into every limb below the top one: limb 0 is
-// kp_0 + 2^29, limbs 1..7 kp_i + 2^29 - 1, limb 8 kp_8 - 1 (same value).  For
-// a normalised c with c_8 < kp_8 every limb of (this - c) is >= 0, so K*p - c
-// needs no borrow propagation.
-template <int K>
+// K*p with B * 2^29 borrowed into every limb below the top one: limb 0 is
+// kp_0 + B 2^29, limbs 1..7 kp_i + B 2^29 - B, limb 8 kp_8 - B (same value).
+// For a c whose limbs are sums of at most B normalised limbs (c_i <=
+// B (2^29 - 1)) and c_8 <= kp_8 - B, every limb of (this - c) is >= 0, so
+// K*p - c needs no borrow propagation.
+template <int K, int B = 1>
 __device__ __forceinline__ uint32_t q_kpr(int i) {
     const uint32_t c = q_kp<K>(i);
-    return i == 0 ? c + (1u << 29) : (i < 8 ? c + (1u << 29) - 1u : c - 1u);
+    return i == 0 ? c + (uint32_t)B * (1u << 29)
+                  : (i < 8 ? c + (uint32_t)B * (1u << 29) - (uint32_t)B : c - (uint32_t)B);
+}
+
+// Q - c + K*p limb by limb with no carry propagation (limbs < 2^29 + B 2^29,
+// value < Q + K*p): a multiplicand for fq_mul_sub / fq_mul whose other operand
+// has normalised limbs.  Q normalised; c as for q_kpr<K, B>.
+template <int K, int B = 1>
+__device__ __forceinline__ Fq fq_sub_lazy(const Fq& q, const Fq& c) {
+    Fq r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.v[i] = q.v[i] + q_kpr<K, B>(i) - c.v[i];
+    return r;
 }
 
 // a * b / 2^261 - c + K*p mod p (K > 0), or a * b / 2^261 (K == 0):
 // result < ab/2^261 + p + K*p with normalised limbs.  The limbs of K*p - c
-// (q_kpr) enter the high columns of the product, where they cost one
+// (q_kpr<K, B>) enter the high columns of the product, where they cost one
 // subtraction each instead of a separate normalised subtraction.
-// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p);
-// c (K > 0): normalised limbs, c_8 < kp_8 (e.g. c < (K - 1/2) p).
-template <int K>
+// a, b: limbs < 2^30, or a normalised and b limbs < 1.5 * 2^30 (column sums
+// then stay below 2^63.3); values with ab < p * 2^261 / 2 (e.g. both < 8p);
+// c (K > 0): as for q_kpr<K, B>.
+template <int K, int B = 1>
 __device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) {
     uint32_t m[9];
     Fq t;
@@ -104,7 +118,7 @@ __device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) 
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
-        if (K > 0) acc += q_kpr<K>(k - 9) - c.v[k - 9];
+        if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
 #pragma unroll
         for (int j = k - 8; j < 9; j++) {
             qmac(acc, a.v[j], b.v[k - j]);
@@ -114,7 +128,7 @@ __device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) 
         acc >>= 29;
         carry_fence(acc);
     }
-    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K>(8) - c.v[8] : 0u);
+    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
     return t;
 }
 
@@ -126,8 +140,8 @@ __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) { return fq_mul_s
 // doubled limb 2 a_i (< 2^30), plus the squares: 45 + 81 multiply-adds.
 // Column bound: 4 doubled products < 2^59, one square < 2^58, 9 m*p products
 // < 2^58 and the carry: < 18 * 2^58 < 2^63.  a: normalised limbs, value < 8p.
-// fq_sqr_sub<K>(a, c) = a^2 / 2^261 - c + K*p, as fq_mul_sub.
-template <int K>
+// fq_sqr_sub<K, B>(a, c) = a^2 / 2^261 - c + K*p, as fq_mul_sub.
+template <int K, int B = 1>
 __device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
     uint32_t a2[9], m[9];
 #pragma unroll
@@ -148,7 +162,7 @@ __device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
     }
 #pragma unroll
     for (int k = 9; k < 17; k++) {
-        if (K > 0) acc += q_kpr<K>(k - 9) - c.v[k - 9];
+        if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
 #pragma unroll
         for (int i = k - 8; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
         if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
@@ -158,7 +172,7 @@ __device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
         acc >>= 29;
         carry_fence(acc);
     }
-    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K>(8) - c.v[8] : 0u);
+    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
     return t;
 }
 

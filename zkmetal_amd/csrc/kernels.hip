@@ -749,7 +749,7 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ long_count,
                                                     const uint32_t* __restrict__ long_list,
                                                     uint32_t long_cap, uint4* __restrict__ dest) {
-    __shared__ uint4 part[4][8];
+    __shared__ uint4 part[4][kBucketBytes / 16];
     __shared__ uint32_t part_has[4];
     const uint32_t n = min(*long_count, long_cap);
     const uint32_t t = threadIdx.x, wv = t >> 6, lane = __lane_id();

@@ -14,6 +14,8 @@ namespace zk {
 
 // bytes per base-table entry (one affine point in the kernels' field form)
 constexpr uint64_t kTableEntryBytes = 128;
+// bytes per stored XYZZ point (buckets, merge records, reduction partials)
+constexpr uint64_t kBucketBytes = 144;
 
 struct MsmDesc {
     uint64_t scalar0;  // first scalar of this MSM in the batch scalar buffer

@@ -15,7 +15,7 @@ namespace zk {
 uint64_t Server::footprint(uint32_t n) {
     const uint32_t c = default_window_bits(n);
     const uint64_t w = (128 + c - 1) / c;
-    return (uint64_t)n * 32 + 2ull * n * w * 16 + w * (1ull << (c - 1)) * 128 + (1u << 20);
+    return (uint64_t)n * 32 + 2ull * n * w * 16 + w * (1ull << (c - 1)) * kBucketBytes + (1u << 20);
 }
 
 int32_t Server::create(const void* table, uint32_t table_n, const ServerConfig& cfg,
