@@ -9,8 +9,8 @@ call (GLV/endomorphism table, decomposition, sort, accumulation, reduction,
 window combine, normalisation) returning the result to the host.
 
 Prints one JSON line.  `value` is ms per MSM at n = 2^22 (lower is better);
-`roofline` is the dominant kernel (bucket accumulation) against the measured
-v_mad_u64_u32 (32x32->64 integer multiply) peak; `cpu_baseline` is the CPU
+`roofline` is the dominant kernel (bucket accumulation) against the MI355X
+32x32->64 integer multiply peak (256 CUs x 64 lanes x 2.4 GHz); `cpu_baseline` is the CPU
 restatement of the reference's Pippenger on this host's cores, on the same
 inputs, timed in the same run.
 
@@ -34,6 +34,14 @@ import numpy as np  # noqa: E402
 import zkmetal_amd as zk  # noqa: E402
 
 PRODUCTS_PER_FIELD_MUL = 136  # 8x8 a*b + 8x8 m*p + 8 m_i (32-bit limbs)
+
+# Peak 32x32->64 integer multiply rate of an MI355X: v_mad_u64_u32 issues one
+# wave64 instruction per 4 clocks per SIMD (64 lanes per clock per CU; the
+# probe's independent-chain microbenchmark sustains that rate at the clock it
+# runs at), 256 CUs at the published 2400 MHz peak engine clock.
+MI355X_CUS = 256
+MI355X_PEAK_CLOCK_HZ = 2.4e9
+PEAK_INT32_MUL = MI355X_CUS * 64 * MI355X_PEAK_CLOCK_HZ
 
 
 def algorithmic_field_muls(n):
@@ -369,7 +377,7 @@ def main():
     args.traffic = _traffic_from_profiles()
     eng = zk.MetalMSM()
     args.rates = eng.probe_rates()
-    peak = args.rates["mad64_per_s"]
+    peak = PEAK_INT32_MUL
     if args.config == "plain":
         line = plain(args, eng, peak)
     elif args.config == "serve":

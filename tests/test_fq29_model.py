@@ -129,7 +129,7 @@ def fq_mul_sub(k, a, b, c, sqr=False, borrow=1):
     # limbs < 2^30, or one operand normalised and the other < 1.5 * 2^30
     assert (all(v < (1 << 30) for v in a + b)
             or (all(v < (1 << 29) for v in a) and all(v < 3 << 29 for v in b)))
-    assert val(a) * val(b) < P * R // 2
+    assert val(a) * val(b) < P * R  # result < 2p
     d = [0] * 9
     if k:
         K = kpr(k, borrow)
@@ -271,18 +271,19 @@ def madd_run(acc, acc_id, x2, y2):
     """qxyzz_madd_run: the accumulation-loop addition, identity as a flag,
     exceptional cases screened on limb 0 of PP"""
     X1, Y1, ZZ1, ZZZ1 = acc
-    lt(X1, 4), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
-    Pp = fq_mul_sub(5, x2, ZZ1, X1)
+    lt(X1, 6.5), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
+    Pp = fq_mul_sub(7, x2, ZZ1, X1)
     Rr = fq_mul_sub(5, y2, ZZZ1, Y1)
-    lt(Pp, 6.5), lt(Rr, 6.5)
+    lt(Pp, 8.5), lt(Rr, 6.5)
     PP = fq_sqr(Pp)
+    lt(PP, 1.6)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(X1, PP)
     T = [2 * Q[i] + PPP[i] for i in range(9)]  # lazy: no carries
     lt(T, 4.5)
-    X3 = fq_csub(3, fq_mul_sub(5, Rr, Rr, T, sqr=True, borrow=3))
-    lt(X3, 3.5)
-    Y3 = fq_mul_sub(2, Rr, sub_lazy(4, Q, X3), fq_mul(Y1, PPP))
+    X3 = fq_mul_sub(5, Rr, Rr, T, sqr=True, borrow=3)
+    lt(X3, 6.5)
+    Y3 = fq_mul_sub(2, Rr, sub_lazy(7, Q, X3), fq_mul(Y1, PPP))
     lt(Y3, 3.5)
     r = (X3, Y3, fq_mul(ZZ1, PP), fq_mul(ZZZ1, PPP))
     cancel = False
@@ -411,8 +412,8 @@ def test_fused_subtraction_extremes():
     for r_ in (worst(6.5), limbs(P - 1)):
         fq_mul_sub(5, r_, r_, T, sqr=True, borrow=3)
         fq_mul_sub(5, r_, r_, T_ones, sqr=True, borrow=3)
-        for x3 in (worst(3.5), limbs(0)):
-            t = sub_lazy(4, [MASK] * 8 + [qq[8]], x3)
+        for x3 in (worst(6.5), limbs(0)):
+            t = sub_lazy(7, [MASK] * 8 + [qq[8]], x3)
             fq_mul_sub(2, r_, t, worst(1.5))
     # a subtrahend past the stated bound is caught (planted on the CPU)
     try:
@@ -473,3 +474,10 @@ def test_bounds_at_extremes():
         assert affine(out2) == B.add(pt, pt)
         out3, f = madd_run(acc, False, to_int(q[0]), to_int(q[1]))
         assert not f and affine(out3) == B.add(pt, q)
+        # the loop's own X bound (< 6.5p), then a doubling and a general add
+        acc65 = (lift(X, 6.5), lift(Y, 4), lift(zz, 2), lift(zzz, 2))
+        out4, f = madd_run(acc65, False, to_int(q[0]), to_int(q[1]))
+        assert not f and affine(out4) == B.add(pt, q)
+        out5, _ = madd_run(out4, False, to_int(q[0]), to_int(q[1]))
+        assert affine(out5) == B.add(B.add(pt, q), q)
+        assert affine(add(out4, out4)) == B.add(affine(out4), affine(out4))

@@ -2,11 +2,11 @@
 // same formulas and exception rules as curve.hpp (madd-2008-s, add-2008-s,
 // dbl-2008-s-1; (0,0) affine is the identity, P + P doubles, P + (-P) is the
 // identity), with lazy reduction.  Bounds, in multiples of p:
-//   products < 1.5p (inputs < 8p);  stored X, Y < 4p;  stored ZZ, ZZZ < 2p;
-//   affine table coordinates < p (canonical).
-// A stored XYZZ value is < 4p < 2^256, so XYZZ points are kept in global
-// memory as 256-bit integers (8 x 32-bit words, 128 B as in curve.hpp) and
-// re-sliced into 29-bit limbs on load; base-table entries keep their limbs.
+//   products < ab/2^261 + p: < 1.5p for inputs < 8p, < 1.6p for inputs up to
+//   8.5p;  stored X < 6.5p (4p outside the accumulation loop), Y < 4p;
+//   stored ZZ, ZZZ < 2p;  affine table coordinates < p (canonical).
+// Stored points (buckets, merge records, reduction partials) keep their
+// normalised 29-bit limbs, 144 B per point; base-table entries likewise.
 #pragma once
 #include "curve.hpp"
 #include "field29.hpp"
@@ -211,15 +211,16 @@ __device__ __forceinline__ QXyzz qxyzz_run_start() {
 
 // acc += (x2, y2) in the bucket-accumulation loop.  acc_id says acc is the
 // identity (a run start, or a cancellation earlier in the run); (x2, y2) is
-// never the identity there.  Doubling (acc == q) and cancellation (acc == -q)
-// both have PP = 0 mod p; PP < 1.5p is then 0 or p, so its low limb is 0 or
-// p_0.  That one-limb test screens every step; the exact tests run only in
+// never the identity there.  X3 is left below 6.5p (no final subtraction):
+// every later use of X takes it as a multiplicand or as the subtrahend of a
+// K = 7 form.  Doubling (acc == q) and cancellation (acc == -q) both have
+// PP = 0 mod p; PP < 1.6p is then 0 or p, so its low limb is 0 or p_0.  That one-limb test screens every step; the exact tests run only in
 // lanes that pass it, and a wave branches to them only if one of its lanes
 // does.  After a cancellation acc holds ZZ = 0 mod p, the stored form of the
 // identity.
 __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
                                                const Fq& y2) {
-    const Fq P = fq_mul_sub<5>(x2, acc.ZZ, acc.X);   // U2 - X1 + 5p < 6.5p
+    const Fq P = fq_mul_sub<7>(x2, acc.ZZ, acc.X);   // U2 - X1 + 7p < 8.5p
     const Fq R = fq_mul_sub<5>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 5p < 6.5p
     const Fq PP = fq_sqr(P);
     const Fq PPP = fq_mul(P, PP);
@@ -228,8 +229,8 @@ __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const F
     Fq T;  // 2Q + PPP < 4.5p, limbs sums of three normalised limbs
 #pragma unroll
     for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
-    r.X = fq_csub<3>(fq_sqr_sub<5, 3>(R, T));                            // < 6.5p -> < 3.5p
-    r.Y = fq_mul_sub<2>(R, fq_sub_lazy<4>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
+    r.X = fq_sqr_sub<5, 3>(R, T);                                        // < 6.4p
+    r.Y = fq_mul_sub<2>(R, fq_sub_lazy<7>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
     r.ZZ = fq_mul(acc.ZZ, PP);
     r.ZZZ = fq_mul(acc.ZZZ, PPP);
     bool cancel = false;

@@ -213,7 +213,7 @@ __device__ __forceinline__ Fq fq_from_fp(const Fp& a) {
     return fq_mul(fq_slice(a), k);
 }
 
-// value < 2^255 with normalised limbs -> [0, p) in the 32-bit form
+// value < 8p with normalised limbs -> [0, p) in the 32-bit form
 __device__ __forceinline__ Fp fq_to_fp(const Fq& a) {
     Fq k;  // 2^256 mod p
     k.v[0] = 0x058f0d9du; k.v[1] = 0x1aea1c6eu; k.v[2] = 0x11c2cf74u; k.v[3] = 0x11d651ebu;

@@ -1064,12 +1064,24 @@ __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_
         x[i] = t * 3266489917u + i;
     }
     if (mode == 0) {
+        // eight independent multiply-adds in one asm statement: separate
+        // statements get a wait state after each from the compiler, which
+        // would understate the issue rate
         for (uint32_t it = 0; it < iters; it++) {
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                uint64_t c;
-                asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc[i]), "=s"(c) : "v"(x[i]), "v"(y));
-            }
+            asm volatile(
+                "v_mad_u64_u32 %0, s[0:1], %8, %16, %0\n"
+                "v_mad_u64_u32 %1, s[0:1], %9, %16, %1\n"
+                "v_mad_u64_u32 %2, s[0:1], %10, %16, %2\n"
+                "v_mad_u64_u32 %3, s[0:1], %11, %16, %3\n"
+                "v_mad_u64_u32 %4, s[0:1], %12, %16, %4\n"
+                "v_mad_u64_u32 %5, s[0:1], %13, %16, %5\n"
+                "v_mad_u64_u32 %6, s[0:1], %14, %16, %6\n"
+                "v_mad_u64_u32 %7, s[0:1], %15, %16, %7\n"
+                : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]),
+                  "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7])
+                : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]),
+                  "v"(x[7]), "v"(y)
+                : "s0", "s1");
         }
     } else {
         for (uint32_t it = 0; it < iters; it++) {
