@@ -41,25 +41,11 @@ __device__ __forceinline__ uint32_t upper_index(const uint64_t* pre, uint32_t n,
     return lo;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
-
 // Sort key of a bucket index x < 2^15: low 7 bits in byte 0, the rest in
 // byte 1.  A real key's low byte is then <= 0x7f, so the first radix pass
 // puts every "no bucket" entry (0xFFFF) in bin 0xFF, alone, and can drop them.
 __device__ __forceinline__ uint32_t key_encode(uint32_t x) { return (x & 0x7fu) | ((x >> 7) << 8); }
 __device__ __forceinline__ uint32_t key_decode(uint32_t k) { return (k & 0x7fu) | ((k >> 8) << 7); }
-
-// mask of active lanes whose 8-bit digit equals this lane's
-__device__ __forceinline__ uint64_t match_digit8(uint32_t digit, bool active) {
-    uint64_t peers = __ballot(active);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        bool bit = (digit >> b) & 1u;
-        uint64_t m = __ballot(active && bit);
-        peers &= bit ? m : ~m;
-    }
-    return active ? peers : 0ull;
-}
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     const uint32_t lane = __lane_id();
@@ -321,14 +307,16 @@ __global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ m
 }
 
 // ------------------------------------------------------ radix sort passes
-// Two stable LSD passes of 8 bits over the 16-bit bucket key, independently
-// per segment.  A tile is 4096 consecutive entries of one segment; wave w of
-// the 256-thread block owns tile positions [1024w, 1024w + 1024) and walks
-// them 64 at a time, so the stable order is (wave, round, lane).  Per-wave
-// digit counters live in LDS and are only touched by their own wave (ranks
-// from ballot matches, no atomics, no block barrier per round).  Tile
-// histograms are stored bin-major per segment, hist[tile0*256 + bin*ntiles +
-// t], so one exclusive scan per segment yields every (bin, tile) destination.
+// Two LSD passes of 8 bits over the 16-bit bucket key, independently per
+// segment.  A tile is kSortTile consecutive entries of one segment; wave w of
+// the block owns tile positions [w * 64 * kSortItems, (w + 1) * 64 *
+// kSortItems) and walks them 64 at a time, so the order kept is (wave, round,
+// lane).  Per-wave digit counters live in LDS and are only touched by their
+// own wave; ranks come from LDS atomics (the first pass need not be stable,
+// the second keeps the first pass's order where it matters, see
+// k_sort_scatter).  Tile histograms are stored bin-major per segment,
+// hist[tile0*256 + bin*ntiles + t], so one exclusive scan per segment yields
+// every (bin, tile) destination.
 
 __device__ __forceinline__ uint32_t sort_key(const void* in, int pass, uint64_t pos) {
     if (pass == 0) return reinterpret_cast<const uint32_t*>(in)[pos] & 0xffffu;
@@ -514,10 +502,24 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
             // the final order): ranks from LDS atomics, no digit matching
             rank[r] = act ? atomicAdd(&cnt[w][digit], 1u) : 0u;
         } else {
-            const uint64_t peers = match_digit8(digit, act);
-            const uint32_t base = cnt[w][digit];
-            rank[r] = base + (uint32_t)__popcll(peers & lanemask_lt());
-            if (act && (peers & lanemask_lt()) == 0) cnt[w][digit] = base + (uint32_t)__popcll(peers);
+            // The second pass must keep the first pass's order between entries
+            // of different low bytes; entries with the same low byte and the
+            // same high byte share a bucket, so their order is free.  The 64
+            // positions of one instruction are consecutive in that order, so
+            // their low bytes ascend with the lane: the lanes of one low byte
+            // at a time take LDS-atomic ranks, lowest low byte first (about
+            // two rounds per instruction for uniform digits).
+            const uint32_t lowb = key[r] & 0xffu;
+            bool todo = act;
+            uint32_t rk = 0;
+            for (uint64_t m = __ballot(todo); m != 0; m = __ballot(todo)) {
+                const uint32_t lb = __builtin_amdgcn_readlane(lowb, __ffsll((long long)m) - 1);
+                if (todo && lowb == lb) {
+                    rk = atomicAdd(&cnt[w][digit], 1u);
+                    todo = false;
+                }
+            }
+            rank[r] = rk;
         }
     }
     __syncthreads();
