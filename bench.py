@@ -139,6 +139,18 @@ def plain(args, eng, peak):
         "msm_algorithmic_field_muls": algorithmic_field_muls(n),
         "measured_rates": {k: float("%.4g" % v) for k, v in args.rates.items()},
     }
+    # PCIe-inclusive host-buffer calls (zkmetal_bn254_msm): bases that are
+    # the resident SRS (detected, no base upload) and the same call with the
+    # detection off (bases uploaded, table rebuilt); not the headline value
+    if not args.skip_host_calls:
+        host_pts, host_sc = bases.tobytes(), scal.tobytes()
+        hc = {}
+        for label, on in (("srs_detected", True), ("bases_uploaded", False)):
+            eng.set_srs_detect(on)
+            hms, _ = timed_steps(lambda: eng.msm(host_pts, host_sc), 3, 1)
+            hc[label] = round(hms, 3)
+        eng.set_srs_detect(True)
+        extra["host_buffer_call_ms"] = hc
     # the metric also names n = 2^20: same buffers, first 2^20 points
     if args.logn >= 20 and not args.skip_2_20:
         n20 = 1 << 20
@@ -367,6 +379,7 @@ def main():
     ap.add_argument("--burst", type=int, default=300, help="serve: requests in the saturation phase")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-2-20", action="store_true")
+    ap.add_argument("--skip-host-calls", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:

@@ -70,6 +70,16 @@ ZkMetalStatus zkmsm_kzg_commit(ZkMetalMSMEngine engine, const uint8_t* coeffs_mo
 ZkMetalStatus zkmsm_kzg_commit_device(ZkMetalMSMEngine engine, const void* device_coeffs_mont,
                                       const uint32_t* counts, uint32_t n_polys, uint8_t* results);
 
+/* Host-point calls (zkmetal_bn254_msm, _batch, and the small-scalar
+ * variants) whose point slices are each byte-for-byte a prefix of the
+ * resident SRS run on the resident table: no base upload and no table build
+ * (the batched launch mapping of zkMetal_ffi.swift:1241-1298).  The check is
+ * exact (a 64-byte compare rejects other inputs at once; a match is confirmed
+ * by comparing every slice with a host image of the SRS).  On by default;
+ * zkmsm_srs_detect_hits counts the calls it served. */
+ZkMetalStatus zkmsm_set_srs_detect(ZkMetalMSMEngine engine, int32_t on);
+uint64_t zkmsm_srs_detect_hits(ZkMetalMSMEngine engine);
+
 /* k MSMs sharing one base set (MultiMSM.swift:21-60 multiMSM): points n x 64 B,
  * scalar_sets k x n x 32 B (set j at offset j*n*32), results k x 96 B.  The
  * base table is built once; the k MSMs run as one batched launch. */

@@ -111,6 +111,8 @@ def lib():
         "zkmsm_server_destroy": (None, [vp]),
         "zkmsm_server_footprint": (u64, [u32]),
         "zkmsm_multi_msm": (i32, [vp, vp, u32, vp, u32, vp]),
+        "zkmsm_set_srs_detect": (i32, [vp, i32]),
+        "zkmsm_srs_detect_hits": (u64, [vp]),
         "zkmsm_inner_product_msm": (i32, [vp, vp, vp, u32, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
@@ -122,13 +124,15 @@ def lib():
 
 
 def _buf(x):
-    """bytes / bytearray / numpy -> (ctypes pointer, keepalive)."""
+    """bytes / bytearray / memoryview / numpy -> (ctypes pointer, keepalive),
+    without copying the data (the library only reads inputs)."""
     if isinstance(x, np.ndarray):
         a = np.ascontiguousarray(x)
         return ctypes.c_void_p(a.ctypes.data), a
-    b = bytes(x)
-    cb = ctypes.create_string_buffer(b, len(b))
-    return ctypes.cast(cb, ctypes.c_void_p), cb
+    a = np.frombuffer(x, dtype=np.uint8)
+    if a.size == 0:
+        return ctypes.c_void_p(0), a
+    return ctypes.c_void_p(a.ctypes.data), a
 
 
 def _check(rc, what=""):
@@ -258,6 +262,14 @@ class MetalMSM:
         _check(lib().zkmetal_bn254_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
                                              len(counts), ctypes.c_void_p(out.ctypes.data)), "msm_batch")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def set_srs_detect(self, on):
+        """Serve host-point calls whose bases are prefixes of the resident SRS
+        from the resident table (default on)."""
+        _check(lib().zkmsm_set_srs_detect(self.handle, 1 if on else 0), "set_srs_detect")
+
+    def srs_detect_hits(self):
+        return int(lib().zkmsm_srs_detect_hits(self.handle))
 
     def multi_msm(self, points, scalar_sets):
         """zkmsm_multi_msm: k scalar sets (k x n x 32 B) over one base set of n

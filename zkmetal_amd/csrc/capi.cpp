@@ -137,6 +137,21 @@ ZkMetalStatus zkmetal_bn254_msm_batch_auto(const uint8_t* all_points, const uint
     return batch_impl(e, all_points, all_scalars, counts, n_msms, results);
 }
 
+ZkMetalStatus zkmsm_set_srs_detect(ZkMetalMSMEngine engine, int32_t on) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    e->set_srs_detect(on != 0);
+    return ZKMETAL_SUCCESS;
+}
+
+uint64_t zkmsm_srs_detect_hits(ZkMetalMSMEngine engine) {
+    Engine* e = as_engine(engine);
+    if (!e) return 0;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_detect_hits();
+}
+
 ZkMetalStatus zkmsm_multi_msm(ZkMetalMSMEngine engine, const uint8_t* points, uint32_t n,
                               const uint8_t* scalar_sets, uint32_t k, uint8_t* results) {
     Engine* e = as_engine(engine);

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <thread>
 
@@ -466,9 +467,65 @@ int32_t Engine::msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t
     return msm_batch_host(points, scalars, counts, 1, out96);
 }
 
+// SURVEY-listed follow-up of zkMetal_ffi.swift:1241-1298: a batch whose
+// point slices are each a prefix of the resident SRS runs on the resident
+// table.  The check is exact: one 64-byte compare against the cached first
+// SRS base rejects unrelated inputs at once; otherwise every slice is compared
+// with the host image of the SRS (in parallel), which is far cheaper than
+// sending the bases over PCIe and rebuilding their table.
+bool Engine::points_are_srs_prefixes(const uint8_t* points, const uint32_t* counts, uint32_t nmsm) {
+    if (!srs_detect_ || srs_n_ == 0) return false;
+    struct Cmp { const uint8_t* a; uint64_t bytes; };
+    std::vector<Cmp> cmp;
+    uint64_t off = 0, total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        const uint64_t n = counts[j];
+        if (n > srs_n_) return false;
+        if (n == 0) continue;
+        if (memcmp(points + off, srs_first_, 64) != 0) return false;
+        cmp.push_back({points + off, n * 64});
+        off += n * 64;
+        total += n * 64;
+    }
+    if (cmp.empty()) return false;
+    if (srs_host_.size() != (uint64_t)srs_n_ * 64) {
+        std::vector<uint8_t> img((uint64_t)srs_n_ * 64);
+        if (srs_download(img.data(), srs_n_) != kOk) return false;
+        srs_host_.swap(img);
+    }
+    // split the bytes to compare into pieces for up to 16 host threads
+    struct Piece { const uint8_t* a; const uint8_t* b; uint64_t bytes; };
+    std::vector<Piece> pieces;
+    const uint64_t step = std::max<uint64_t>(1ull << 22, total / 16 + 1);
+    for (const Cmp& c : cmp)
+        for (uint64_t o = 0; o < c.bytes; o += step)
+            pieces.push_back({c.a + o, srs_host_.data() + o, std::min(step, c.bytes - o)});
+    std::atomic<bool> same{true};
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i = next++; i < pieces.size() && same.load(std::memory_order_relaxed); i = next++)
+            if (memcmp(pieces[i].a, pieces[i].b, pieces[i].bytes) != 0) same = false;
+    };
+    const uint32_t nt = (uint32_t)std::min<size_t>(16, pieces.size());
+    std::vector<std::thread> th;
+    for (uint32_t t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+    return same.load();
+}
+
+int32_t Engine::srs_set_first() {
+    srs_host_.clear();
+    return srs_download(srs_first_, 1);
+}
+
 int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
                                const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
     if (!points || !scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
+    if (points_are_srs_prefixes(points, counts, nmsm)) {
+        srs_hits_++;
+        return msm_resident(scalars, false, counts, nmsm, out96);
+    }
     std::vector<uint32_t> ns;
     std::vector<uint64_t> t0;
     std::vector<uint32_t> idx;
@@ -556,6 +613,8 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipGetLastError());
     srs_n_ = n;
+    srs_host_.assign(points, points + (uint64_t)n * 64);  // the caller's own bytes
+    memcpy(srs_first_, points, 64);
     return kOk;
 }
 
@@ -609,6 +668,8 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
     ZK_TRY(hipGetLastError());
     if (bad) return kInvalidInput;  // not canonical or not on the curve
     srs_n_ = (uint32_t)n;
+    const int32_t rc = srs_set_first();
+    if (rc != kOk) return rc;
     if (n_loaded) *n_loaded = srs_n_;
     return kOk;
 }
@@ -657,7 +718,7 @@ int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipGetLastError());
     srs_n_ = n;
-    return kOk;
+    return srs_set_first();
 }
 
 int32_t Engine::srs_download(uint8_t* points, uint32_t n) {

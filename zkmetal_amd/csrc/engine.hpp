@@ -110,6 +110,11 @@ class Engine {
     // G1 powers of an SRS file image (see engine.hip); max_points 0 = all
     int32_t srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_points, uint32_t* n_loaded);
     uint32_t srs_size() const { return srs_n_; }
+    // host-point calls whose bases are byte-for-byte prefixes of the resident
+    // SRS run on the resident table (no base upload, no table build); on by
+    // default
+    void set_srs_detect(bool on) { srs_detect_ = on; }
+    uint64_t srs_detect_hits() const { return srs_hits_; }
 
     // MSMs over prefixes of the resident table; scalars on host or device
     int32_t msm_resident(const void* scalars, bool scalars_on_device, const uint32_t* counts,
@@ -188,6 +193,15 @@ class Engine {
     DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
+    // host image of the resident bases (64 B each) for the prefix check:
+    // srs_first_ is filled whenever an SRS is set, srs_host_ on the first
+    // call whose first base matches it
+    std::vector<uint8_t> srs_host_;
+    uint8_t srs_first_[64] = {};
+    bool srs_detect_ = true;
+    uint64_t srs_hits_ = 0;
+    int32_t srs_set_first();
+    bool points_are_srs_prefixes(const uint8_t* points, const uint32_t* counts, uint32_t nmsm);
     hipEvent_t ev_[kNumStages + 1] = {};
 };
 
