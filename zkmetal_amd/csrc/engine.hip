@@ -99,6 +99,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     pl = Plan();
     pl.msms.resize(nmsm);
     pl.msm_scalar0.resize(nmsm + 1);
+    pl.msm_dtile0.resize(nmsm + 1);
     for (uint32_t j = 0; j < nmsm; j++) {
         if (ns[j] == 0) return kInvalidInput;
         uint32_t c = (cs && cs[j]) ? cs[j] : default_window_bits(ns[j]);
@@ -112,6 +113,8 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         m.seg0 = (uint32_t)pl.segs.size();
         m.form = form;
         pl.msm_scalar0[j] = pl.scalars;
+        pl.msm_dtile0[j] = pl.dtiles;
+        pl.dtiles += (2 * ns[j] + kSortTile - 1) / kSortTile;
         pl.scalars += ns[j];
         for (uint32_t w = 0; w < m.nwin; w++) {
             SegDesc s{};
@@ -133,6 +136,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         }
     }
     pl.msm_scalar0[nmsm] = pl.scalars;
+    pl.msm_dtile0[nmsm] = pl.dtiles;
     // 32-bit positions / bucket ids on the device
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
@@ -214,7 +218,8 @@ Engine::~Engine() {
 
 // pack the plan's descriptor arrays into one blob (16-byte aligned sections)
 struct DescLayout {
-    size_t msms, msm_scalar0, segs, seg_lane0, seg_grp0, seg_rcw0, tile_seg, chunk_seg, total;
+    size_t msms, msm_scalar0, msm_dtile0, segs, seg_lane0, seg_grp0, seg_rcw0, tile_seg, chunk_seg,
+        total;
 };
 
 static DescLayout layout_for(const Plan& pl) {
@@ -227,6 +232,7 @@ static DescLayout layout_for(const Plan& pl) {
     };
     L.msms = take(pl.msms.size() * sizeof(MsmDesc));
     L.msm_scalar0 = take(pl.msm_scalar0.size() * 8);
+    L.msm_dtile0 = take(pl.msm_dtile0.size() * 8);
     L.segs = take(pl.segs.size() * sizeof(SegDesc));
     L.seg_lane0 = take(pl.seg_lane0.size() * 8);
     L.seg_grp0 = take(pl.seg_grp0.size() * 8);
@@ -269,6 +275,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     uint8_t* hb = (uint8_t*)hdesc_.p;
     memcpy(hb + L.msms, pl.msms.data(), pl.msms.size() * sizeof(MsmDesc));
     memcpy(hb + L.msm_scalar0, pl.msm_scalar0.data(), pl.msm_scalar0.size() * 8);
+    memcpy(hb + L.msm_dtile0, pl.msm_dtile0.data(), pl.msm_dtile0.size() * 8);
     memcpy(hb + L.segs, pl.segs.data(), pl.segs.size() * sizeof(SegDesc));
     memcpy(hb + L.seg_lane0, pl.seg_lane0.data(), pl.seg_lane0.size() * 8);
     memcpy(hb + L.seg_grp0, pl.seg_grp0.data(), pl.seg_grp0.size() * 8);
@@ -278,6 +285,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     uint8_t* db = (uint8_t*)desc_.p;
     const MsmDesc* d_msms = (const MsmDesc*)(db + L.msms);
     const uint64_t* d_msm_scalar0 = (const uint64_t*)(db + L.msm_scalar0);
+    const uint64_t* d_msm_dtile0 = (const uint64_t*)(db + L.msm_dtile0);
     const SegDesc* d_segs = (const SegDesc*)(db + L.segs);
     const uint64_t* d_seg_lane0 = (const uint64_t*)(db + L.seg_lane0);
     const uint64_t* d_seg_grp0 = (const uint64_t*)(db + L.seg_grp0);
@@ -296,12 +304,13 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
         launch_fill_zero(err_.p, 16, st);
         ZK_TRY(mark(0));
-        launch_decompose(d_msms, d_msm_scalar0, nmsm, d_segs, d_scalars, d_table, table_n, pl.scalars,
-                         digits_.p, d_err, st);
+        // the decomposition also counts the first sort pass's tile histograms
+        launch_decompose(d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
+                         digits_.p, (uint32_t*)hist_.p, d_err, st);
         ZK_TRY(mark(1));
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
                          nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
-                         (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st);
+                         (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st, false);
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
                          (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
                          (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);

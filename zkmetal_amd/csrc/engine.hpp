@@ -65,6 +65,9 @@ struct Timing {
 struct Plan {
     std::vector<MsmDesc> msms;
     std::vector<uint64_t> msm_scalar0;  // nmsm + 1
+    // nmsm + 1: first decomposition block of each MSM; a block decomposes
+    // the kSortTile / 2 scalars of one first-pass sort tile of every window
+    std::vector<uint64_t> msm_dtile0;
     std::vector<SegDesc> segs;
     std::vector<uint64_t> seg_lane0;  // nseg + 1
     std::vector<uint64_t> seg_grp0;   // nseg + 1: first row/column sum of the segment
@@ -72,7 +75,7 @@ struct Plan {
     std::vector<uint32_t> tile_seg;
     std::vector<uint32_t> chunk_seg;
     uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0, rc_waves = 0;
-    uint32_t tiles = 0, chunks = 0, T = 0;
+    uint32_t tiles = 0, chunks = 0, T = 0, dtiles = 0;
     uint32_t G = 8;  // buckets per bucket-reduction lane (a power of 2, 8..32)
 };
 

@@ -181,14 +181,16 @@ __device__ inline GlvSplit glv_split(const uint32_t kin[8]) {
 }
 
 // c-bit window w of a 128-bit value held in 4 little-endian limbs
+// bits [w c, w c + c) of the 128-bit v, from two 64-bit halves by shifts (an
+// indexed register array would be placed in scratch memory)
 __device__ __forceinline__ uint32_t window_bits128(const uint32_t v[4], uint32_t w, uint32_t c) {
-    uint32_t off = w * c, limb = off >> 5, sh = off & 31u;
-    uint32_t x = 0;
-    if (limb < 4) {
-        x = v[limb] >> sh;
-        if (sh + c > 32u && limb + 1 < 4) x |= v[limb + 1] << (32u - sh);
-    }
-    return x & ((1u << c) - 1u);
+    const uint64_t lo = ((uint64_t)v[1] << 32) | v[0], hi = ((uint64_t)v[3] << 32) | v[2];
+    const uint32_t off = w * c;
+    uint64_t x;
+    if (off >= 128u) x = 0;
+    else if (off >= 64u) x = hi >> (off - 64u);
+    else x = (lo >> off) | (off ? hi << (64u - off) : 0ull);
+    return (uint32_t)x & ((1u << c) - 1u);
 }
 
 // Digit word layout (one per window per half-scalar):

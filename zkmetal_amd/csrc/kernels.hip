@@ -267,43 +267,74 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
 
 // --------------------------------------------------- GLV + signed digits
 
-__global__ __launch_bounds__(256) void k_decompose(const MsmDesc* __restrict__ msms,
-                                                   const uint64_t* __restrict__ msm_scalar0,
-                                                   uint32_t nmsm, const SegDesc* __restrict__ segs,
-                                                   const uint4* __restrict__ scalars,
-                                                   const uint64_t* __restrict__ idmask,
-                                                   uint64_t total, uint2* __restrict__ digits,
-                                                   uint32_t* __restrict__ err) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    uint32_t j = upper_index(msm_scalar0, nmsm, g);
+// Block b decomposes the kSortTile / 2 scalars [tin * 4096, ...) of MSM j
+// (msm_dtile0[j] <= b < msm_dtile0[j+1], tin = b - msm_dtile0[j]): their
+// entries are first-pass sort tile tin of every window's segment, so the
+// block also counts that tile's 256-bin histogram of key byte 0 for each
+// window (LDS atomics) and writes it where the scan expects it; the first
+// pass then needs no histogram kernel.
+constexpr int kDecThreads = 256;
+constexpr int kDecPerThread = kSortTile / 2 / kDecThreads;  // scalars per thread
+
+__global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __restrict__ msms,
+                                                          const uint64_t* __restrict__ msm_dtile0,
+                                                          uint32_t nmsm,
+                                                          const SegDesc* __restrict__ segs,
+                                                          const uint4* __restrict__ scalars,
+                                                          const uint64_t* __restrict__ idmask,
+                                                          uint2* __restrict__ digits,
+                                                          uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ err) {
+    // windows (nwin <= 64) x key byte 0, two 16-bit counts per word (a tile
+    // has at most kSortTile = 8192 entries, so no count reaches 2^16)
+    __shared__ uint32_t cnt[64 * 128];
+    const uint32_t b = blockIdx.x;
+    const uint32_t j = upper_index(msm_dtile0, nmsm, b);
     const MsmDesc m = msms[j];
-    uint64_t i = g - m.scalar0;
-    uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
-    uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-    if (m.form == 1) fr_from_mont(k);
-    const uint64_t pt = (m.table0 >> 1) + i;  // the base's point index in the table
-    if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) k[j] = 0;  // identity base: no digits
-    }
-    GlvSplit sp = glv_split(k);
+    const uint32_t tin = b - (uint32_t)msm_dtile0[j];
+    for (uint32_t x = threadIdx.x; x < m.nwin * 128u; x += kDecThreads) cnt[x] = 0;
+    __syncthreads();
     const uint32_t c = m.c, half = 1u << (c - 1), full = 1u << c;
-    uint32_t carry1 = 0, carry2 = 0;
-    for (uint32_t w = 0; w < m.nwin; w++) {
-        uint32_t d1 = window_bits128(sp.k1, w, c) + carry1;
-        uint32_t d2 = window_bits128(sp.k2, w, c) + carry2;
-        uint32_t neg1 = sp.neg1 ? 1u : 0u, neg2 = sp.neg2 ? 1u : 0u;
-        carry1 = d1 > half;
-        carry2 = d2 > half;
-        if (carry1) { d1 = full - d1; neg1 ^= 1u; }
-        if (carry2) { d2 = full - d2; neg2 ^= 1u; }
-        uint32_t w1 = d1 ? (key_encode(d1 - 1u) | (neg1 << 31)) : ZK_KEY_NONE;
-        uint32_t w2 = d2 ? (key_encode(d2 - 1u) | (neg2 << 31)) : ZK_KEY_NONE;
-        const uint64_t e = segs[m.seg0 + w].ent0 + 2 * i;
-        digits[e >> 1] = make_uint2(w1, w2);
+    bool carry_left = false;
+    for (int r = 0; r < kDecPerThread; r++) {
+        const uint64_t i = (uint64_t)tin * (kSortTile / 2) + (uint64_t)r * kDecThreads + threadIdx.x;
+        if (i >= m.n) break;
+        const uint64_t g = m.scalar0 + i;
+        uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
+        uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+        if (m.form == 1) fr_from_mont(k);
+        const uint64_t pt = (m.table0 >> 1) + i;  // the base's point index in the table
+        if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) k[q] = 0;  // identity base: no digits
+        }
+        GlvSplit sp = glv_split(k);
+        uint32_t carry1 = 0, carry2 = 0;
+        for (uint32_t w = 0; w < m.nwin; w++) {
+            uint32_t d1 = window_bits128(sp.k1, w, c) + carry1;
+            uint32_t d2 = window_bits128(sp.k2, w, c) + carry2;
+            uint32_t neg1 = sp.neg1 ? 1u : 0u, neg2 = sp.neg2 ? 1u : 0u;
+            carry1 = d1 > half;
+            carry2 = d2 > half;
+            if (carry1) { d1 = full - d1; neg1 ^= 1u; }
+            if (carry2) { d2 = full - d2; neg2 ^= 1u; }
+            uint32_t w1 = d1 ? (key_encode(d1 - 1u) | (neg1 << 31)) : ZK_KEY_NONE;
+            uint32_t w2 = d2 ? (key_encode(d2 - 1u) | (neg2 << 31)) : ZK_KEY_NONE;
+            const uint64_t e = segs[m.seg0 + w].ent0 + 2 * i;
+            digits[e >> 1] = make_uint2(w1, w2);
+            atomicAdd(&cnt[w * 128 + ((w1 & 0xffu) >> 1)], 1u << ((w1 & 1u) * 16));
+            atomicAdd(&cnt[w * 128 + ((w2 & 0xffu) >> 1)], 1u << ((w2 & 1u) * 16));
+        }
+        carry_left |= (carry1 | carry2) != 0;
     }
-    if (carry1 | carry2) atomicOr(err, 1u);
+    if (carry_left) atomicOr(err, 1u);
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < m.nwin * 256u; x += kDecThreads) {
+        const uint32_t w = x >> 8, d = x & 255u;
+        const SegDesc sg = segs[m.seg0 + w];
+        hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin] =
+            (cnt[w * 128 + (d >> 1)] >> ((d & 1u) * 16)) & 0xffffu;
+    }
 }
 
 // ------------------------------------------------------ radix sort passes
@@ -1135,25 +1166,26 @@ void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st
                        (const uint4*)pts, (uint4*)table, table_idmask(table, n), n);
 }
 
-void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,
-                      const SegDesc* segs, const void* scalars, const void* table,
-                      uint64_t table_n, uint64_t total, void* digits, uint32_t* err,
-                      hipStream_t st) {
-    if (!total) return;
-    hipLaunchKernelGGL(k_decompose, dim3(blocks_for(total, 256)), dim3(256), 0, st, msms,
-                       msm_scalar0, nmsm, segs, (const uint4*)scalars,
-                       (const uint64_t*)table_idmask(const_cast<void*>(table), table_n), total,
-                       (uint2*)digits, err);
+void launch_decompose(const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
+                      uint32_t dtiles, const SegDesc* segs, const void* scalars,
+                      const void* table, uint64_t table_n, void* digits, uint32_t* hist,
+                      uint32_t* err, hipStream_t st) {
+    if (!dtiles) return;
+    hipLaunchKernelGGL(k_decompose, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+                       nmsm, segs, (const uint4*)scalars,
+                       (const uint64_t*)table_idmask(const_cast<void*>(table), table_n),
+                       (uint2*)digits, hist, err);
 }
 
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
                       uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
                       const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
                       uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
-                      uint32_t* out_vals, hipStream_t st) {
+                      uint32_t* out_vals, hipStream_t st, bool count) {
     if (!ntiles) return;
-    hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
-                       in_keys, pass, (const uint32_t*)seg_valid, hist);
+    if (count)  // else the tile histograms are already in hist
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
+                           in_keys, pass, (const uint32_t*)seg_valid, hist);
     hipLaunchKernelGGL(k_scan_up, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg,
                        (const uint32_t*)hist, part);
     hipLaunchKernelGGL(k_scan_spine, dim3(nseg), dim3(256), 0, st, segs, part);

@@ -14,15 +14,17 @@ inline uint64_t table_bytes(uint64_t n) { return n * 2 * kTableEntryBytes + (n +
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
 void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);
-void launch_decompose(const MsmDesc* msms, const uint64_t* msm_scalar0, uint32_t nmsm,
-                      const SegDesc* segs, const void* scalars, const void* table,
-                      uint64_t table_n, uint64_t total, void* digits, uint32_t* err,
-                      hipStream_t st);
+// one block per first-pass sort tile of each MSM (msm_dtile0, dtiles blocks):
+// digits, and the first sort pass's tile histograms into hist
+void launch_decompose(const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
+                      uint32_t dtiles, const SegDesc* segs, const void* scalars,
+                      const void* table, uint64_t table_n, void* digits, uint32_t* hist,
+                      uint32_t* err, hipStream_t st);
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
                       uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
                       const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
                       uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
-                      uint32_t* out_vals, hipStream_t st);
+                      uint32_t* out_vals, hipStream_t st, bool count = true);
 void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 // lanes of k_accumulate resident at once on the device (CUs x workgroups per
 // CU at its register use x 256): a lane count just above it leaves a second,
