@@ -275,6 +275,7 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
 // pass then needs no histogram kernel.
 constexpr int kDecThreads = 256;
 constexpr int kDecPerThread = kSortTile / 2 / kDecThreads;  // scalars per thread
+static_assert(kSortTile <= (1 << 15) && kDecPerThread >= 1, "decomposition tiling");
 
 __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __restrict__ msms,
                                                           const uint64_t* __restrict__ msm_dtile0,
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
                                                           uint32_t* __restrict__ hist,
                                                           uint32_t* __restrict__ err) {
     // windows (nwin <= 64) x key byte 0, two 16-bit counts per word (a tile
-    // has at most kSortTile = 8192 entries, so no count reaches 2^16)
+    // has at most kSortTile <= 2^15 entries, so no count reaches 2^16)
     __shared__ uint32_t cnt[64 * 128];
     const uint32_t b = blockIdx.x;
     const uint32_t j = upper_index(msm_dtile0, nmsm, b);
