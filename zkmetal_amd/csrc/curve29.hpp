@@ -29,82 +29,6 @@ __device__ __forceinline__ Fq fq_zero() {
     return r;
 }
 
-// R mod p, R = 2^261
-__device__ __forceinline__ Fq fq_one() {
-    Fq r;
-    r.v[0] = 0x157ccc21u; r.v[1] = 0x141c2758u; r.v[2] = 0x185230d3u; r.v[3] = 0x014c0419u;
-    r.v[4] = 0x0aa36fb9u; r.v[5] = 0x1d4240ceu; r.v[6] = 0x11d54c07u; r.v[7] = 0x052ac7a8u;
-    r.v[8] = 0x000dc836u;
-    return r;
-}
-
-// beta * R mod p (phi(x, y) = (beta x, y))
-__device__ __forceinline__ Fq fq_beta() {
-    Fq r;
-    r.v[0] = 0x0a337995u; r.v[1] = 0x158d1d23u; r.v[2] = 0x189c9b98u; r.v[3] = 0x12fa4e45u;
-    r.v[4] = 0x185faadcu; r.v[5] = 0x0176f16du; r.v[6] = 0x0eed93bau; r.v[7] = 0x14291140u;
-    r.v[8] = 0x000c0afeu;
-    return r;
-}
-
-// a + b, normalised limbs (no reduction: the bound is the sum of the bounds)
-__device__ __forceinline__ Fq fq_add(const Fq& a, const Fq& b) {
-    Fq r;
-    uint32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const uint32_t s = a.v[i] + b.v[i] + c;
-        r.v[i] = s & ZQ_MASK;
-        c = s >> 29;
-    }
-    return r;
-}
-
-// a - b + K*p for b < K*p: in [a - b + (K p - b), a + K p)
-template <int K>
-__device__ __forceinline__ Fq fq_sub(const Fq& a, const Fq& b) {
-    Fq r;
-    int32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const int32_t s = (int32_t)(a.v[i] + q_kp<K>(i)) - (int32_t)b.v[i] + c;
-        r.v[i] = (uint32_t)s & ZQ_MASK;
-        c = s >> 29;  // arithmetic: -1, 0 or 1
-    }
-    return r;
-}
-
-// a >= K*p ? a - K*p : a
-template <int K>
-__device__ __forceinline__ Fq fq_csub(const Fq& a) {
-    Fq d;
-    int32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const int32_t s = (int32_t)a.v[i] - (int32_t)q_kp<K>(i) + c;
-        d.v[i] = (uint32_t)s & ZQ_MASK;
-        c = s >> 29;
-    }
-    const bool keep = c < 0;
-    Fq r;
-#pragma unroll
-    for (int i = 0; i < 9; i++) r.v[i] = keep ? a.v[i] : d.v[i];
-    return r;
-}
-
-// a == 0 mod p for a < 2p.  The empty asm keeps the two OR-reductions whole:
-// without it the compiler splits "z == 0" into one compare per limb.
-__device__ __forceinline__ bool fq_is_zero2(const Fq& a) {
-    uint32_t z = 0, e = 0;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        z |= a.v[i];
-        e |= a.v[i] ^ q_p(i);
-    }
-    asm("" : "+v"(z), "+v"(e));
-    return z == 0 || e == 0;
-}
-
 __device__ __forceinline__ Fq fq_select(bool c, const Fq& a, const Fq& b) {
     Fq r;
 #pragma unroll
@@ -121,8 +45,6 @@ __device__ __forceinline__ QXyzz qxyzz_identity() {
     return r;
 }
 
-__device__ __forceinline__ bool qxyzz_is_id(const QXyzz& p) { return fq_is_zero2(p.ZZ); }
-
 __device__ __forceinline__ bool qaff_is_id(const QAff& q) {
     uint32_t x = 0;
 #pragma unroll
@@ -136,163 +58,6 @@ __device__ __forceinline__ QXyzz qxyzz_select(bool c, const QXyzz& a, const QXyz
     r.Y = fq_select(c, a.Y, b.Y);
     r.ZZ = fq_select(c, a.ZZ, b.ZZ);
     r.ZZZ = fq_select(c, a.ZZZ, b.ZZZ);
-    return r;
-}
-
-// doubling core: X, Y < 4p.  Returns X3 < 2.5p, Y3 < 3.5p; V, W < 1.5p.
-__device__ __forceinline__ void qdbl_core(const Fq& X, const Fq& Y, Fq& X3, Fq& Y3, Fq& V, Fq& W) {
-    const Fq U = fq_add(Y, Y);  // < 8p
-    V = fq_sqr(U);
-    W = fq_mul(U, V);
-    const Fq S = fq_mul(X, V);
-    const Fq xx = fq_sqr(X);
-    const Fq M = fq_add(fq_add(xx, xx), xx);                    // < 4.5p
-    X3 = fq_csub<2>(fq_sub<3>(fq_sqr(M), fq_add(S, S)));        // < 4.5p -> < 2.5p
-    Y3 = fq_sub<2>(fq_mul(M, fq_sub<3>(S, X3)), fq_mul(W, Y));  // < 3.5p
-}
-
-// 2*(x, y) for an affine, non-identity point
-__device__ __forceinline__ QXyzz qxyzz_dbl_aff(const Fq& x, const Fq& y) {
-    QXyzz r;
-    qdbl_core(x, y, r.X, r.Y, r.ZZ, r.ZZZ);
-    return r;
-}
-
-// 2*P; identity stays identity (ZZ3 = V*ZZ1 = 0 mod p)
-__device__ __forceinline__ QXyzz qxyzz_dbl(const QXyzz& p) {
-    QXyzz r;
-    Fq V, W;
-    qdbl_core(p.X, p.Y, r.X, r.Y, V, W);
-    r.ZZ = fq_mul(V, p.ZZ);
-    r.ZZZ = fq_mul(W, p.ZZZ);
-    return r;
-}
-
-// acc += (x2, y2); x2, y2 < p.  q_id: the affine point is (0,0).
-__device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y2, bool q_id) {
-    const bool p_id = qxyzz_is_id(acc);
-    const Fq U2 = fq_mul(x2, acc.ZZ);
-    const Fq S2 = fq_mul(y2, acc.ZZZ);
-    const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
-    const Fq R = fq_sub<4>(S2, acc.Y);
-    const Fq PP = fq_sqr(P);
-    const Fq PPP = fq_mul(P, PP);
-    const Fq Q = fq_mul(acc.X, PP);
-    const Fq RR = fq_sqr(R);
-    const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
-    QXyzz r;
-    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
-    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(acc.Y, PPP));  // < 3.5p
-    r.ZZ = fq_mul(acc.ZZ, PP);
-    r.ZZZ = fq_mul(acc.ZZZ, PPP);
-    if (__builtin_expect(dbl, 0)) r = qxyzz_dbl_aff(x2, y2);
-    // P == -Q falls out of the formulas: P = 0 gives ZZ3 = 0 mod p, the identity.
-    QXyzz q;
-    q.X = x2;
-    q.Y = y2;
-    q.ZZ = fq_one();
-    q.ZZZ = q.ZZ;
-    r = qxyzz_select(p_id, q, r);
-    acc = qxyzz_select(q_id, acc, r);
-}
-
-// Placeholder for an accumulator that starts a bucket run: the value is never
-// used (the run's first addition selects the point itself), but ZZ = ZZZ = 1
-// and X = 0 keep P = x2 - X away from 0, so the first addition does not look
-// exceptional.
-__device__ __forceinline__ QXyzz qxyzz_run_start() {
-    QXyzz r;
-    r.X = fq_zero();
-    r.Y = fq_zero();
-    r.ZZ = fq_one();
-    r.ZZZ = r.ZZ;
-    return r;
-}
-
-// acc += (x2, y2) in the bucket-accumulation loop.  acc_id says acc is the
-// identity (a run start, or a cancellation earlier in the run); (x2, y2) is
-// never the identity there.  X3 is left below 6.5p (no final subtraction):
-// every later use of X takes it as a multiplicand or as the subtrahend of a
-// K = 7 form.  Doubling (acc == q) and cancellation (acc == -q) both have
-// PP = 0 mod p; PP < 1.6p is then 0 or p, so its low limb is 0 or p_0.  That one-limb test screens every step; the exact tests run only in
-// lanes that pass it, and a wave branches to them only if one of its lanes
-// does.  After a cancellation acc holds ZZ = 0 mod p, the stored form of the
-// identity.
-__device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
-                                               const Fq& y2) {
-    const Fq P = fq_mul_sub<7>(x2, acc.ZZ, acc.X);   // U2 - X1 + 7p < 8.5p
-    const Fq R = fq_mul_sub<5>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 5p < 6.5p
-    const Fq PP = fq_sqr(P);
-    const Fq PPP = fq_mul(P, PP);
-    const Fq Q = fq_mul(acc.X, PP);
-    QXyzz r;
-    Fq T;  // 2Q + PPP < 4.5p, limbs sums of three normalised limbs
-#pragma unroll
-    for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
-    r.X = fq_sqr_sub<5, 3>(R, T);                                        // < 6.4p
-    r.Y = fq_mul_sub<2>(R, fq_sub_lazy<7>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
-    r.ZZ = fq_mul(acc.ZZ, PP);
-    r.ZZZ = fq_mul(acc.ZZZ, PPP);
-    bool cancel = false;
-    if (__builtin_expect(PP.v[0] == 0u || PP.v[0] == q_p(0), 0)) {
-        const bool z = !acc_id && fq_is_zero2(PP);
-        const bool d = z && fq_is_zero2(fq_sqr(R));  // R^2 < 1.5p
-        cancel = z && !d;
-        if (d) r = qxyzz_dbl_aff(x2, y2);
-    }
-    QXyzz q;
-    q.X = x2;
-    q.Y = y2;
-    q.ZZ = fq_one();
-    q.ZZZ = q.ZZ;
-    acc = qxyzz_select(acc_id, q, r);
-    acc_id = cancel;
-}
-
-// acc += q (both XYZZ)
-__device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
-    const bool p_id = qxyzz_is_id(acc);
-    const bool q_id = qxyzz_is_id(q);
-    const Fq U1 = fq_mul(acc.X, q.ZZ);
-    const Fq U2 = fq_mul(q.X, acc.ZZ);
-    const Fq S1 = fq_mul(acc.Y, q.ZZZ);
-    const Fq S2 = fq_mul(q.Y, acc.ZZZ);
-    const Fq P = fq_sub<2>(U2, U1);  // < 3.5p
-    const Fq R = fq_sub<2>(S2, S1);
-    const Fq PP = fq_sqr(P);
-    const Fq PPP = fq_mul(P, PP);
-    const Fq Q = fq_mul(U1, PP);
-    const Fq RR = fq_sqr(R);
-    const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
-    QXyzz r;
-    r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));
-    r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(S1, PPP));
-    r.ZZ = fq_mul(fq_mul(acc.ZZ, q.ZZ), PP);
-    r.ZZZ = fq_mul(fq_mul(acc.ZZZ, q.ZZZ), PPP);
-    if (__builtin_expect(dbl, 0)) r = qxyzz_dbl(acc);
-    r = qxyzz_select(p_id, q, r);
-    acc = qxyzz_select(q_id, acc, r);
-}
-
-// ---- conversions between the two forms (canonical 32-bit form: [0, p),
-// R = 2^256; internal: R = 2^261)
-__device__ __forceinline__ Fq fq_from_fp_canon(const Fp& a) { return fq_csub<1>(fq_from_fp(a)); }
-
-__device__ __forceinline__ Xyzz qxyzz_to_xyzz(const QXyzz& a) {
-    Xyzz r;
-    r.X = fq_to_fp(a.X);
-    r.Y = fq_to_fp(a.Y);
-    r.ZZ = fq_to_fp(a.ZZ);
-    r.ZZZ = fq_to_fp(a.ZZZ);
-    return r;
-}
-
-__device__ __forceinline__ QXyzz qxyzz_from_xyzz(const Xyzz& a) {
-    QXyzz r;
-    r.X = fq_from_fp_canon(fp_canon(a.X));
-    r.Y = fq_from_fp_canon(fp_canon(a.Y));
-    r.ZZ = fq_from_fp_canon(fp_canon(a.ZZ));
-    r.ZZZ = fq_from_fp_canon(fp_canon(a.ZZZ));
     return r;
 }
 
@@ -317,24 +82,6 @@ __device__ __forceinline__ QAff load_qtab(const uint4* base, uint64_t idx, bool 
     a.x.v[8] = h.x;
     a.y.v[8] = h.y;
     return a;
-}
-
-__device__ __forceinline__ void store_qtab(uint4* base, uint64_t idx, const QAff& a) {
-    uint32_t w[32];
-    const Fq ny = fq_csub<1>(fq_sub<1>(fq_zero(), a.y));  // p - y; 0 stays 0
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        w[i] = a.x.v[i];
-        w[8 + i] = a.y.v[i];
-        w[16 + i] = ny.v[i];
-    }
-    w[24] = a.x.v[8]; w[25] = a.y.v[8];
-    w[26] = a.x.v[8]; w[27] = ny.v[8];
-#pragma unroll
-    for (int i = 28; i < 32; i++) w[i] = 0;
-    uint4* p = base + idx * 8;
-#pragma unroll
-    for (int i = 0; i < 8; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 // Buckets, merge records and reduction partials: the 36 limbs as they are
@@ -373,5 +120,382 @@ __device__ __forceinline__ void store_qxyzz(uint4* base, uint64_t idx, const QXy
 #pragma unroll
     for (int i = 0; i < 9; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
+
+// The 29-bit field arithmetic over the field F (field29.hpp) and the XYZZ
+// curve arithmetic on y^2 = x^3 + b over it (b enters no formula here).
+template <class F>
+struct Arith {
+    // limb i of K*p, normalised (folded at compile time inside unrolled loops)
+    template <int K>
+    static __device__ __forceinline__ uint32_t q_kp(int i) {
+        uint64_t c = 0;
+        uint32_t r = 0;
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            const uint64_t s = (uint64_t)K * F::p(j) + c;
+            if (j == i) r = (uint32_t)s & ZQ_MASK;
+            c = s >> 29;
+        }
+        return r;
+    }
+
+    // K*p with B * 2^29 borrowed into every limb below the top one: limb 0 is
+    // kp_0 + B 2^29, limbs 1..7 kp_i + B 2^29 - B, limb 8 kp_8 - B (same value).
+    // For a c whose limbs are sums of at most B normalised limbs (c_i <=
+    // B (2^29 - 1)) and c_8 <= kp_8 - B, every limb of (this - c) is >= 0, so
+    // K*p - c needs no borrow propagation.
+    template <int K, int B = 1>
+    static __device__ __forceinline__ uint32_t q_kpr(int i) {
+        const uint32_t c = q_kp<K>(i);
+        return i == 0 ? c + (uint32_t)B * (1u << 29)
+                      : (i < 8 ? c + (uint32_t)B * (1u << 29) - (uint32_t)B : c - (uint32_t)B);
+    }
+
+    // Q - c + K*p limb by limb with no carry propagation (limbs < 2^29 + B 2^29,
+    // value < Q + K*p): a multiplicand for fq_mul_sub / fq_mul whose other operand
+    // has normalised limbs.  Q normalised; c as for q_kpr<K, B>.
+    template <int K, int B = 1>
+    static __device__ __forceinline__ Fq fq_sub_lazy(const Fq& q, const Fq& c) {
+        Fq r;
+#pragma unroll
+        for (int i = 0; i < 9; i++) r.v[i] = q.v[i] + q_kpr<K, B>(i) - c.v[i];
+        return r;
+    }
+
+    // a * b / 2^261 - c + K*p mod p (K > 0), or a * b / 2^261 (K == 0):
+    // result < ab/2^261 + p + K*p with normalised limbs.  The limbs of K*p - c
+    // (q_kpr<K, B>) enter the high columns of the product, where they cost one
+    // subtraction each instead of a separate normalised subtraction.
+    // a, b: limbs < 2^30, or a normalised and b limbs < 1.5 * 2^30 (column sums
+    // then stay below 2^63.3); values with ab < p * 2^261 / 2 (e.g. both < 8p);
+    // c (K > 0): as for q_kpr<K, B>.
+    template <int K, int B = 1>
+    static __device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) {
+        uint32_t m[9];
+        Fq t;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(acc, a.v[j], b.v[k - j]);
+                qmac(acc, m[j], F::p(k - j));
+            }
+            qmac(acc, a.v[k], b.v[0]);
+            m[k] = ((uint32_t)acc * F::kPinv) & ZQ_MASK;
+            qmac(acc, m[k], F::p(0));
+            acc >>= 29;
+            carry_fence(acc);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+            if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(acc, a.v[j], b.v[k - j]);
+                qmac(acc, m[j], F::p(k - j));
+            }
+            t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
+            acc >>= 29;
+            carry_fence(acc);
+        }
+        t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
+        return t;
+    }
+
+    // a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
+    // a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
+    static __device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) { return fq_mul_sub<0>(a, b, a); }
+
+    // a^2 / 2^261 mod p: the symmetric products a_i a_j (i != j) once, from the
+    // doubled limb 2 a_i (< 2^30), plus the squares: 45 + 81 multiply-adds.
+    // Column bound: 4 doubled products < 2^59, one square < 2^58, 9 m*p products
+    // < 2^58 and the carry: < 18 * 2^58 < 2^63.  a: normalised limbs, value < 8p.
+    // fq_sqr_sub<K, B>(a, c) = a^2 / 2^261 - c + K*p, as fq_mul_sub.
+    template <int K, int B = 1>
+    static __device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
+        uint32_t a2[9], m[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) a2[i] = a.v[i] << 1;
+        Fq t;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int i = 0; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
+            if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
+#pragma unroll
+            for (int j = 0; j < k; j++) qmac(acc, m[j], F::p(k - j));
+            m[k] = ((uint32_t)acc * F::kPinv) & ZQ_MASK;
+            qmac(acc, m[k], F::p(0));
+            acc >>= 29;
+            carry_fence(acc);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+            if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
+#pragma unroll
+            for (int i = k - 8; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
+            if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) qmac(acc, m[j], F::p(k - j));
+            t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
+            acc >>= 29;
+            carry_fence(acc);
+        }
+        t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
+        return t;
+    }
+
+    static __device__ __forceinline__ Fq fq_sqr(const Fq& a) { return fq_sqr_sub<0>(a, a); }
+    // x * 2^256 mod p (the 32-bit-limb Montgomery form) -> x * 2^261 mod p
+    static __device__ __forceinline__ Fq fq_from_fp(const Fp& a) { return fq_mul(fq_slice(a), F::from32()); }
+
+    // value < 8p with normalised limbs -> [0, p) in the 32-bit form (product < 1.1p)
+    static __device__ __forceinline__ Fp fq_to_fp(const Fq& a) {
+        return reduce_once32<F>(fq_pack(fq_mul(a, F::to32())));
+    }
+
+    // R mod p, R = 2^261
+    static __device__ __forceinline__ Fq fq_one() { return F::one(); }
+
+    // beta * R mod p (phi(x, y) = (beta x, y)); BN254 G1 only
+    static __device__ __forceinline__ Fq fq_beta() { return F::beta(); }
+
+    // a + b, normalised limbs (no reduction: the bound is the sum of the bounds)
+    static __device__ __forceinline__ Fq fq_add(const Fq& a, const Fq& b) {
+        Fq r;
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const uint32_t s = a.v[i] + b.v[i] + c;
+            r.v[i] = s & ZQ_MASK;
+            c = s >> 29;
+        }
+        return r;
+    }
+
+    // a - b + K*p for b < K*p: in [a - b + (K p - b), a + K p)
+    template <int K>
+    static __device__ __forceinline__ Fq fq_sub(const Fq& a, const Fq& b) {
+        Fq r;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int32_t s = (int32_t)(a.v[i] + q_kp<K>(i)) - (int32_t)b.v[i] + c;
+            r.v[i] = (uint32_t)s & ZQ_MASK;
+            c = s >> 29;  // arithmetic: -1, 0 or 1
+        }
+        return r;
+    }
+
+    // a >= K*p ? a - K*p : a
+    template <int K>
+    static __device__ __forceinline__ Fq fq_csub(const Fq& a) {
+        Fq d;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int32_t s = (int32_t)a.v[i] - (int32_t)q_kp<K>(i) + c;
+            d.v[i] = (uint32_t)s & ZQ_MASK;
+            c = s >> 29;
+        }
+        const bool keep = c < 0;
+        Fq r;
+#pragma unroll
+        for (int i = 0; i < 9; i++) r.v[i] = keep ? a.v[i] : d.v[i];
+        return r;
+    }
+
+    // a == 0 mod p for a < 2p.  The empty asm keeps the two OR-reductions whole:
+    // without it the compiler splits "z == 0" into one compare per limb.
+    static __device__ __forceinline__ bool fq_is_zero2(const Fq& a) {
+        uint32_t z = 0, e = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            z |= a.v[i];
+            e |= a.v[i] ^ F::p(i);
+        }
+        asm("" : "+v"(z), "+v"(e));
+        return z == 0 || e == 0;
+    }
+
+    static __device__ __forceinline__ bool qxyzz_is_id(const QXyzz& p) { return fq_is_zero2(p.ZZ); }
+
+    // doubling core: X, Y < 4p.  Returns X3 < 2.5p, Y3 < 3.5p; V, W < 1.5p.
+    static __device__ __forceinline__ void qdbl_core(const Fq& X, const Fq& Y, Fq& X3, Fq& Y3, Fq& V, Fq& W) {
+        const Fq U = fq_add(Y, Y);  // < 8p
+        V = fq_sqr(U);
+        W = fq_mul(U, V);
+        const Fq S = fq_mul(X, V);
+        const Fq xx = fq_sqr(X);
+        const Fq M = fq_add(fq_add(xx, xx), xx);                    // < 4.5p
+        X3 = fq_csub<2>(fq_sub<3>(fq_sqr(M), fq_add(S, S)));        // < 4.5p -> < 2.5p
+        Y3 = fq_sub<2>(fq_mul(M, fq_sub<3>(S, X3)), fq_mul(W, Y));  // < 3.5p
+    }
+
+    // 2*(x, y) for an affine, non-identity point
+    static __device__ __forceinline__ QXyzz qxyzz_dbl_aff(const Fq& x, const Fq& y) {
+        QXyzz r;
+        qdbl_core(x, y, r.X, r.Y, r.ZZ, r.ZZZ);
+        return r;
+    }
+
+    // 2*P; identity stays identity (ZZ3 = V*ZZ1 = 0 mod p)
+    static __device__ __forceinline__ QXyzz qxyzz_dbl(const QXyzz& p) {
+        QXyzz r;
+        Fq V, W;
+        qdbl_core(p.X, p.Y, r.X, r.Y, V, W);
+        r.ZZ = fq_mul(V, p.ZZ);
+        r.ZZZ = fq_mul(W, p.ZZZ);
+        return r;
+    }
+
+    // acc += (x2, y2); x2, y2 < p.  q_id: the affine point is (0,0).
+    static __device__ __forceinline__ void qxyzz_madd(QXyzz& acc, const Fq& x2, const Fq& y2, bool q_id) {
+        const bool p_id = qxyzz_is_id(acc);
+        const Fq U2 = fq_mul(x2, acc.ZZ);
+        const Fq S2 = fq_mul(y2, acc.ZZZ);
+        const Fq P = fq_sub<4>(U2, acc.X);  // < 5.5p
+        const Fq R = fq_sub<4>(S2, acc.Y);
+        const Fq PP = fq_sqr(P);
+        const Fq PPP = fq_mul(P, PP);
+        const Fq Q = fq_mul(acc.X, PP);
+        const Fq RR = fq_sqr(R);
+        const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
+        QXyzz r;
+        r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));         // < 6.5p -> < 3.5p
+        r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(acc.Y, PPP));  // < 3.5p
+        r.ZZ = fq_mul(acc.ZZ, PP);
+        r.ZZZ = fq_mul(acc.ZZZ, PPP);
+        if (__builtin_expect(dbl, 0)) r = qxyzz_dbl_aff(x2, y2);
+        // P == -Q falls out of the formulas: P = 0 gives ZZ3 = 0 mod p, the identity.
+        QXyzz q;
+        q.X = x2;
+        q.Y = y2;
+        q.ZZ = fq_one();
+        q.ZZZ = q.ZZ;
+        r = qxyzz_select(p_id, q, r);
+        acc = qxyzz_select(q_id, acc, r);
+    }
+
+    // Placeholder for an accumulator that starts a bucket run: the value is never
+    // used (the run's first addition selects the point itself), but ZZ = ZZZ = 1
+    // and X = 0 keep P = x2 - X away from 0, so the first addition does not look
+    // exceptional.
+    static __device__ __forceinline__ QXyzz qxyzz_run_start() {
+        QXyzz r;
+        r.X = fq_zero();
+        r.Y = fq_zero();
+        r.ZZ = fq_one();
+        r.ZZZ = r.ZZ;
+        return r;
+    }
+
+    // acc += (x2, y2) in the bucket-accumulation loop.  acc_id says acc is the
+    // identity (a run start, or a cancellation earlier in the run); (x2, y2) is
+    // never the identity there.  X3 is left below 6.5p (no final subtraction):
+    // every later use of X takes it as a multiplicand or as the subtrahend of a
+    // K = 7 form.  Doubling (acc == q) and cancellation (acc == -q) both have
+    // PP = 0 mod p; PP < 1.6p is then 0 or p, so its low limb is 0 or p_0.  That one-limb test screens every step; the exact tests run only in
+    // lanes that pass it, and a wave branches to them only if one of its lanes
+    // does.  After a cancellation acc holds ZZ = 0 mod p, the stored form of the
+    // identity.
+    static __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
+                                                   const Fq& y2) {
+        const Fq P = fq_mul_sub<7>(x2, acc.ZZ, acc.X);   // U2 - X1 + 7p < 8.5p
+        const Fq R = fq_mul_sub<5>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 5p < 6.5p
+        const Fq PP = fq_sqr(P);
+        const Fq PPP = fq_mul(P, PP);
+        const Fq Q = fq_mul(acc.X, PP);
+        QXyzz r;
+        Fq T;  // 2Q + PPP < 4.5p, limbs sums of three normalised limbs
+#pragma unroll
+        for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
+        r.X = fq_sqr_sub<5, 3>(R, T);                                        // < 6.4p
+        r.Y = fq_mul_sub<2>(R, fq_sub_lazy<7>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
+        r.ZZ = fq_mul(acc.ZZ, PP);
+        r.ZZZ = fq_mul(acc.ZZZ, PPP);
+        bool cancel = false;
+        if (__builtin_expect(PP.v[0] == 0u || PP.v[0] == F::p(0), 0)) {
+            const bool z = !acc_id && fq_is_zero2(PP);
+            const bool d = z && fq_is_zero2(fq_sqr(R));  // R^2 < 1.5p
+            cancel = z && !d;
+            if (d) r = qxyzz_dbl_aff(x2, y2);
+        }
+        QXyzz q;
+        q.X = x2;
+        q.Y = y2;
+        q.ZZ = fq_one();
+        q.ZZZ = q.ZZ;
+        acc = qxyzz_select(acc_id, q, r);
+        acc_id = cancel;
+    }
+
+    // acc += q (both XYZZ)
+    static __device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
+        const bool p_id = qxyzz_is_id(acc);
+        const bool q_id = qxyzz_is_id(q);
+        const Fq U1 = fq_mul(acc.X, q.ZZ);
+        const Fq U2 = fq_mul(q.X, acc.ZZ);
+        const Fq S1 = fq_mul(acc.Y, q.ZZZ);
+        const Fq S2 = fq_mul(q.Y, acc.ZZZ);
+        const Fq P = fq_sub<2>(U2, U1);  // < 3.5p
+        const Fq R = fq_sub<2>(S2, S1);
+        const Fq PP = fq_sqr(P);
+        const Fq PPP = fq_mul(P, PP);
+        const Fq Q = fq_mul(U1, PP);
+        const Fq RR = fq_sqr(R);
+        const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
+        QXyzz r;
+        r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));
+        r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(S1, PPP));
+        r.ZZ = fq_mul(fq_mul(acc.ZZ, q.ZZ), PP);
+        r.ZZZ = fq_mul(fq_mul(acc.ZZZ, q.ZZZ), PPP);
+        if (__builtin_expect(dbl, 0)) r = qxyzz_dbl(acc);
+        r = qxyzz_select(p_id, q, r);
+        acc = qxyzz_select(q_id, acc, r);
+    }
+
+    // ---- conversions between the two forms (canonical 32-bit form: [0, p),
+    // R = 2^256; internal: R = 2^261)
+    static __device__ __forceinline__ Fq fq_from_fp_canon(const Fp& a) { return fq_csub<1>(fq_from_fp(a)); }
+
+    static __device__ __forceinline__ Xyzz qxyzz_to_xyzz(const QXyzz& a) {
+        Xyzz r;
+        r.X = fq_to_fp(a.X);
+        r.Y = fq_to_fp(a.Y);
+        r.ZZ = fq_to_fp(a.ZZ);
+        r.ZZZ = fq_to_fp(a.ZZZ);
+        return r;
+    }
+
+    static __device__ __forceinline__ QXyzz qxyzz_from_xyzz(const Xyzz& a) {
+        QXyzz r;
+        r.X = fq_from_fp_canon(canon32<F>(a.X));
+        r.Y = fq_from_fp_canon(canon32<F>(a.Y));
+        r.ZZ = fq_from_fp_canon(canon32<F>(a.ZZ));
+        r.ZZZ = fq_from_fp_canon(canon32<F>(a.ZZZ));
+        return r;
+    }
+
+    static __device__ __forceinline__ void store_qtab(uint4* base, uint64_t idx, const QAff& a) {
+        uint32_t w[32];
+        const Fq ny = fq_csub<1>(fq_sub<1>(fq_zero(), a.y));  // p - y; 0 stays 0
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            w[i] = a.x.v[i];
+            w[8 + i] = a.y.v[i];
+            w[16 + i] = ny.v[i];
+        }
+        w[24] = a.x.v[8]; w[25] = a.y.v[8];
+        w[26] = a.x.v[8]; w[27] = ny.v[8];
+#pragma unroll
+        for (int i = 28; i < 32; i++) w[i] = 0;
+        uint4* p = base + idx * 8;
+#pragma unroll
+        for (int i = 0; i < 8; i++) p[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    }
+};
 
 }  // namespace zk

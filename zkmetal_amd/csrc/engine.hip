@@ -95,8 +95,10 @@ uint32_t default_window_bits(uint64_t n) {
 }
 
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
-              Plan& pl, uint32_t form, uint64_t lane_cap) {
+              Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv) {
     pl = Plan();
+    pl.curve = cv;
+    const uint32_t eps = entries_per_point(cv);
     pl.msms.resize(nmsm);
     pl.msm_scalar0.resize(nmsm + 1);
     pl.msm_dtile0.resize(nmsm + 1);
@@ -109,18 +111,18 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         m.table0 = table0[j];
         m.n = ns[j];
         m.c = c;
-        m.nwin = nwin_for(c);
+        m.nwin = (scalar_window_span(cv) + c - 1) / c;
         m.seg0 = (uint32_t)pl.segs.size();
         m.form = form;
         pl.msm_scalar0[j] = pl.scalars;
         pl.msm_dtile0[j] = pl.dtiles;
-        pl.dtiles += (2 * ns[j] + kSortTile - 1) / kSortTile;
+        pl.dtiles += (eps * ns[j] + kSortTile - 1) / kSortTile;
         pl.scalars += ns[j];
         for (uint32_t w = 0; w < m.nwin; w++) {
             SegDesc s{};
             s.ent0 = pl.entries;
             s.bkt0 = pl.buckets;
-            s.len = 2 * ns[j];
+            s.len = eps * ns[j];
             s.nbk = 1u << (c - 1);
             s.tile0 = pl.tiles;
             s.ntiles = (s.len + kSortTile - 1) / kSortTile;
@@ -245,6 +247,66 @@ static DescLayout layout_for(const Plan& pl) {
 
 static constexpr uint32_t kRunLimit = 8;
 
+// Host tail of a call: per MSM, result = sum_w 2^(c w) (U_w + sum_{t < c-1}
+// 2^t Q_{w,t}) by one Horner over the bit positions b = c w + t (Q_{w,t} =
+// Pl_t for t < g, else Ph_{t-g}; U_w at t = 0): ~c W doublings and W c
+// additions per MSM; then the affine normalisation of all results with one
+// inversion.  hw: the window planes as copied back; H: the curve's field.
+template <class H>
+static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out96) {
+    const uint32_t nmsm = (uint32_t)pl.msms.size();
+    std::vector<zkh::Xyzz> res(nmsm);
+    auto tail = [&](uint32_t jb, uint32_t je) {
+        for (uint32_t j = jb; j < je; j++) {
+            const MsmDesc& m = pl.msms[j];
+            const uint32_t c = m.c, cb = c - 1, g = cb / 2;
+            zkh::Xyzz acc = H::identity();
+            for (int b = (int)(c * (m.nwin - 1) + cb - 1); b >= 0; b--) {
+                if (!H::is_id(acc)) acc = H::xdbl(acc);
+                const uint32_t w = (uint32_t)b / c, t = (uint32_t)b % c;
+                if (t >= cb) continue;  // bit c-1 of a window carries no plane
+                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * 128;
+                const uint32_t slot = t < g ? 1 + t : 1 + g + (t - g);
+                zkh::Xyzz q;
+                memcpy(&q, bs + (uint64_t)slot * 128, 128);
+                acc = H::xadd(acc, q);
+                if (t == 0) {
+                    memcpy(&q, bs, 128);  // U_w
+                    acc = H::xadd(acc, q);
+                }
+            }
+            res[j] = acc;
+        }
+    };
+    uint32_t nt = nmsm >= 16 ? std::min<uint32_t>(8, nmsm / 8) : 1;
+    if (nt > 1) {
+        std::vector<std::thread> th;
+        uint32_t chunk = (nmsm + nt - 1) / nt;
+        for (uint32_t t = 1; t < nt; t++)
+            th.emplace_back(tail, std::min(nmsm, t * chunk), std::min(nmsm, (t + 1) * chunk));
+        tail(0, std::min(nmsm, chunk));
+        for (auto& x : th) x.join();
+    } else {
+        tail(0, nmsm);
+    }
+    std::vector<zkh::Fp> xs, ys;
+    H::batch_to_affine(res, xs, ys);
+    for (uint32_t j = 0; j < nmsm; j++) {
+        uint8_t* o = out96 + (uint64_t)j * 96;
+        if (H::is_id(res[j])) {
+            zkh::Fp one = H::one(), zero = zkh::zero();
+            memcpy(o, one.v, 32);
+            memcpy(o + 32, one.v, 32);
+            memcpy(o + 64, zero.v, 32);
+        } else {
+            zkh::Fp one = H::one();
+            memcpy(o, xs[j].v, 32);
+            memcpy(o + 32, ys[j].v, 32);
+            memcpy(o + 64, one.v, 32);
+        }
+    }
+}
+
 int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
                     uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
@@ -305,7 +367,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         launch_fill_zero(err_.p, 16, st);
         ZK_TRY(mark(0));
         // the decomposition also counts the first sort pass's tile histograms
-        launch_decompose(d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
+        launch_decompose(pl.curve, d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
                          digits_.p, (uint32_t*)hist_.p, d_err, st);
         ZK_TRY(mark(1));
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
@@ -316,7 +378,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                          (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
         launch_fill_zero(buckets_.p, pl.buckets * kBucketBytes, st);
         ZK_TRY(mark(2));
-        launch_accumulate(d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
+        launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
                           (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
                           buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
         ZK_TRY(mark(3));
@@ -324,10 +386,10 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         // lane, longer runs (heavy buckets) by lane groups.  A listed run has more
         // than kRunLimit records, so each list holds at most nrec / (kRunLimit+1).
         const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
-        launch_merge_runs((const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
+        launch_merge_runs(pl.curve, (const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
                           d_err + 1, (uint32_t*)recBk_.p, list_cap, st);
         ZK_TRY(mark(4));
-        launch_bucket_reduce(d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, buckets_.p, rc_.p,
+        launch_bucket_reduce(pl.curve, d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, buckets_.p, rc_.p,
                              winsum_.p, st);
         ZK_TRY(mark(5));
         ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
@@ -398,59 +460,8 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
 
     // host tail: Horner over windows, then affine normalisation
     auto t_tail0 = std::chrono::steady_clock::now();
-    std::vector<zkh::Xyzz> res(nmsm);
-    // result = sum_w 2^(c w) (U_w + sum_{t < c-1} 2^t Q_{w,t}): one Horner over
-    // the bit positions b = c w + t (Q_{w,t} = Pl_t for t < g, else Ph_{t-g};
-    // U_w at t = 0), ~127 doublings and W c additions per MSM
-    auto tail = [&](uint32_t jb, uint32_t je) {
-        for (uint32_t j = jb; j < je; j++) {
-            const MsmDesc& m = pl.msms[j];
-            const uint32_t c = m.c, cb = c - 1, g = cb / 2;
-            zkh::Xyzz acc = zkh::identity();
-            for (int b = (int)(c * (m.nwin - 1) + cb - 1); b >= 0; b--) {
-                if (!zkh::is_id(acc)) acc = zkh::xdbl(acc);
-                const uint32_t w = (uint32_t)b / c, t = (uint32_t)b % c;
-                if (t >= cb) continue;  // bit c-1 of a window carries no plane
-                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * 128;
-                const uint32_t slot = t < g ? 1 + t : 1 + g + (t - g);
-                zkh::Xyzz q;
-                memcpy(&q, bs + (uint64_t)slot * 128, 128);
-                acc = zkh::xadd(acc, q);
-                if (t == 0) {
-                    memcpy(&q, bs, 128);  // U_w
-                    acc = zkh::xadd(acc, q);
-                }
-            }
-            res[j] = acc;
-        }
-    };
-    uint32_t nt = nmsm >= 16 ? std::min<uint32_t>(8, nmsm / 8) : 1;
-    if (nt > 1) {
-        std::vector<std::thread> th;
-        uint32_t chunk = (nmsm + nt - 1) / nt;
-        for (uint32_t t = 1; t < nt; t++)
-            th.emplace_back(tail, std::min(nmsm, t * chunk), std::min(nmsm, (t + 1) * chunk));
-        tail(0, std::min(nmsm, chunk));
-        for (auto& x : th) x.join();
-    } else {
-        tail(0, nmsm);
-    }
-    std::vector<zkh::Fp> xs, ys;
-    zkh::batch_to_affine(res, xs, ys);
-    for (uint32_t j = 0; j < nmsm; j++) {
-        uint8_t* o = out96 + (uint64_t)j * 96;
-        if (zkh::is_id(res[j])) {
-            zkh::Fp one = zkh::one(), zero = zkh::zero();
-            memcpy(o, one.v, 32);
-            memcpy(o + 32, one.v, 32);
-            memcpy(o + 64, zero.v, 32);
-        } else {
-            zkh::Fp one = zkh::one();
-            memcpy(o, xs[j].v, 32);
-            memcpy(o + 32, ys[j].v, 32);
-            memcpy(o + 64, one.v, 32);
-        }
-    }
+    if (pl.curve == CurveId::Grumpkin) host_tail<zkh::HR>(pl, hw, out96);
+    else host_tail<zkh::HP>(pl, hw, out96);
     auto t_end = std::chrono::steady_clock::now();
     timing_.host_tail_ms = std::chrono::duration<float, std::milli>(t_end - t_tail0).count();
     timing_.wall_ms = std::chrono::duration<float, std::milli>(t_end - t_wall0).count();
@@ -529,9 +540,9 @@ int32_t Engine::srs_set_first() {
 }
 
 int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
-                               const uint32_t* counts, uint32_t nmsm, uint8_t* out96) {
+                               const uint32_t* counts, uint32_t nmsm, uint8_t* out96, CurveId cv) {
     if (!points || !scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
-    if (points_are_srs_prefixes(points, counts, nmsm)) {
+    if (cv == CurveId::Bn254 && points_are_srs_prefixes(points, counts, nmsm)) {
         srs_hits_++;
         return msm_resident(scalars, false, counts, nmsm, out96);
     }
@@ -545,21 +556,22 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
             continue;
         }
         ns.push_back(counts[j]);
-        t0.push_back(2 * total);
+        t0.push_back(entries_per_point(cv) * total);
         idx.push_back(j);
         total += counts[j];
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0,
+                       acc_cap_, cv);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * 64);
-    ZK_ALLOC(ptable_, table_bytes(total));
+    ZK_ALLOC(ptable_, table_bytes(total, cv));
     ZK_ALLOC(scalars_, total * 32);
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * 64, hipMemcpyHostToDevice, stream_));
     ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
-    launch_build_table(pts_in_.p, ptable_.p, total, stream_);
+    launch_build_table(cv, pts_in_.p, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
     rc = run(pl, scalars_.p, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
@@ -578,7 +590,7 @@ int32_t Engine::multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t*
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
     ZK_ALLOC(ptable_, table_bytes(n));
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
-    launch_build_table(pts_in_.p, ptable_.p, n, stream_);
+    launch_build_table(CurveId::Bn254, pts_in_.p, ptable_.p, n, stream_);
     std::vector<uint32_t> counts(k, n);
     return msm_table(ptable_.p, n, scalars, false, counts.data(), k, out96);
 }
@@ -605,7 +617,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
     if (rc != kOk) return rc;
     ZK_ALLOC(ptable_, table_bytes(total));
-    launch_build_table(d_points, ptable_.p, total, stream_);
+    launch_build_table(CurveId::Bn254, d_points, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
     rc = run(pl, d_scalars, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
@@ -618,7 +630,7 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
     ZK_ALLOC(srs_, table_bytes(n));
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
-    launch_build_table(pts_in_.p, srs_.p, n, stream_);
+    launch_build_table(CurveId::Bn254, pts_in_.p, srs_.p, n, stream_);
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipGetLastError());
     srs_n_ = n;
@@ -687,20 +699,21 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
 int32_t Engine::ensure_gtab() {
     if (gtab_.p) return kOk;
     std::vector<zkh::Xyzz> pts(32 * 256);
-    zkh::Fp gx = zkh::mul(zkh::Fp{{1, 0, 0, 0}}, zkh::r2());
-    zkh::Fp gy = zkh::mul(zkh::Fp{{2, 0, 0, 0}}, zkh::r2());
-    zkh::Xyzz base = zkh::from_affine(gx, gy);
+    using H = zkh::HP;
+    zkh::Fp gx = H::mul(zkh::Fp{{1, 0, 0, 0}}, H::r2());
+    zkh::Fp gy = H::mul(zkh::Fp{{2, 0, 0, 0}}, H::r2());
+    zkh::Xyzz base = H::from_affine(gx, gy);
     for (int w = 0; w < 32; w++) {
-        zkh::Xyzz acc = zkh::identity();
+        zkh::Xyzz acc = H::identity();
         pts[w * 256] = acc;
         for (int b = 1; b < 256; b++) {
-            acc = zkh::xadd(acc, base);
+            acc = H::xadd(acc, base);
             pts[w * 256 + b] = acc;
         }
-        base = zkh::xadd(acc, base);  // 256 * base
+        base = H::xadd(acc, base);  // 256 * base
     }
     std::vector<zkh::Fp> xs, ys;
-    zkh::batch_to_affine(pts, xs, ys);
+    H::batch_to_affine(pts, xs, ys);
     std::vector<uint8_t> blob(32 * 256 * 64);
     for (size_t i = 0; i < pts.size(); i++) {
         memcpy(&blob[i * 64], xs[i].v, 32);
@@ -735,7 +748,7 @@ int32_t Engine::srs_download(uint8_t* points, uint32_t n) {
     if (n == 0) return kOk;
     // table entries 2i are the bases, in the kernels' internal form
     ZK_ALLOC(tmp_, (uint64_t)n * 64);
-    launch_table_export(srs_.p, n, tmp_.p, stream_);
+    launch_table_export(CurveId::Bn254, srs_.p, n, tmp_.p, stream_);
     ZK_TRY(hipMemcpyAsync(points, tmp_.p, (uint64_t)n * 64, hipMemcpyDeviceToHost, stream_));
     ZK_TRY(hipStreamSynchronize(stream_));
     return kOk;

@@ -76,6 +76,7 @@ struct Plan {
     std::vector<uint32_t> chunk_seg;
     uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0, rc_waves = 0;
     uint32_t tiles = 0, chunks = 0, T = 0, dtiles = 0;
+    CurveId curve = CurveId::Bn254;
     uint32_t G = 8;  // buckets per bucket-reduction lane (a power of 2, 8..32)
 };
 
@@ -85,7 +86,7 @@ uint32_t default_window_bits(uint64_t n);
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
               const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0,
-              uint64_t lane_cap = 1u << 17);
+              uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254);
 
 class Engine {
   public:
@@ -94,8 +95,11 @@ class Engine {
 
     // host buffers in, host results out (96 B Jacobian X|Y|Z per MSM)
     int32_t msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t n, uint8_t* out96);
+    // cv = CurveId::Grumpkin: points are Grumpkin affine points (x, y in Fr,
+    // Montgomery form), scalars are reduced mod p, results are Grumpkin points
     int32_t msm_batch_host(const uint8_t* points, const uint8_t* scalars,
-                           const uint32_t* counts, uint32_t nmsm, uint8_t* out96);
+                           const uint32_t* counts, uint32_t nmsm, uint8_t* out96,
+                           CurveId cv = CurveId::Bn254);
 
     // k MSMs over one base set of n points (scalar set j at scalars + j*n*32):
     // the table is built once and the k MSMs run as one batched launch

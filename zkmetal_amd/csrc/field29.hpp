@@ -1,5 +1,5 @@
-// BN254 base field Fp on gfx950 in nine 29-bit limbs (radix 2^29), Montgomery
-// form with R = 2^261.
+// BN254's two prime fields on gfx950 in nine 29-bit limbs (radix 2^29),
+// Montgomery form with R = 2^261; the arithmetic is Arith<F> (curve29.hpp).
 //
 // Why 29 bits: a column of the FIPS Montgomery product sums at most 9 limb
 // products of a*b and 9 of m*p, each < 2^58, so the column stays far below
@@ -25,20 +25,117 @@ struct Fq {
 };
 
 #define ZQ_MASK 0x1fffffffu
-#define ZQ_PINV 0x04866389u  // -p^-1 mod 2^29
 
-__device__ __forceinline__ uint32_t q_p(int i) {
-    switch (i) {
-        case 0: return 0x187cfd47u;
-        case 1: return 0x010460b6u;
-        case 2: return 0x1c72a34fu;
-        case 3: return 0x02d522d0u;
-        case 4: return 0x1585d978u;
-        case 5: return 0x02db40c0u;
-        case 6: return 0x00a6e141u;
-        case 7: return 0x0e5c2634u;
-        default: return 0x0030644eu;
+// The two fields of the BN254 cycle, as constants for the 29-bit arithmetic
+// (curve29.hpp, Arith<F>): p(i) the nine 29-bit limbs of the modulus m,
+// kPinv = -m^-1 mod 2^29, p32(i) its eight 32-bit limbs, one() = 2^261 mod m,
+// from32() = 2^266 mod m (a Montgomery product with it maps the 32-bit
+// Montgomery form x 2^256 to x 2^261), to32() = 2^256 mod m (maps back).
+// Both moduli are below 2^254, so the lazy bounds hold for either
+// (2^261 > 2^7 m).
+
+// BN254 base field: coordinates of BN254 G1
+struct Bn254Fp {
+    __device__ static __forceinline__ uint32_t p(int i) {
+        switch (i) {
+            case 0: return 0x187cfd47u;
+            case 1: return 0x010460b6u;
+            case 2: return 0x1c72a34fu;
+            case 3: return 0x02d522d0u;
+            case 4: return 0x1585d978u;
+            case 5: return 0x02db40c0u;
+            case 6: return 0x00a6e141u;
+            case 7: return 0x0e5c2634u;
+            default: return 0x0030644eu;
+        }
     }
+    static constexpr uint32_t kPinv = 0x04866389u;
+    __device__ static __forceinline__ uint32_t p32(int i) { return p_limb(i); }
+    __device__ static __forceinline__ Fq one() {
+        return Fq{{0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu,
+                   0x11d54c07u, 0x052ac7a8u, 0x000dc836u}};
+    }
+    __device__ static __forceinline__ Fq from32() {
+        return Fq{{0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u,
+                   0x14cb95b3u, 0x041b9d3du, 0x00058003u}};
+    }
+    __device__ static __forceinline__ Fq to32() {
+        return Fq{{0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u, 0x11b7bc3cu,
+                   0x1cbd99bau, 0x183340fbu, 0x000e0a77u}};
+    }
+    // beta * 2^261 mod p, beta a cube root of unity: phi(x, y) = (beta x, y)
+    __device__ static __forceinline__ Fq beta() {
+        return Fq{{0x0a337995u, 0x158d1d23u, 0x189c9b98u, 0x12fa4e45u, 0x185faadcu, 0x0176f16du,
+                   0x0eed93bau, 0x14291140u, 0x000c0afeu}};
+    }
+};
+
+// BN254 scalar field r: coordinates of Grumpkin (y^2 = x^3 - 17 over Fr)
+struct Bn254Fr {
+    __device__ static __forceinline__ uint32_t p(int i) {
+        switch (i) {
+            case 0: return 0x10000001u;
+            case 1: return 0x1f0fac9fu;
+            case 2: return 0x0e5c2450u;
+            case 3: return 0x07d090f3u;
+            case 4: return 0x1585d283u;
+            case 5: return 0x02db40c0u;
+            case 6: return 0x00a6e141u;
+            case 7: return 0x0e5c2634u;
+            default: return 0x0030644eu;
+        }
+    }
+    static constexpr uint32_t kPinv = 0x0fffffffu;
+    __device__ static __forceinline__ uint32_t p32(int i) {
+        switch (i) {
+            case 0: return 0xf0000001u;
+            case 1: return 0x43e1f593u;
+            case 2: return 0x79b97091u;
+            case 3: return 0x2833e848u;
+            case 4: return 0x8181585du;
+            case 5: return 0xb85045b6u;
+            case 6: return 0xe131a029u;
+            default: return 0x30644e72u;
+        }
+    }
+    __device__ static __forceinline__ Fq one() {
+        return Fq{{0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu,
+                   0x11d54c07u, 0x052ac7a8u, 0x000dc836u}};
+    }
+    __device__ static __forceinline__ Fq from32() {
+        return Fq{{0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u,
+                   0x14cb95b3u, 0x041b9d3du, 0x00058003u}};
+    }
+    __device__ static __forceinline__ Fq to32() {
+        return Fq{{0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu,
+                   0x1cbd99bau, 0x183340fbu, 0x000e0a77u}};
+    }
+};
+
+// t < 2m (32-bit limbs) -> t mod m
+template <class F>
+__device__ __forceinline__ Fp reduce_once32(const Fp& t) {
+    Fp d;
+    unsigned br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(t.v[i], F::p32(i), br, &br);
+    return br ? t : d;
+}
+
+// the canonical representative of a 256-bit value (at most 6 subtractions
+// of m: 2^256 < 6 m)
+template <class F>
+__device__ __forceinline__ Fp canon32(Fp a) {
+#pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+        Fp d;
+        unsigned br = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(a.v[i], F::p32(i), br, &br);
+        if (br) break;
+        a = d;
+    }
+    return a;
 }
 
 // column accumulation: acc += x * y (one v_mad_u64_u32)
@@ -53,130 +150,6 @@ __device__ __forceinline__ void qmac(uint64_t& acc, uint32_t x, uint32_t y) {
 // Measured in the bucket accumulation: 7% fewer VALU instructions per mixed
 // addition, 3% less time.
 __device__ __forceinline__ void carry_fence(uint64_t& acc) { asm("" : "+v"(acc)); }
-
-// limb i of K*p, normalised (folded at compile time inside unrolled loops)
-template <int K>
-__device__ __forceinline__ uint32_t q_kp(int i) {
-    uint64_t c = 0;
-    uint32_t r = 0;
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-        const uint64_t s = (uint64_t)K * q_p(j) + c;
-        if (j == i) r = (uint32_t)s & ZQ_MASK;
-        c = s >> 29;
-    }
-    return r;
-}
-
-// K*p with B * 2^29 borrowed into every limb below the top one: limb 0 is
-// kp_0 + B 2^29, limbs 1..7 kp_i + B 2^29 - B, limb 8 kp_8 - B (same value).
-// For a c whose limbs are sums of at most B normalised limbs (c_i <=
-// B (2^29 - 1)) and c_8 <= kp_8 - B, every limb of (this - c) is >= 0, so
-// K*p - c needs no borrow propagation.
-template <int K, int B = 1>
-__device__ __forceinline__ uint32_t q_kpr(int i) {
-    const uint32_t c = q_kp<K>(i);
-    return i == 0 ? c + (uint32_t)B * (1u << 29)
-                  : (i < 8 ? c + (uint32_t)B * (1u << 29) - (uint32_t)B : c - (uint32_t)B);
-}
-
-// Q - c + K*p limb by limb with no carry propagation (limbs < 2^29 + B 2^29,
-// value < Q + K*p): a multiplicand for fq_mul_sub / fq_mul whose other operand
-// has normalised limbs.  Q normalised; c as for q_kpr<K, B>.
-template <int K, int B = 1>
-__device__ __forceinline__ Fq fq_sub_lazy(const Fq& q, const Fq& c) {
-    Fq r;
-#pragma unroll
-    for (int i = 0; i < 9; i++) r.v[i] = q.v[i] + q_kpr<K, B>(i) - c.v[i];
-    return r;
-}
-
-// a * b / 2^261 - c + K*p mod p (K > 0), or a * b / 2^261 (K == 0):
-// result < ab/2^261 + p + K*p with normalised limbs.  The limbs of K*p - c
-// (q_kpr<K, B>) enter the high columns of the product, where they cost one
-// subtraction each instead of a separate normalised subtraction.
-// a, b: limbs < 2^30, or a normalised and b limbs < 1.5 * 2^30 (column sums
-// then stay below 2^63.3); values with ab < p * 2^261 / 2 (e.g. both < 8p);
-// c (K > 0): as for q_kpr<K, B>.
-template <int K, int B = 1>
-__device__ __forceinline__ Fq fq_mul_sub(const Fq& a, const Fq& b, const Fq& c) {
-    uint32_t m[9];
-    Fq t;
-    uint64_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-#pragma unroll
-        for (int j = 0; j < k; j++) {
-            qmac(acc, a.v[j], b.v[k - j]);
-            qmac(acc, m[j], q_p(k - j));
-        }
-        qmac(acc, a.v[k], b.v[0]);
-        m[k] = ((uint32_t)acc * ZQ_PINV) & ZQ_MASK;
-        qmac(acc, m[k], q_p(0));
-        acc >>= 29;
-        carry_fence(acc);
-    }
-#pragma unroll
-    for (int k = 9; k < 17; k++) {
-        if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
-#pragma unroll
-        for (int j = k - 8; j < 9; j++) {
-            qmac(acc, a.v[j], b.v[k - j]);
-            qmac(acc, m[j], q_p(k - j));
-        }
-        t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
-        acc >>= 29;
-        carry_fence(acc);
-    }
-    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
-    return t;
-}
-
-// a * b / 2^261 mod p, result < ab/2^261 + p (normalised limbs).
-// a, b: limbs < 2^30, values with ab < p * 2^261 / 2 (e.g. both < 8p).
-__device__ __forceinline__ Fq fq_mul(const Fq& a, const Fq& b) { return fq_mul_sub<0>(a, b, a); }
-
-// a^2 / 2^261 mod p: the symmetric products a_i a_j (i != j) once, from the
-// doubled limb 2 a_i (< 2^30), plus the squares: 45 + 81 multiply-adds.
-// Column bound: 4 doubled products < 2^59, one square < 2^58, 9 m*p products
-// < 2^58 and the carry: < 18 * 2^58 < 2^63.  a: normalised limbs, value < 8p.
-// fq_sqr_sub<K, B>(a, c) = a^2 / 2^261 - c + K*p, as fq_mul_sub.
-template <int K, int B = 1>
-__device__ __forceinline__ Fq fq_sqr_sub(const Fq& a, const Fq& c) {
-    uint32_t a2[9], m[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) a2[i] = a.v[i] << 1;
-    Fq t;
-    uint64_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-#pragma unroll
-        for (int i = 0; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
-        if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
-#pragma unroll
-        for (int j = 0; j < k; j++) qmac(acc, m[j], q_p(k - j));
-        m[k] = ((uint32_t)acc * ZQ_PINV) & ZQ_MASK;
-        qmac(acc, m[k], q_p(0));
-        acc >>= 29;
-        carry_fence(acc);
-    }
-#pragma unroll
-    for (int k = 9; k < 17; k++) {
-        if (K > 0) acc += q_kpr<K, B>(k - 9) - c.v[k - 9];
-#pragma unroll
-        for (int i = k - 8; 2 * i < k; i++) qmac(acc, a2[i], a.v[k - i]);
-        if ((k & 1) == 0) qmac(acc, a.v[k / 2], a.v[k / 2]);
-#pragma unroll
-        for (int j = k - 8; j < 9; j++) qmac(acc, m[j], q_p(k - j));
-        t.v[k - 9] = (uint32_t)acc & ZQ_MASK;
-        acc >>= 29;
-        carry_fence(acc);
-    }
-    t.v[8] = (uint32_t)acc + (K > 0 ? q_kpr<K, B>(8) - c.v[8] : 0u);
-    return t;
-}
-
-__device__ __forceinline__ Fq fq_sqr(const Fq& a) { return fq_sqr_sub<0>(a, a); }
 
 // 256-bit integer (8 x 32) <-> 9 x 29 bit slices of the same integer
 __device__ __forceinline__ Fq fq_slice(const Fp& a) {
@@ -202,24 +175,6 @@ __device__ __forceinline__ Fp fq_pack(const Fq& a) {
         r.v[w] = x;
     }
     return r;
-}
-
-// x * 2^256 mod p (the 32-bit-limb Montgomery form) -> x * 2^261 mod p
-__device__ __forceinline__ Fq fq_from_fp(const Fp& a) {
-    Fq k;  // 2^5 * 2^261 mod p
-    k.v[0] = 0x13349ca1u; k.v[1] = 0x1a5d84a8u; k.v[2] = 0x0a3e5cacu; k.v[3] = 0x100249e0u;
-    k.v[4] = 0x12b951e8u; k.v[5] = 0x0e92d304u; k.v[6] = 0x14cb95b3u; k.v[7] = 0x041b9d3du;
-    k.v[8] = 0x00058003u;
-    return fq_mul(fq_slice(a), k);
-}
-
-// value < 8p with normalised limbs -> [0, p) in the 32-bit form
-__device__ __forceinline__ Fp fq_to_fp(const Fq& a) {
-    Fq k;  // 2^256 mod p
-    k.v[0] = 0x058f0d9du; k.v[1] = 0x1aea1c6eu; k.v[2] = 0x11c2cf74u; k.v[3] = 0x11d651ebu;
-    k.v[4] = 0x1462c0a7u; k.v[5] = 0x11b7bc3cu; k.v[6] = 0x1cbd99bau; k.v[7] = 0x183340fbu;
-    k.v[8] = 0x000e0a77u;
-    return fp_reduce_once(fq_pack(fq_mul(a, k)));  // product < 1.5p
 }
 
 }  // namespace zk

@@ -28,6 +28,11 @@
 
 namespace zk {
 
+// BN254 G1 over Fp: the GLV curve of the MSM, the SRS import / generation and
+// the primitive tests; Grumpkin over Fr: the sibling curve (no GLV)
+using Ap = Arith<Bn254Fp>;
+using Ar = Arith<Bn254Fr>;
+
 // ------------------------------------------------------------------ helpers
 
 // largest j in [0, n) with pre[j] <= x (pre ascending, pre[0] == 0)
@@ -107,7 +112,7 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-template <class Load>
+template <class F, class Load>
 __device__ __forceinline__ QXyzz sum_then_tree(uint32_t nload, uint32_t P, Load load) {
     const uint32_t lane = __lane_id();
     uint32_t levels = 0;
@@ -123,17 +128,18 @@ __device__ __forceinline__ QXyzz sum_then_tree(uint32_t nload, uint32_t P, Load 
             q = shfl_down_qxyzz(acc, s);
             if ((lane & (P - 1)) >= s) q = qxyzz_identity();
         }
-        qxyzz_add(acc, q);
+        Arith<F>::qxyzz_add(acc, q);
     }
     return acc;
 }
 
+template <class F>
 __device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
     const uint32_t lane = __lane_id();
 #pragma unroll 1
     for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
         QXyzz o = shfl_down_qxyzz(v, s);
-        if ((lane & (P - 1)) < s) qxyzz_add(v, o);
+        if ((lane & (P - 1)) < s) Arith<F>::qxyzz_add(v, o);
     }
     return v;
 }
@@ -157,12 +163,12 @@ __device__ __forceinline__ void store_id_bit(uint64_t* idmask, uint64_t i, bool 
 __device__ __forceinline__ void store_table_pair(uint4* table, uint64_t* idmask, uint64_t i,
                                                  const Aff& a) {
     QAff q;
-    q.x = fq_from_fp_canon(a.x);
-    q.y = fq_from_fp_canon(a.y);
+    q.x = Ap::fq_from_fp_canon(a.x);
+    q.y = Ap::fq_from_fp_canon(a.y);
     store_id_bit(idmask, i, qaff_is_id(q));
-    store_qtab(table, 2 * i, q);
-    q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
-    store_qtab(table, 2 * i + 1, q);
+    Ap::store_qtab(table, 2 * i, q);
+    q.x = Ap::fq_csub<1>(Ap::fq_mul(Ap::fq_beta(), q.x));
+    Ap::store_qtab(table, 2 * i + 1, q);
 }
 
 // SRS import from an SRS file's G1 section (Sources/zkMetal/KZG/
@@ -206,44 +212,49 @@ __global__ __launch_bounds__(256) void k_import_be(const uint8_t* __restrict__ r
     k2.v[4] = 0x02632639u; k2.v[5] = 0x02c0169fu; k2.v[6] = 0x169bfd53u; k2.v[7] = 0x11869d4cu;
     k2.v[8] = 0x002a11a6u;
     QAff q;
-    q.x = fq_csub<1>(fq_mul(fq_slice(x), k2));
-    q.y = fq_csub<1>(fq_mul(fq_slice(y), k2));
+    q.x = Ap::fq_csub<1>(Ap::fq_mul(fq_slice(x), k2));
+    q.y = Ap::fq_csub<1>(Ap::fq_mul(fq_slice(y), k2));
     bool ok = fp_lt_p(x) && fp_lt_p(y);
     if (!qaff_is_id(q)) {
         Fq b3;  // 3 * 2^261 mod p
         b3.v[0] = 0x00766463u; b3.v[1] = 0x1c54760au; b3.v[2] = 0x08f6927au; b3.v[3] = 0x03e40c4du;
         b3.v[4] = 0x1fea4f2bu; b3.v[5] = 0x17c6c26au; b3.v[6] = 0x157fe417u; b3.v[7] = 0x0f8056f9u;
         b3.v[8] = 0x002958a2u;
-        const Fq lhs = fq_csub<1>(fq_mul(q.y, q.y));                                // < p
-        const Fq rhs = fq_csub<1>(fq_csub<2>(fq_add(fq_mul(fq_mul(q.x, q.x), q.x), b3)));  // < 2.5p -> < p
+        const Fq lhs = Ap::fq_csub<1>(Ap::fq_mul(q.y, q.y));                                // < p
+        const Fq rhs = Ap::fq_csub<1>(Ap::fq_csub<2>(Ap::fq_add(Ap::fq_mul(Ap::fq_mul(q.x, q.x), q.x), b3)));  // < 2.5p -> < p
         ok = ok && fq_eq(lhs, rhs);
     }
     if (!ok) atomicAdd(bad, 1u);
     store_id_bit(idmask, i, qaff_is_id(q));
-    store_qtab(table, 2 * i, q);
-    q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
-    store_qtab(table, 2 * i + 1, q);
+    Ap::store_qtab(table, 2 * i, q);
+    q.x = Ap::fq_csub<1>(Ap::fq_mul(Ap::fq_beta(), q.x));
+    Ap::store_qtab(table, 2 * i + 1, q);
 }
 
-// table entry 2i back to the 2^256 Montgomery form (64 B per point)
+// table entry EPS i (the base itself) back to the 2^256 Montgomery form
+// (64 B per point); EPS = table entries per base (2: P and phi(P), 1: P)
+template <class F, int EPS>
 __global__ __launch_bounds__(256) void k_table_export(const uint4* __restrict__ table, uint64_t n,
                                                       uint4* __restrict__ pts) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    QAff q = load_qtab(table, 2 * i, false);
+    QAff q = load_qtab(table, EPS * i, false);
     Aff a;
-    a.x = fq_to_fp(q.x);
-    a.y = fq_to_fp(q.y);
+    a.x = Arith<F>::fq_to_fp(q.x);
+    a.y = Arith<F>::fq_to_fp(q.y);
     store_aff(pts, i, a);
 }
 
-// The block's 256 table pairs are staged in LDS (a row per point, padded by
-// 16 B) and written out as consecutive 16-byte pieces across the block, so
-// every store instruction of a wave covers 1 KB of contiguous table.
+// The block's 256 table rows (EPS entries per point: P, and phi(P) for the
+// GLV curve) are staged in LDS (a row per point, padded by 16 B) and written
+// out as consecutive 16-byte pieces across the block, so every store
+// instruction of a wave covers 1 KB of contiguous table.
+template <class F, int EPS>
 __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ pts,
                                                      uint4* __restrict__ table,
                                                      uint64_t* __restrict__ idmask, uint64_t n) {
-    constexpr uint32_t kPair = 2 * kTableEntryBytes / 16;  // uint4 per point
+    using A = Arith<F>;
+    constexpr uint32_t kPair = EPS * kTableEntryBytes / 16;  // uint4 per point
     constexpr uint32_t kRow = kPair + 1;
     __shared__ uint4 stage[256 * kRow];
     const uint64_t i0 = (uint64_t)blockIdx.x * 256;
@@ -251,13 +262,15 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
     if (i < n) {
         Aff a = load_aff(pts, i);
         QAff q;
-        q.x = fq_from_fp_canon(fp_canon(a.x));
-        q.y = fq_from_fp_canon(fp_canon(a.y));
+        q.x = A::fq_from_fp_canon(canon32<F>(a.x));
+        q.y = A::fq_from_fp_canon(canon32<F>(a.y));
         store_id_bit(idmask, i, qaff_is_id(q));
         uint4* row = stage + threadIdx.x * kRow;
-        store_qtab(row, 0, q);
-        q.x = fq_csub<1>(fq_mul(fq_beta(), q.x));
-        store_qtab(row, 1, q);
+        A::store_qtab(row, 0, q);
+        if constexpr (EPS == 2) {
+            q.x = A::template fq_csub<1>(A::fq_mul(A::fq_beta(), q.x));
+            A::store_qtab(row, 1, q);
+        }
     }
     __syncthreads();
     const uint32_t words = (uint32_t)min<uint64_t>(256, n - i0) * kPair;
@@ -267,16 +280,48 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
 
 // --------------------------------------------------- GLV + signed digits
 
-// Block b decomposes the kSortTile / 2 scalars [tin * 4096, ...) of MSM j
-// (msm_dtile0[j] <= b < msm_dtile0[j+1], tin = b - msm_dtile0[j]): their
-// entries are first-pass sort tile tin of every window's segment, so the
-// block also counts that tile's 256-bin histogram of key byte 0 for each
-// window (LDS atomics) and writes it where the scan expects it; the first
-// pass then needs no histogram kernel.
-constexpr int kDecThreads = 256;
-constexpr int kDecPerThread = kSortTile / 2 / kDecThreads;  // scalars per thread
-static_assert(kSortTile <= (1 << 15) && kDecPerThread >= 1, "decomposition tiling");
+// Grumpkin's group order is the BN254 base-field prime p: scalars are reduced
+// mod p (GrumpkinMSMEngine.swift:178-196) and recoded over 255 bits, no GLV.
+__device__ __forceinline__ void reduce_mod_p(uint32_t k[8]) {
+#pragma unroll 1
+    for (int it = 0; it < 6; it++) {
+        uint32_t d[8];
+        unsigned br = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) d[i] = __builtin_subc(k[i], p_limb(i), br, &br);
+        if (br) break;
+#pragma unroll
+        for (int i = 0; i < 8; i++) k[i] = d[i];
+    }
+}
 
+// c-bit window w of a 256-bit value, from 64-bit words by shifts (w is the
+// same in every lane, so the word choice is a uniform branch, no indexing)
+__device__ __forceinline__ uint32_t window_bits256(const uint32_t k[8], uint32_t w, uint32_t c) {
+    const uint32_t off = w * c, q = off >> 6, sh = off & 63u;
+    const uint64_t w0 = ((uint64_t)k[1] << 32) | k[0], w1 = ((uint64_t)k[3] << 32) | k[2];
+    const uint64_t w2 = ((uint64_t)k[5] << 32) | k[4], w3 = ((uint64_t)k[7] << 32) | k[6];
+    uint64_t lo = 0, hi = 0;
+    if (q == 0) { lo = w0; hi = w1; }
+    else if (q == 1) { lo = w1; hi = w2; }
+    else if (q == 2) { lo = w2; hi = w3; }
+    else if (q == 3) { lo = w3; }
+    const uint64_t x = (lo >> sh) | (sh ? hi << (64u - sh) : 0ull);
+    return (uint32_t)x & ((1u << c) - 1u);
+}
+
+// Block b decomposes the kSortTile / EPS scalars of first-pass sort tile tin
+// of MSM j (msm_dtile0[j] <= b < msm_dtile0[j+1], tin = b - msm_dtile0[j]):
+// their entries are that tile of every window's segment, so the block also
+// counts the tile's 256-bin histogram of key byte 0 for each window (LDS
+// atomics) and writes it where the scan expects it; the first pass then
+// needs no histogram kernel.  EPS = 2: BN254 G1, GLV halves k1 (entry 2i,
+// base P_i) and k2 (entry 2i + 1, phi(P_i)); EPS = 1: Grumpkin, one entry per
+// scalar over 255 bits.
+constexpr int kDecThreads = 256;
+static_assert(kSortTile <= (1 << 15) && kSortTile / 2 / kDecThreads >= 1, "decomposition tiling");
+
+template <int EPS>
 __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __restrict__ msms,
                                                           const uint64_t* __restrict__ msm_dtile0,
                                                           uint32_t nmsm,
@@ -297,17 +342,32 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
     __syncthreads();
     const uint32_t c = m.c, half = 1u << (c - 1), full = 1u << c;
     bool carry_left = false;
-    for (int r = 0; r < kDecPerThread; r++) {
-        const uint64_t i = (uint64_t)tin * (kSortTile / 2) + (uint64_t)r * kDecThreads + threadIdx.x;
+    constexpr int kPer = kSortTile / EPS / kDecThreads;  // scalars per thread
+    for (int r = 0; r < kPer; r++) {
+        const uint64_t i = (uint64_t)tin * (kSortTile / EPS) + (uint64_t)r * kDecThreads + threadIdx.x;
         if (i >= m.n) break;
         const uint64_t g = m.scalar0 + i;
         uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
         uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
         if (m.form == 1) fr_from_mont(k);
-        const uint64_t pt = (m.table0 >> 1) + i;  // the base's point index in the table
+        const uint64_t pt = m.table0 / EPS + i;  // the base's point index in the table
         if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
 #pragma unroll
             for (int q = 0; q < 8; q++) k[q] = 0;  // identity base: no digits
+        }
+        if constexpr (EPS == 1) {
+            reduce_mod_p(k);
+            uint32_t carry = 0;
+            for (uint32_t w = 0; w < m.nwin; w++) {
+                uint32_t d = window_bits256(k, w, c) + carry, neg = 0;
+                carry = d > half;
+                if (carry) { d = full - d; neg = 1u; }
+                const uint32_t wd = d ? (key_encode(d - 1u) | (neg << 31)) : ZK_KEY_NONE;
+                reinterpret_cast<uint32_t*>(digits)[segs[m.seg0 + w].ent0 + i] = wd;
+                atomicAdd(&cnt[w * 128 + ((wd & 0xffu) >> 1)], 1u << ((wd & 1u) * 16));
+            }
+            carry_left |= carry != 0;
+            continue;
         }
         GlvSplit sp = glv_split(k);
         uint32_t carry1 = 0, carry2 = 0;
@@ -620,6 +680,7 @@ __device__ __forceinline__ void put_record(uint32_t* keys, uint4* pts, uint64_t 
     if (valid) store_qxyzz(pts, slot, v);
 }
 
+template <class F>
 __global__ __launch_bounds__(256) void k_accumulate(
     const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_lane0, uint32_t nseg,
     const MsmDesc* __restrict__ msms, const uint32_t* __restrict__ seg_valid,
@@ -643,7 +704,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
-    QXyzz acc = qxyzz_run_start();
+    QXyzz acc = Arith<F>::qxyzz_run_start();
     bool acc_id = true;
     // software pipeline: the next entry's key, value and base are loaded
     // before the current mixed addition, hiding the random 128-byte gather
@@ -663,13 +724,13 @@ __global__ __launch_bounds__(256) void k_accumulate(
                 if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, acc);
             }
             first = false;
-            acc = qxyzz_run_start();
+            acc = Arith<F>::qxyzz_run_start();
             acc_id = true;
             cur = k;
         }
         // y is already negated for a negative digit; identity bases have no
         // digits (table bitmap), so q is never the identity
-        qxyzz_madd_run(acc, acc_id, q.x, q.y);
+        Arith<F>::qxyzz_madd_run(acc, acc_id, q.x, q.y);
         k = kn;
         v = vn;
         q = qn;
@@ -698,6 +759,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
 // tree.  k_merge_long: one 256-lane workgroup per listed run, a shuffle tree
 // per wave, then one lane adds the four wave partials: depth len/256 + 9.
 
+template <class F>
 __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     uint32_t limit, uint4* __restrict__ dest,
@@ -735,7 +797,7 @@ __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__
         for (uint64_t p = i; p < j; p++) {
             if (!(keys[p] & ZK_REC_INVALID)) {
                 QXyzz q = load_qxyzz(pts, p);
-                if (has) qxyzz_add(acc, q);
+                if (has) Arith<F>::qxyzz_add(acc, q);
                 else acc = q;
                 has = true;
             }
@@ -744,6 +806,7 @@ __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__
     }
 }
 
+template <class F>
 __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ keys,
                                                    const uint4* __restrict__ pts, uint64_t nrec,
                                                    const uint32_t* __restrict__ mid_count,
@@ -768,7 +831,7 @@ __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ 
                 has |= !(kr & ZK_REC_INVALID);
             }
         }
-        const QXyzz acc = sum_then_tree(wave_max(cnt), 8, [&](uint32_t e) {
+        const QXyzz acc = sum_then_tree<F>(wave_max(cnt), 8, [&](uint32_t e) {
             const uint64_t p = start + sub + 8ull * e;
             return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
         });
@@ -778,6 +841,7 @@ __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ 
     }
 }
 
+template <class F>
 __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     const uint32_t* __restrict__ long_count,
@@ -798,7 +862,7 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
             cnt++;
             has |= !(kr & ZK_REC_INVALID);
         }
-        QXyzz acc = sum_then_tree(wave_max(cnt), 64, [&](uint32_t e) {
+        QXyzz acc = sum_then_tree<F>(wave_max(cnt), 64, [&](uint32_t e) {
             const uint64_t p = start + t + 256ull * e;
             return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
         });
@@ -812,7 +876,7 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
 #pragma unroll 1
             for (uint32_t w = 1; w < 4; w++) {
                 const QXyzz o = load_qxyzz(&part[w][0], 0);
-                qxyzz_add(acc, o);
+                Arith<F>::qxyzz_add(acc, o);
             }
             if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_qxyzz(dest, k, acc);
         }
@@ -842,6 +906,7 @@ __device__ __forceinline__ void seg_shape(uint32_t nbk, uint32_t& g, uint32_t& h
 
 __device__ __forceinline__ uint32_t lanes_per(uint32_t len, uint32_t per) { return len >= per ? len / per : 1; }
 
+template <class F>
 __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
                                                    const uint64_t* __restrict__ seg_wave0,
                                                    const uint64_t* __restrict__ seg_rc0,
@@ -867,7 +932,7 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     const uint32_t nitem = rows ? nrow : ncol;
     const uint32_t len = rows ? ncol : nrow;  // buckets per item
     const uint32_t cnt = len / P;  // buckets this lane sums
-    const QXyzz acc = sum_then_tree(cnt, P, [&](uint32_t e) {
+    const QXyzz acc = sum_then_tree<F>(cnt, P, [&](uint32_t e) {
         const uint32_t x = part * cnt + e;
         const uint64_t kk = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
         return item < nitem ? load_qxyzz(buckets, sg.bkt0 + kk) : qxyzz_identity();
@@ -880,6 +945,7 @@ __device__ __forceinline__ uint32_t with_bit(uint32_t i, uint32_t b) {
     return ((i >> b) << (b + 1)) | (1u << b) | (i & ((1u << b) - 1u));
 }
 
+template <class F>
 __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
                                                  const uint64_t* __restrict__ seg_rc0,
                                                  uint32_t nseg, const uint4* __restrict__ rc,
@@ -906,11 +972,11 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
     }
     // count is a power of 2: lanes take elements lane, lane+64, ...
     const uint32_t P = count >= 64 ? 64u : count;
-    const QXyzz acc = sum_then_tree((count + 63) / 64, P, [&](uint32_t e) {
+    const QXyzz acc = sum_then_tree<F>((count + 63) / 64, P, [&](uint32_t e) {
         const uint32_t i = lane + 64 * e;
         return i < count ? load_qxyzz(rc, base + (masked ? with_bit(i, b) : i)) : qxyzz_identity();
     });
-    if (lane == 0) store_xyzz(bits, item, qxyzz_to_xyzz(acc));  // host form
+    if (lane == 0) store_xyzz(bits, item, Arith<F>::qxyzz_to_xyzz(acc));  // host form
 }
 
 // ------------------------------------------------ synthetic base table
@@ -959,8 +1025,11 @@ __global__ __launch_bounds__(64) void k_test_fp(int op, const uint4* __restrict_
         case 2: r = fp_sub(x, y); break;
         case 3: r = fp_inv(x); break;
         case 4: r = fp_neg(x); break;
-        case 6: r = fq_to_fp(fq_mul(fq_from_fp(x), fq_from_fp(y))); break;
-        case 7: r = fq_to_fp(fq_sqr(fq_from_fp(x))); break;
+        case 6: r = Ap::fq_to_fp(Ap::fq_mul(Ap::fq_from_fp(x), Ap::fq_from_fp(y))); break;
+        case 7: r = Ap::fq_to_fp(Ap::fq_sqr(Ap::fq_from_fp(x))); break;
+        // the same over Fr (inputs and outputs x 2^256 mod r)
+        case 8: r = Ar::fq_to_fp(Ar::fq_mul(Ar::fq_from_fp(x), Ar::fq_from_fp(y))); break;
+        case 9: r = Ar::fq_to_fp(Ar::fq_sqr(Ar::fq_from_fp(x))); break;
         default: r = fp_canon(x); break;
     }
     store_fp(out + 2 * i, r);
@@ -969,7 +1038,8 @@ __global__ __launch_bounds__(64) void k_test_fp(int op, const uint4* __restrict_
 // Points in and out in the 2^256 Montgomery form.  op 0: P + Q (XYZZ add),
 // 1: 2P, 2: P + affine(Q.X, Q.Y) with (0,0) as identity -- on the 29-bit
 // arithmetic of the MSM kernels; ops 3, 4, 5: the same on the 32-bit
-// arithmetic (used by the base generator).
+// arithmetic (used by the base generator); ops 6, 7, 8: ops 0, 1, 2 on
+// Grumpkin (coordinates in Fr).
 __global__ __launch_bounds__(64) void k_test_xyzz(int op, const uint4* __restrict__ p, const uint4* __restrict__ q,
                             uint4* __restrict__ out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -987,16 +1057,29 @@ __global__ __launch_bounds__(64) void k_test_xyzz(int op, const uint4* __restric
         store_xyzz(out, i, a);
         return;
     }
-    QXyzz qa = qxyzz_from_xyzz(a), qb = qxyzz_from_xyzz(b);
-    if (op == 0) qxyzz_add(qa, qb);
-    else if (op == 1) qa = qxyzz_dbl(qa);
+    if (op >= 6) {
+        QXyzz qa = Ar::qxyzz_from_xyzz(a), qb = Ar::qxyzz_from_xyzz(b);
+        if (op == 6) Ar::qxyzz_add(qa, qb);
+        else if (op == 7) qa = Ar::qxyzz_dbl(qa);
+        else {
+            QAff t;
+            t.x = qb.X;
+            t.y = qb.Y;
+            Ar::qxyzz_madd(qa, t.x, t.y, qaff_is_id(t));
+        }
+        store_xyzz(out, i, Ar::qxyzz_to_xyzz(qa));
+        return;
+    }
+    QXyzz qa = Ap::qxyzz_from_xyzz(a), qb = Ap::qxyzz_from_xyzz(b);
+    if (op == 0) Ap::qxyzz_add(qa, qb);
+    else if (op == 1) qa = Ap::qxyzz_dbl(qa);
     else {
         QAff t;
         t.x = qb.X;
         t.y = qb.Y;
-        qxyzz_madd(qa, t.x, t.y, qaff_is_id(t));
+        Ap::qxyzz_madd(qa, t.x, t.y, qaff_is_id(t));
     }
-    store_xyzz(out, i, qxyzz_to_xyzz(qa));
+    store_xyzz(out, i, Ap::qxyzz_to_xyzz(qa));
 }
 
 __global__ __launch_bounds__(64) void k_test_glv(const uint4* __restrict__ scalars, uint4* __restrict__ out,
@@ -1028,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_
             b.v[i] = (t * 2654435761u + i) & ZQ_MASK;
             a.v[i] = (t * 40503u + i * 7u) & ZQ_MASK;
         }
-        for (uint32_t it = 0; it < iters; it++) a = fq_mul(a, b);
+        for (uint32_t it = 0; it < iters; it++) a = Ap::fq_mul(a, b);
         uint64_t x = 0;
 #pragma unroll
         for (int i = 0; i < 9; i++) x ^= a.v[i];
@@ -1045,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_probe(int mode, uint32_t iters, uint64_
             for (int i = 0; i < 9; i++) a[k].v[i] = ((t + k) * 40503u + i * 7u) & ZQ_MASK;
         for (uint32_t it = 0; it < iters; it++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) a[k] = fq_mul(a[k], b);
+            for (int k = 0; k < 4; k++) a[k] = Ap::fq_mul(a[k], b);
         uint64_t x = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -1145,37 +1228,51 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) {
     return (unsigned)((n + bs - 1) / bs);
 }
 
-static inline uint64_t* table_idmask(void* table, uint64_t n) {
-    return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) + n * 2 * kTableEntryBytes);
+static inline uint64_t* table_idmask(void* table, uint64_t n, CurveId cv) {
+    return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) +
+                                       n * entries_per_point(cv) * kTableEntryBytes);
 }
 
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_import_be, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint8_t*)raw, n, (uint4*)table, table_idmask(table, n), bad);
+                       (const uint8_t*)raw, n, (uint4*)table, table_idmask(table, n, CurveId::Bn254),
+                       bad);
 }
 
-void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st) {
+void launch_table_export(CurveId cv, const void* table, uint64_t n, void* pts, hipStream_t st) {
     if (!n) return;
-    hipLaunchKernelGGL(k_table_export, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint4*)table, n, (uint4*)pts);
+    if (cv == CurveId::Grumpkin)
+        hipLaunchKernelGGL((k_table_export<Bn254Fr, 1>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const uint4*)table, n, (uint4*)pts);
+    else
+        hipLaunchKernelGGL((k_table_export<Bn254Fp, 2>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const uint4*)table, n, (uint4*)pts);
 }
 
-void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st) {
+void launch_build_table(CurveId cv, const void* pts, void* table, uint64_t n, hipStream_t st) {
     if (!n) return;
-    hipLaunchKernelGGL(k_build_table, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                       (const uint4*)pts, (uint4*)table, table_idmask(table, n), n);
+    uint64_t* idm = table_idmask(table, n, cv);
+    if (cv == CurveId::Grumpkin)
+        hipLaunchKernelGGL((k_build_table<Bn254Fr, 1>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const uint4*)pts, (uint4*)table, idm, n);
+    else
+        hipLaunchKernelGGL((k_build_table<Bn254Fp, 2>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const uint4*)pts, (uint4*)table, idm, n);
 }
 
-void launch_decompose(const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
+void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
                       uint32_t dtiles, const SegDesc* segs, const void* scalars,
                       const void* table, uint64_t table_n, void* digits, uint32_t* hist,
                       uint32_t* err, hipStream_t st) {
     if (!dtiles) return;
-    hipLaunchKernelGGL(k_decompose, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
-                       nmsm, segs, (const uint4*)scalars,
-                       (const uint64_t*)table_idmask(const_cast<void*>(table), table_n),
-                       (uint2*)digits, hist, err);
+    const uint64_t* idm = table_idmask(const_cast<void*>(table), table_n, cv);
+    if (cv == CurveId::Grumpkin)
+        hipLaunchKernelGGL(k_decompose<1>, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+                           nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
+    else
+        hipLaunchKernelGGL(k_decompose<2>, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+                           nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
 }
 
 void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
@@ -1208,7 +1305,7 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st) {
 uint64_t accumulate_capacity(int device) {
     int cus = 0, blocks = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate, 256, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate<Bn254Fp>, 256, 0) != hipSuccess ||
         cus <= 0 || blocks <= 0) {
         (void)hipGetLastError();
         return 1u << 17;
@@ -1216,43 +1313,68 @@ uint64_t accumulate_capacity(int device) {
     return (uint64_t)cus * (uint64_t)blocks * 256u;
 }
 
-void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
+void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st) {
     if (!lanes) return;
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(lanes, 256)), dim3(256), 0, st, segs,
-                       seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table, T,
-                       lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
+    if (cv == CurveId::Grumpkin)
+        hipLaunchKernelGGL(k_accumulate<Bn254Fr>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st,
+                           segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table,
+                           T, lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
+    else
+        hipLaunchKernelGGL(k_accumulate<Bn254Fp>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st,
+                           segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table,
+                           T, lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
 }
 
-void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
-                       void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
-                       hipStream_t st) {
-    if (!nrec || !list_cap) return;
+template <class F>
+static void merge_runs_t(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+                         void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
+                         hipStream_t st) {
     uint32_t* long_list = lists;
     uint32_t* mid_list = lists + list_cap;
-    hipLaunchKernelGGL(k_merge_runs, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
+    hipLaunchKernelGGL(k_merge_runs<F>, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
                        (const uint4*)pts, nrec, limit, (uint4*)dest, counts, long_list,
                        counts + 1, mid_list, list_cap);
     // grid-stride over the lists (their lengths are known on the device only)
-    hipLaunchKernelGGL(k_merge_mid, dim3(std::min<uint32_t>(blocks_for(list_cap, 32), 1024)), dim3(256),
+    hipLaunchKernelGGL(k_merge_mid<F>, dim3(std::min<uint32_t>(blocks_for(list_cap, 32), 1024)), dim3(256),
                        0, st, keys, (const uint4*)pts, nrec, (const uint32_t*)(counts + 1),
                        (const uint32_t*)mid_list, list_cap, (uint4*)dest);
-    hipLaunchKernelGGL(k_merge_long, dim3(std::min<uint32_t>(list_cap, 2048)), dim3(256), 0, st, keys,
+    hipLaunchKernelGGL(k_merge_long<F>, dim3(std::min<uint32_t>(list_cap, 2048)), dim3(256), 0, st, keys,
                        (const uint4*)pts, nrec, (const uint32_t*)counts,
                        (const uint32_t*)long_list, list_cap, (uint4*)dest);
 }
 
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
+void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64_t nrec,
+                       uint32_t limit, void* dest, uint32_t* counts, uint32_t* lists,
+                       uint32_t list_cap, hipStream_t st) {
+    if (!nrec || !list_cap) return;
+    if (cv == CurveId::Grumpkin)
+        merge_runs_t<Bn254Fr>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
+    else
+        merge_runs_t<Bn254Fp>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
+}
+
+template <class F>
+static void bucket_reduce_t(const SegDesc* segs, const uint64_t* seg_wave0,
+                            const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
+                            const void* buckets, void* rc, void* bits, hipStream_t st) {
+    hipLaunchKernelGGL(k_br_rowcol<F>, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
+                       seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
+    const uint64_t bit_items = (uint64_t)nseg * 16;
+    hipLaunchKernelGGL(k_br_bits<F>, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
+                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);
+}
+
+void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
                           const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
                           const void* buckets, void* rc, void* bits, hipStream_t st) {
     if (!rc_waves) return;
-    hipLaunchKernelGGL(k_br_rowcol, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
-                       seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
-    const uint64_t bit_items = (uint64_t)nseg * 16;
-    hipLaunchKernelGGL(k_br_bits, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
-                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);
+    if (cv == CurveId::Grumpkin)
+        bucket_reduce_t<Bn254Fr>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+    else
+        bucket_reduce_t<Bn254Fp>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
 }
 
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
@@ -1260,7 +1382,7 @@ void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
     if (!n) return;
     hipLaunchKernelGGL(k_gen_bases, dim3(blocks_for(n, 256)), dim3(256), 0, st,
                        (const uint4*)ks, (const uint4*)gtab, n, (uint4*)table,
-                       table_idmask(table, n));
+                       table_idmask(table, n, CurveId::Bn254));
 }
 
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,

@@ -7,16 +7,20 @@
 
 namespace zk {
 
-// A table of n points takes table_bytes(n): n pairs of 128-byte entries, then
-// a bitmap with bit i set when point i is the identity (written by the table
-// builders, read by the decomposition).
-inline uint64_t table_bytes(uint64_t n) { return n * 2 * kTableEntryBytes + (n + 63) / 64 * 8; }
+// A table of n points of curve cv takes table_bytes(n, cv): n rows of
+// entries_per_point(cv) 128-byte entries, then a bitmap with bit i set when
+// point i is the identity (written by the table builders, read by the
+// decomposition).
+inline uint64_t table_bytes(uint64_t n, CurveId cv = CurveId::Bn254) {
+    return n * entries_per_point(cv) * kTableEntryBytes + (n + 63) / 64 * 8;
+}
+// BN254 G1 only: SRS import from big-endian standard-form coordinates
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
-void launch_table_export(const void* table, uint64_t n, void* pts, hipStream_t st);
-void launch_build_table(const void* pts, void* table, uint64_t n, hipStream_t st);
+void launch_table_export(CurveId cv, const void* table, uint64_t n, void* pts, hipStream_t st);
+void launch_build_table(CurveId cv, const void* pts, void* table, uint64_t n, hipStream_t st);
 // one block per first-pass sort tile of each MSM (msm_dtile0, dtiles blocks):
 // digits, and the first sort pass's tile histograms into hist
-void launch_decompose(const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
+void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,
                       uint32_t dtiles, const SegDesc* segs, const void* scalars,
                       const void* table, uint64_t table_n, void* digits, uint32_t* hist,
                       uint32_t* err, hipStream_t st);
@@ -30,16 +34,16 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 // CU at its register use x 256): a lane count just above it leaves a second,
 // nearly empty round of waves
 uint64_t accumulate_capacity(int device);
-void launch_accumulate(const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
+void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
 // counts[0] / counts[1]: long / mid run counts (zeroed by the caller);
 // lists: 2 * list_cap words (long list, then mid list)
-void launch_merge_runs(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
+void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
                        void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                        hipStream_t st);
-void launch_bucket_reduce(const SegDesc* segs, const uint64_t* seg_wave0,
+void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
                           const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
                           const void* buckets, void* rc, void* bits, hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,

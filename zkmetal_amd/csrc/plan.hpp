@@ -12,6 +12,14 @@
 
 namespace zk {
 
+// The two curves of the BN254 cycle the engine runs MSMs on: BN254 G1
+// (coordinates in Fp, scalars mod r, GLV: two table entries per base, P and
+// phi(P), and two 128-bit half-scalars) and Grumpkin (y^2 = x^3 - 17 over Fr,
+// scalars mod p, no GLV: one entry per base, 255-bit scalars).
+enum class CurveId : uint32_t { Bn254 = 0, Grumpkin = 1 };
+constexpr uint32_t entries_per_point(CurveId cv) { return cv == CurveId::Bn254 ? 2u : 1u; }
+constexpr uint32_t scalar_window_span(CurveId cv) { return cv == CurveId::Bn254 ? 128u : 255u; }
+
 // bytes per base-table entry (one affine point in the kernels' field form)
 constexpr uint64_t kTableEntryBytes = 128;
 // bytes per stored XYZZ point (buckets, merge records, reduction partials)
