@@ -70,6 +70,22 @@ ZkMetalStatus zkmsm_kzg_commit(ZkMetalMSMEngine engine, const uint8_t* coeffs_mo
 ZkMetalStatus zkmsm_kzg_commit_device(ZkMetalMSMEngine engine, const void* device_coeffs_mont,
                                       const uint32_t* counts, uint32_t n_polys, uint8_t* results);
 
+/* Grumpkin, BN254's cycle partner (y^2 = x^3 - 17 over BN254 Fr; the
+ * reference's GrumpkinMSM.msm, Sources/zkMetal/MSM/GrumpkinMSMEngine.swift:246),
+ * on the same engine.  points: n x 64 B affine (x, y in Fr Montgomery form,
+ * 8 x u32 LE each; (0,0) = identity); scalars: n x 32 B standard form,
+ * reduced mod the Grumpkin order (BN254's p) as the reference does
+ * (GrumpkinMSMEngine.swift:178-196); result: Jacobian X | Y | Z in Fr
+ * Montgomery form, normalised (Z = 1; the identity is (1, 1, 0)).  The batch
+ * form takes concatenated inputs like zkmetal_bn254_msm_batch (a zero count
+ * writes 96 zero bytes). */
+ZkMetalStatus zkmsm_grumpkin_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                 const uint8_t* scalars, uint32_t n, uint8_t* X, uint8_t* Y,
+                                 uint8_t* Z);
+ZkMetalStatus zkmsm_grumpkin_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
+                                       const uint8_t* all_scalars, const uint32_t* counts,
+                                       uint32_t n_msms, uint8_t* results);
+
 /* Host-point calls (zkmetal_bn254_msm, _batch, and the small-scalar
  * variants) whose point slices are each byte-for-byte a prefix of the
  * resident SRS run on the resident table: no base upload and no table build

@@ -1,0 +1,62 @@
+"""CPU pinning of the Grumpkin oracle (oracle/grumpkin.py) against the
+reference's constants and the group structure (no GPU)."""
+import random
+
+from oracle import bn254 as B
+from oracle import grumpkin as G
+
+
+def test_field_constants_match_the_reference():
+    # Sources/NeonFieldOps/grumpkin_ops.c:17-25
+    assert B.int_from_limbs(G.REF_P_LIMBS64, bits=64) == G.R
+    assert G.REF_INV64 == (-pow(G.R, -1, 2**64)) % 2**64
+    assert B.int_from_limbs(G.REF_ONE_LIMBS64, bits=64) == pow(2, 256, G.R)
+
+
+def test_generator_and_order():
+    # GrumpkinCurve.swift:215-222: G = (1, sqrt(-16)), the quoted y
+    assert G.on_curve(G.G)
+    assert (G.GEN_Y * G.GEN_Y + 16) % G.R == 0
+    assert G.mul(G.G, G.ORDER) is None  # order p (prime), G != O
+    assert G.add(G.G, G.G) is not None and G.add(G.G, G.neg(G.G)) is None
+    # a planted mistake in b is caught by the on-curve check
+    x, y = G.G
+    assert (y * y - x * x * x - (G.B + 1)) % G.R != 0
+
+
+def test_group_law_consistency():
+    rnd = random.Random(3)
+    pts = [G.mul(G.G, rnd.randrange(1, G.ORDER)) for _ in range(6)]
+    for p in pts:
+        assert G.on_curve(p)
+    a, b, c = pts[:3]
+    assert G.add(G.add(a, b), c) == G.add(a, G.add(b, c))
+    assert G.add(a, b) == G.add(b, a)
+    k1, k2 = rnd.randrange(G.ORDER), rnd.randrange(G.ORDER)
+    assert G.add(G.mul(G.G, k1), G.mul(G.G, k2)) == G.mul(G.G, k1 + k2)
+    # 5 G by repeated addition
+    acc = None
+    for _ in range(5):
+        acc = G.add(acc, G.G)
+    assert acc == G.mul(G.G, 5)
+
+
+def test_closed_form_equals_brute_force():
+    rnd = random.Random(4)
+    ks = [rnd.randrange(1, G.ORDER) for _ in range(5)]
+    ss = [rnd.randrange(0, 2**256) for _ in range(5)]
+    brute = None
+    for k, s in zip(ks, ss):
+        brute = G.add(brute, G.mul(G.mul(G.G, k), s))
+    assert brute == G.msm_closed_form(ks, ss)
+
+
+def test_byte_forms_round_trip():
+    p = G.mul(G.G, 12345)
+    b = G.point_to_bytes(p)
+    x = G.from_mont(int.from_bytes(b[:32], "little"))
+    y = G.from_mont(int.from_bytes(b[32:], "little"))
+    assert (x, y) == p
+    one = G.to_mont(1).to_bytes(32, "little")
+    assert G.jacobian_bytes_to_affine(b + one) == p
+    assert G.jacobian_bytes_to_affine(one + one + bytes(32)) is None

@@ -112,6 +112,8 @@ def lib():
         "zkmsm_server_footprint": (u64, [u32]),
         "zkmsm_multi_msm": (i32, [vp, vp, u32, vp, u32, vp]),
         "zkmsm_set_srs_detect": (i32, [vp, i32]),
+        "zkmsm_grumpkin_msm": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+        "zkmsm_grumpkin_msm_batch": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_srs_detect_hits": (u64, [vp]),
         "zkmsm_inner_product_msm": (i32, [vp, vp, vp, u32, vp, u32, vp]),
     }
@@ -261,6 +263,31 @@ class MetalMSM:
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
         _check(lib().zkmetal_bn254_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
                                              len(counts), ctypes.c_void_p(out.ctypes.data)), "msm_batch")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def grumpkin_msm(self, points, scalars):
+        """zkmsm_grumpkin_msm: Grumpkin points (x, y in Fr Montgomery form) and
+        scalars (standard form) -> 96-byte Jacobian X|Y|Z (Fr Montgomery,
+        normalised)."""
+        n = _count(points, 64)
+        if n == 0 or _count(scalars, 32) != n:
+            raise MSMError(ERR_INVALID_INPUT, "points/scalars")
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = (ctypes.c_uint8 * 96)()
+        base = ctypes.addressof(out)
+        _check(lib().zkmsm_grumpkin_msm(self.handle, pp, sp, n, base, base + 32, base + 64),
+               "grumpkin_msm")
+        return bytes(out)
+
+    def grumpkin_msm_batch(self, points, scalars, counts):
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        _check(lib().zkmsm_grumpkin_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
+                                              len(counts), ctypes.c_void_p(out.ctypes.data)),
+               "grumpkin_msm_batch")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
 
     def set_srs_detect(self, on):

@@ -137,6 +137,31 @@ ZkMetalStatus zkmetal_bn254_msm_batch_auto(const uint8_t* all_points, const uint
     return batch_impl(e, all_points, all_scalars, counts, n_msms, results);
 }
 
+ZkMetalStatus zkmsm_grumpkin_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                 const uint8_t* scalars, uint32_t n, uint8_t* x, uint8_t* y,
+                                 uint8_t* z) {
+    Engine* e = as_engine(engine);
+    if (!e || n == 0 || !points || !scalars || !x || !y || !z) return ZKMETAL_ERR_INVALID_INPUT;
+    uint8_t r[96];
+    int32_t rc;
+    {
+        std::lock_guard<std::mutex> lk(e->mutex());
+        rc = e->msm_batch_host(points, scalars, &n, 1, r, zk::CurveId::Grumpkin);
+    }
+    if (rc != zk::kOk) return rc;
+    return split_out(r, x, y, z);
+}
+
+ZkMetalStatus zkmsm_grumpkin_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
+                                       const uint8_t* all_scalars, const uint32_t* counts,
+                                       uint32_t n_msms, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e || n_msms == 0) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_batch_host(all_points, all_scalars, counts, n_msms, results,
+                             zk::CurveId::Grumpkin);
+}
+
 ZkMetalStatus zkmsm_set_srs_detect(ZkMetalMSMEngine engine, int32_t on) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;

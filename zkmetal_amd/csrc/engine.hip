@@ -112,6 +112,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         m.n = ns[j];
         m.c = c;
         m.nwin = (scalar_window_span(cv) + c - 1) / c;
+        if (m.nwin > kMaxWindows) return kInvalidInput;  // Grumpkin: c >= 4
         m.seg0 = (uint32_t)pl.segs.size();
         m.form = form;
         pl.msm_scalar0[j] = pl.scalars;

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
                                                           uint32_t* __restrict__ err) {
     // windows (nwin <= 64) x key byte 0, two 16-bit counts per word (a tile
     // has at most kSortTile <= 2^15 entries, so no count reaches 2^16)
-    __shared__ uint32_t cnt[64 * 128];
+    __shared__ uint32_t cnt[kMaxWindows * 128];
     const uint32_t b = blockIdx.x;
     const uint32_t j = upper_index(msm_dtile0, nmsm, b);
     const MsmDesc m = msms[j];
@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(64) void k_test_xyzz(int op, const uint4* __restric
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Xyzz a = load_xyzz(p, i), b = load_xyzz(q, i);
-    if (op >= 3) {
+    if (op >= 3 && op < 6) {
         if (op == 3) xyzz_add(a, b);
         else if (op == 4) a = xyzz_dbl(a);
         else {

@@ -19,6 +19,9 @@ namespace zk {
 enum class CurveId : uint32_t { Bn254 = 0, Grumpkin = 1 };
 constexpr uint32_t entries_per_point(CurveId cv) { return cv == CurveId::Bn254 ? 2u : 1u; }
 constexpr uint32_t scalar_window_span(CurveId cv) { return cv == CurveId::Bn254 ? 128u : 255u; }
+// windows per MSM the decomposition's histogram counters hold (c >= 2 for
+// BN254 G1, c >= 4 for Grumpkin)
+constexpr uint32_t kMaxWindows = 64;
 
 // bytes per base-table entry (one affine point in the kernels' field form)
 constexpr uint64_t kTableEntryBytes = 128;
