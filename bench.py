@@ -17,6 +17,7 @@ inputs, timed in the same run.
     python bench.py [--steps K] [--warmup W] [--logn 22] [--no-cpu]
     python bench.py --config batched|skewed|serve   (the batched, skewed and
                                                      serving workloads)
+    python bench.py --config grumpkin              (the same MSM on Grumpkin G1)
 """
 import argparse
 import json
@@ -57,6 +58,17 @@ def algorithmic_field_muls(n):
     return best
 
 
+def algorithmic_field_muls_grumpkin(n):
+    """The same count without GLV: n entries per window over ceil(254/c)
+    windows of the scalar (< p < 2^254)."""
+    best = None
+    for c in range(4, 25):
+        w = math.ceil(254 / c)
+        f = 10 * n * w + 28 * w * 2 ** (c - 1)
+        best = f if best is None else min(best, f)
+    return best
+
+
 def timed_steps(fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -70,24 +82,25 @@ def timed_steps(fn, steps, warmup):
     return total / steps, samples
 
 
-def cpu_baseline(bases_host, scalars_host, n, gpu_result, threads):
+def cpu_baseline(bases_host, scalars_host, n, gpu_result, threads, curve="bn254"):
     from oracle import ref  # the oracle is only the checker / CPU baseline leg
 
     t = []
     res = None
     for _ in range(2):
         s = time.perf_counter()
-        res = ref.pippenger(bases_host, scalars_host, n, 0, threads)
+        res = ref.pippenger(bases_host, scalars_host, n, 0, threads, curve=curve)
         t.append((time.perf_counter() - s) * 1e3)
-    aff = ref.jacobian_to_affine_bytes(res)
+    aff = ref.jacobian_to_affine_bytes(res, curve=curve)
     match = aff == bytes(gpu_result[:64])
+    fn = "bn254_pippenger_msm" if curve == "bn254" else "grumpkin_pippenger_msm"
     return {
         "value": round(min(t), 1),
         "unit": "ms per MSM",
         "cores": threads,
         "kind": "port",
         "sample": f"2 runs of the full n=2^{int(math.log2(n))} MSM on the same inputs (best reported); "
-                  f"C restatement of bn254_pippenger_msm, one thread per 16-bit window",
+                  f"C restatement of {fn}, one thread per 16-bit window",
         "matches_gpu": bool(match),
         "host_cpu": _cpu_model(),
         "host_nproc": os.cpu_count(),
@@ -192,6 +205,75 @@ def plain(args, eng, peak):
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(bases.tobytes(), scal.tobytes(), n, result["r"], args.cpu_threads)
+    dpts.free()
+    dsc.free()
+    return line
+
+
+def grumpkin(args, eng, peak):
+    """Grumpkin G1 MSM (no GLV: 255-bit signed windows over n entries), bases
+    k_i * G generated on the device, scalars uniform below r, both resident."""
+    n = 1 << args.logn
+    dpts = zk.DeviceBuffer(64 * n)
+    eng.grumpkin_points_generate(args.seed, n, dpts)
+    scal = zk.gen_scalars(args.seed + 1, n, 0)
+    dsc = zk.DeviceBuffer(scal.nbytes)
+    dsc.upload(scal)
+    result = {}
+
+    def step():
+        result["r"] = eng.grumpkin_msm_batch_device(dpts, dsc, [n])[0]
+        result.setdefault("t", []).append(eng.last_timing())
+
+    ms, samples = timed_steps(step, args.steps, args.warmup)
+    timings = result["t"][args.warmup:]
+    acc_ms = statistics.mean(t["accumulate_ms"] for t in timings)
+    valid = statistics.mean(t["valid_entries"] for t in timings)
+    stage = {k: round(statistics.mean(t[k] for t in timings), 4)
+             for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms",
+                       "copyout_ms", "device_ms", "host_tail_ms")}
+    achieved = valid * 10 * PRODUCTS_PER_FIELD_MUL / (acc_ms * 1e-3)
+    msm_products = algorithmic_field_muls_grumpkin(n) * PRODUCTS_PER_FIELD_MUL
+    line = {
+        "metric": f"Grumpkin G1 MSM ms at n=2^{args.logn}",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain Grumpkin G1 MSM, n=2^{args.logn}, scalars uniform below BN254 r (seeded), "
+                        f"bases k_i*G (seeded, generated on the device), affine Fr Montgomery; "
+                        f"bases+scalars resident in HBM",
+            "n": n,
+            "window_bits": int(timings[-1]["window_bits"]),
+        },
+        "roofline": {
+            "bound": "valu_int32_mul",
+            "kernel": "k_accumulate",
+            "achieved": float("%.4g" % (achieved / 1e12)),
+            "peak": float("%.4g" % (peak / 1e12)),
+            "unit": "T int32 products/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": None,
+        },
+        "extra": {
+            "ms_p50": round(statistics.median(samples), 4),
+            "msms_per_s": round(1e3 / ms, 2),
+            "stage_ms": stage,
+            "msm_roofline_frac": round(msm_products / (ms * 1e-3) / peak, 4),
+            "msm_algorithmic_field_muls": algorithmic_field_muls_grumpkin(n),
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(dpts.download(), scal.tobytes(), n, result["r"],
+                                            args.cpu_threads, curve="grumpkin")
     dpts.free()
     dsc.free()
     return line
@@ -370,7 +452,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=22)
     ap.add_argument("--seed", type=int, default=0x5EED)
-    ap.add_argument("--config", default="plain", choices=["plain", "batched", "skewed", "serve"])
+    ap.add_argument("--config", default="plain", choices=["plain", "batched", "skewed", "serve", "grumpkin"])
     ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=2, help="serve: engine slots")
@@ -395,6 +477,8 @@ def main():
         line = plain(args, eng, peak)
     elif args.config == "serve":
         line = serve(args, eng)
+    elif args.config == "grumpkin":
+        line = grumpkin(args, eng, peak)
     else:
         line = batched(args, eng, peak)
     eng.close()

@@ -85,6 +85,17 @@ ZkMetalStatus zkmsm_grumpkin_msm(ZkMetalMSMEngine engine, const uint8_t* points,
 ZkMetalStatus zkmsm_grumpkin_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
                                        const uint8_t* all_scalars, const uint32_t* counts,
                                        uint32_t n_msms, uint8_t* results);
+/* The same with bases and scalars already in device memory (the Grumpkin
+ * counterpart of zkmsm_msm_batch_device). */
+ZkMetalStatus zkmsm_grumpkin_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                              const void* device_scalars, const uint32_t* counts,
+                                              uint32_t n_msms, uint8_t* results);
+/* Synthetic Grumpkin bases: writes n points k_i * G (64 B each, Fr Montgomery,
+ * (0,0) for k_i = 0) into device memory, k_i = zkmsm_gen_scalars(seed, n, 0)
+ * and G = (1, sqrt(-16)) (GrumpkinCurve.swift:215-222).  For benchmarks and
+ * tests, like zkmsm_srs_generate. */
+ZkMetalStatus zkmsm_grumpkin_points_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n,
+                                             void* device_points);
 
 /* Host-point calls (zkmetal_bn254_msm, _batch, and the small-scalar
  * variants) whose point slices are each byte-for-byte a prefix of the

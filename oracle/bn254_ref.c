@@ -26,6 +26,11 @@
  * signed-digit recoding (Sources/Shaders/msm/msm_kernels.metal:399-439), and
  * scalar arithmetic mod r for the closed form sum(s_i*k_i)*G.
  *
+ * Built a second time with -DORACLE_GRUMPKIN (liboracle_grumpkin.so): the
+ * reference's Grumpkin CPU Pippenger (grumpkin_ops.c:528-560) is the same
+ * algorithm over the field r with group order p; only the constants differ.
+ * The GLV routines are BN254-only and are left out of that build.
+ *
  * Parity pins: see oracle/bn254.py's header (EIP-196 vectors, closed form,
  * brute force, algebraic invariants) and tests/test_oracle.py.
  */
@@ -36,6 +41,7 @@
 
 typedef unsigned __int128 u128;
 
+#ifndef ORACLE_GRUMPKIN
 /* p, -p^-1 mod 2^64, R mod p (bn254_msm.c:24-32) */
 static const uint64_t Q[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL,
                               0xb85045b68181585dULL, 0x30644e72e131a029ULL};
@@ -44,9 +50,25 @@ static const uint64_t MONE[4] = {0xd35d438dc58f0d9dULL, 0x0a78eb28f5c70b3dULL,
                                  0x666ea36f7879462cULL, 0x0e0a77c19a07df2fULL};
 static const uint64_t RR[4] = {0xf32cfc5b538afa89ULL, 0xb5e71911d44501fbULL,
                                0x47ab1eff0a417ff6ULL, 0x06d89f71cab8351fULL};
-/* scalar field order r (bn254_msm.c:704-707, GLV.swift:8-11) */
+/* group order r (bn254_msm.c:704-707, GLV.swift:8-11) */
 static const uint64_t RORD[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL,
                                  0xb85045b68181585dULL, 0x30644e72e131a029ULL};
+#else
+/* Grumpkin (y^2 = x^3 - 17 over BN254's r): the same Pippenger, point
+ * formulas and window rule as grumpkin_pippenger_msm
+ * (Sources/NeonFieldOps/grumpkin_ops.c:129-186, 304-377, 434-560) with the
+ * base field r: r, -r^-1 mod 2^64, R mod r (grumpkin_ops.c:17-25), R^2 mod r. */
+static const uint64_t Q[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL,
+                              0xb85045b68181585dULL, 0x30644e72e131a029ULL};
+static const uint64_t QINV = 0xc2e1f593efffffffULL;
+static const uint64_t MONE[4] = {0xac96341c4ffffffbULL, 0x36fc76959f60cd29ULL,
+                                 0x666ea36f7879462eULL, 0x0e0a77c19a07df2fULL};
+static const uint64_t RR[4] = {0x1bb8e645ae216da7ULL, 0x53fe3ab1e35c59e3ULL,
+                               0x8c49833d53bb8085ULL, 0x0216d0b17f4e44a5ULL};
+/* group order: BN254's base-field prime p */
+static const uint64_t RORD[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL,
+                                 0xb85045b68181585dULL, 0x30644e72e131a029ULL};
+#endif
 
 /* ---------------- Fp ---------------- */
 
@@ -516,6 +538,7 @@ void oracle_scalar_dot(const uint32_t *s, const uint32_t *k, int n, uint32_t out
     store_s(acc, out);
 }
 
+#ifndef ORACLE_GRUMPKIN
 /* GLV split (GLV.swift:37-92): k mod r -> (k1, k2, neg1, neg2) with
  * k == (neg1 ? -k1 : k1) + lambda * (neg2 ? -k2 : k2)  (mod r). */
 static const uint64_t G1C[3] = {0x7a7bd9d4391eb18dULL, 0x4ccef014a773d2cfULL, 0x2ULL};
@@ -583,6 +606,8 @@ void oracle_glv_decompose(const uint32_t kin[8], uint32_t k1o[8], uint32_t k2o[8
     *neg1 = n1;
     *neg2 = n2;
 }
+
+#endif /* ORACLE_GRUMPKIN */
 
 /* Signed c-bit digits (msm_kernels.metal:399-439): nwin windows, digit d in
  * (-2^(c-1), 2^(c-1)], returned as signed ints; *carry_out is the carry

@@ -196,3 +196,72 @@ def test_invalid_inputs(engine):
         engine.grumpkin_msm(b"", b"")
     with pytest.raises(zk.MSMError):
         engine.grumpkin_msm(bytes(64), bytes(64))
+
+
+def _gen_device(engine, seed, n):
+    d = zk.DeviceBuffer(64 * n)
+    engine.grumpkin_points_generate(seed, n, d)
+    return d
+
+
+def test_points_generate_matches_scalar_multiples(engine):
+    from helpers import scalars_to_ints
+    n = 3000
+    d = _gen_device(engine, 0x71, n)
+    try:
+        pts = d.download()
+    finally:
+        d.free()
+    ks = scalars_to_ints(zk.gen_scalars(0x71, n, 0))
+    rnd = random.Random(26)
+    for i in list(range(20)) + rnd.sample(range(n), 30):
+        b = pts[64 * i:64 * i + 64]
+        want = G.mul(G.G, ks[i])
+        assert b == G.point_to_bytes(want), i
+        assert G.on_curve(want)
+
+
+def test_device_entry_equals_host_entry_and_c_oracle(engine):
+    """2^16 bases on the device: device-buffer MSM == host-buffer MSM, and
+    both equal the C restatement of grumpkin_pippenger_msm."""
+    from oracle import ref
+    n = 1 << 16
+    d = _gen_device(engine, 0x72, n)
+    sc = zk.gen_scalars(0x73, n, 0)
+    ds = zk.DeviceBuffer(32 * n)
+    try:
+        ds.upload(sc)
+        pts = d.download()
+        dev = engine.grumpkin_msm_batch_device(d, ds, [n])[0]
+    finally:
+        d.free()
+        ds.free()
+    host = engine.grumpkin_msm(pts, sc.tobytes())
+    assert dev == host
+    cpu = ref.pippenger(pts, sc.tobytes(), n, curve="grumpkin")
+    assert G.jacobian_bytes_to_affine(dev) == G.jacobian_bytes_to_affine(cpu)
+
+
+def test_device_entry_closed_form_at_2_20_and_batch(engine):
+    from oracle import ref
+    n = 1 << 20
+    seed = 0x74
+    d = _gen_device(engine, seed, n)
+    ks = zk.gen_scalars(seed, n, 0)
+    sc = zk.gen_scalars(0x75, n, 0)
+    ds = zk.DeviceBuffer(32 * n)
+    try:
+        ds.upload(sc)
+        r = engine.grumpkin_msm_batch_device(d, ds, [n])[0]
+        # a batch over the same buffers: MSM j covers points/scalars of its slice
+        counts = [1000, 0, n - 1000]
+        rb = engine.grumpkin_msm_batch_device(d, ds, counts)
+    finally:
+        d.free()
+        ds.free()
+    want = G.mul(G.G, ref.scalar_dot(sc.tobytes(), ks.tobytes(), n, curve="grumpkin"))
+    assert G.jacobian_bytes_to_affine(r) == want
+    assert rb[1] == bytes(96)
+    lo = G.mul(G.G, ref.scalar_dot(sc[:1000].tobytes(), ks[:1000].tobytes(), 1000, curve="grumpkin"))
+    assert G.jacobian_bytes_to_affine(rb[0]) == lo
+    assert G.add(G.jacobian_bytes_to_affine(rb[0]), G.jacobian_bytes_to_affine(rb[2])) == want

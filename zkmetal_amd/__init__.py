@@ -114,6 +114,8 @@ def lib():
         "zkmsm_set_srs_detect": (i32, [vp, i32]),
         "zkmsm_grumpkin_msm": (i32, [vp, vp, vp, u32, vp, vp, vp]),
         "zkmsm_grumpkin_msm_batch": (i32, [vp, vp, vp, vp, u32, vp]),
+        "zkmsm_grumpkin_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
+        "zkmsm_grumpkin_points_generate": (i32, [vp, u64, u32, vp]),
         "zkmsm_srs_detect_hits": (u64, [vp]),
         "zkmsm_inner_product_msm": (i32, [vp, vp, vp, u32, vp, u32, vp]),
     }
@@ -173,6 +175,13 @@ class DeviceBuffer:
         p, keep = _buf(host)
         _check(lib().zkmsm_copy_to_device(self.ptr, p, self.nbytes), "copy_to_device")
         del keep
+
+    def download(self, nbytes=None):
+        nbytes = self.nbytes if nbytes is None else nbytes
+        out = np.zeros(nbytes, dtype=np.uint8)
+        _check(lib().zkmsm_copy_to_host(ctypes.c_void_p(out.ctypes.data), self.ptr, nbytes),
+               "copy_to_host")
+        return out.tobytes()
 
     def free(self):
         if self.ptr:
@@ -289,6 +298,24 @@ class MetalMSM:
                                               len(counts), ctypes.c_void_p(out.ctypes.data)),
                "grumpkin_msm_batch")
         return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def grumpkin_msm_batch_device(self, dpoints, dscalars, counts):
+        """Grumpkin MSMs with bases and scalars in DeviceBuffers."""
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        out = np.zeros(len(counts) * 96, dtype=np.uint8)
+        _check(lib().zkmsm_grumpkin_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
+                                                     ctypes.c_void_p(counts.ctypes.data), len(counts),
+                                                     ctypes.c_void_p(out.ctypes.data)),
+               "grumpkin_msm_batch_device")
+        return [out[i * 96:(i + 1) * 96].tobytes() for i in range(len(counts))]
+
+    def grumpkin_points_generate(self, seed, n, dpoints):
+        """n Grumpkin bases k_i * G (k_i = gen_scalars(seed, n, 0)) into a
+        DeviceBuffer of at least n * 64 bytes."""
+        if dpoints.nbytes < 64 * n:
+            raise MSMError(ERR_INVALID_INPUT, "device buffer too small")
+        _check(lib().zkmsm_grumpkin_points_generate(self.handle, seed, n, dpoints.ptr),
+               "grumpkin_points_generate")
 
     def set_srs_detect(self, on):
         """Serve host-point calls whose bases are prefixes of the resident SRS

@@ -162,6 +162,24 @@ ZkMetalStatus zkmsm_grumpkin_msm_batch(ZkMetalMSMEngine engine, const uint8_t* a
                              zk::CurveId::Grumpkin);
 }
 
+ZkMetalStatus zkmsm_grumpkin_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                              const void* device_scalars, const uint32_t* counts,
+                                              uint32_t n_msms, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_device(device_points, device_scalars, counts, n_msms, results,
+                         zk::CurveId::Grumpkin);
+}
+
+ZkMetalStatus zkmsm_grumpkin_points_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n,
+                                             void* device_points) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->grumpkin_points_generate(seed, n, device_points);
+}
+
 ZkMetalStatus zkmsm_set_srs_detect(ZkMetalMSMEngine engine, int32_t on) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;

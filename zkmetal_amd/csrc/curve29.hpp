@@ -259,6 +259,23 @@ struct Arith {
     // R mod p, R = 2^261
     static __device__ __forceinline__ Fq fq_one() { return F::one(); }
 
+    // a^(p-2) = a^-1 for a != 0 mod p (0 -> 0); a < 8p with normalised limbs.
+    // Left-to-right binary exponentiation over the 32-bit words of p - 2:
+    // 256 squarings and one multiply per set bit.  Result < 1.5p.
+    static __device__ Fq fq_inv(const Fq& a) {
+        Fq r = fq_one();
+#pragma unroll 1
+        for (int w = 7; w >= 0; w--) {
+            const uint32_t e = F::p32(w) - (w == 0 ? 2u : 0u);  // p32(0) >= 2
+#pragma unroll 1
+            for (int b = 31; b >= 0; b--) {
+                r = fq_sqr(r);
+                if ((e >> b) & 1u) r = fq_mul(r, a);
+            }
+        }
+        return r;
+    }
+
     // beta * R mod p (phi(x, y) = (beta x, y)); BN254 G1 only
     static __device__ __forceinline__ Fq fq_beta() { return F::beta(); }
 

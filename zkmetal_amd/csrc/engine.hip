@@ -77,15 +77,14 @@ void HostBuf::release() {
 
 // ------------------------------------------------------------------ planner
 
-static inline uint32_t nwin_for(uint32_t c) { return (128 + c - 1) / c; }
-
-uint32_t default_window_bits(uint64_t n) {
-    // field multiplications: W * (2n * 10 madd + 2^(c-1) * ~40 for bucket reduction)
+uint32_t default_window_bits(uint64_t n, CurveId cv) {
+    // field multiplications: W * (entries * 10 madd + 2^(c-1) * ~40 for bucket
+    // reduction), W = ceil(span / c) windows of entries_per_point * n entries
     uint32_t best = 16;
     double best_cost = 1e300;
     for (uint32_t c = 4; c <= 16; c++) {
-        double W = nwin_for(c);
-        double cost = W * (20.0 * (double)n + 40.0 * (double)(1u << (c - 1)));
+        double W = (scalar_window_span(cv) + c - 1) / c;
+        double cost = W * (10.0 * entries_per_point(cv) * (double)n + 40.0 * (double)(1u << (c - 1)));
         if (cost < best_cost) {
             best_cost = cost;
             best = c;
@@ -104,7 +103,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     pl.msm_dtile0.resize(nmsm + 1);
     for (uint32_t j = 0; j < nmsm; j++) {
         if (ns[j] == 0) return kInvalidInput;
-        uint32_t c = (cs && cs[j]) ? cs[j] : default_window_bits(ns[j]);
+        uint32_t c = (cs && cs[j]) ? cs[j] : default_window_bits(ns[j], cv);
         if (c < 2 || c > 16) return kInvalidInput;
         MsmDesc& m = pl.msms[j];
         m.scalar0 = pl.scalars;
@@ -564,7 +563,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0,
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
                        acc_cap_, cv);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * 64);
@@ -597,7 +596,7 @@ int32_t Engine::multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t*
 }
 
 int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
-                           uint32_t nmsm, uint8_t* out96) {
+                           uint32_t nmsm, uint8_t* out96, CurveId cv) {
     if (!d_points || !d_scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
     std::vector<uint32_t> ns, idx;
     std::vector<uint64_t> t0;
@@ -608,17 +607,18 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
             continue;
         }
         ns.push_back(counts[j]);
-        t0.push_back(2 * total);
+        t0.push_back(entries_per_point(cv) * total);
         idx.push_back(j);
         total += counts[j];
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, 0, acc_cap_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
+                       acc_cap_, cv);
     if (rc != kOk) return rc;
-    ZK_ALLOC(ptable_, table_bytes(total));
-    launch_build_table(CurveId::Bn254, d_points, ptable_.p, total, stream_);
+    ZK_ALLOC(ptable_, table_bytes(total, cv));
+    launch_build_table(cv, d_points, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * 96);
     rc = run(pl, d_scalars, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
@@ -697,12 +697,11 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
 }
 
 // b * 2^(8w) * G for w < 32, b < 256, affine Montgomery (entry 0 = (0,0))
-int32_t Engine::ensure_gtab() {
-    if (gtab_.p) return kOk;
+// gtab[w*256 + b] = b * 2^(8w) * G, affine in the 2^256 Montgomery form of
+// the field H works in, entry 0 = (0,0); uploaded into dst
+template <class H>
+static int32_t build_gtab(const zkh::Fp& gx, const zkh::Fp& gy, DevBuf& dst) {
     std::vector<zkh::Xyzz> pts(32 * 256);
-    using H = zkh::HP;
-    zkh::Fp gx = H::mul(zkh::Fp{{1, 0, 0, 0}}, H::r2());
-    zkh::Fp gy = H::mul(zkh::Fp{{2, 0, 0, 0}}, H::r2());
     zkh::Xyzz base = H::from_affine(gx, gy);
     for (int w = 0; w < 32; w++) {
         zkh::Xyzz acc = H::identity();
@@ -723,8 +722,39 @@ int32_t Engine::ensure_gtab() {
     DevBuf tmp;
     if (tmp.ensure(blob.size()) != hipSuccess) return kAllocFailed;
     ZK_TRY(hipMemcpy(tmp.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    std::swap(gtab_.p, tmp.p);
-    std::swap(gtab_.cap, tmp.cap);
+    std::swap(dst.p, tmp.p);
+    std::swap(dst.cap, tmp.cap);
+    return kOk;
+}
+
+// BN254 G1 generator (1, 2)
+int32_t Engine::ensure_gtab() {
+    if (gtab_.p) return kOk;
+    using H = zkh::HP;
+    return build_gtab<H>(H::mul(zkh::Fp{{1, 0, 0, 0}}, H::r2()),
+                         H::mul(zkh::Fp{{2, 0, 0, 0}}, H::r2()), gtab_);
+}
+
+// Grumpkin generator (1, sqrt(-16)) with the y of the Aztec/Noir
+// specification (Sources/zkMetal/Curve/GrumpkinCurve.swift:215-222)
+int32_t Engine::ensure_gtab_grumpkin() {
+    if (gtab_gk_.p) return kOk;
+    using H = zkh::HR;
+    const zkh::Fp gy{{0x833fc48d823f272cULL, 0x2d270d45f1181294ULL, 0xcf135e7506a45d63ULL, 0x2ULL}};
+    return build_gtab<H>(H::mul(zkh::Fp{{1, 0, 0, 0}}, H::r2()), H::mul(gy, H::r2()), gtab_gk_);
+}
+
+int32_t Engine::grumpkin_points_generate(uint64_t seed, uint32_t n, void* d_out) {
+    if (n == 0 || !d_out) return kInvalidInput;
+    int rc = ensure_gtab_grumpkin();
+    if (rc != kOk) return rc;
+    std::vector<uint8_t> ks((uint64_t)n * 32);
+    zkh::gen_scalars(seed, n, 0, ks.data());
+    ZK_ALLOC(tmp_, (uint64_t)n * 32);
+    ZK_TRY(hipMemcpyAsync(tmp_.p, ks.data(), ks.size(), hipMemcpyHostToDevice, stream_));
+    launch_gen_points_grumpkin(tmp_.p, gtab_gk_.p, n, d_out, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
     return kOk;
 }
 
@@ -833,11 +863,11 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     return kOk;
 }
 
-std::vector<uint32_t> Engine::window_bits_for(const std::vector<uint32_t>& ns) const {
+std::vector<uint32_t> Engine::window_bits_for(const std::vector<uint32_t>& ns, CurveId cv) const {
     std::vector<uint32_t> cs(ns.size());
     for (size_t j = 0; j < ns.size(); j++) {
         uint32_t c = c_override_;
-        if (!c) {
+        if (!c && cv == CurveId::Bn254) {
             uint32_t cls = 0;
             while ((1ull << cls) < ns[j]) cls++;  // ceil(log2 n)
             cls = std::min<uint32_t>(cls, 32);

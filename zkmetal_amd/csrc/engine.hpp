@@ -81,7 +81,7 @@ struct Plan {
 };
 
 // window width for an MSM of n points (cost model; overridable)
-uint32_t default_window_bits(uint64_t n);
+uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
 
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
@@ -106,9 +106,12 @@ class Engine {
     int32_t multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t* scalars, uint32_t k,
                            uint8_t* out96);
 
-    // bases and scalars already in device memory (plain MSMs, phi table built per call)
+    // bases and scalars already in device memory (plain MSMs, table built per call)
     int32_t msm_device(const void* d_points, const void* d_scalars, const uint32_t* counts,
-                       uint32_t nmsm, uint8_t* out96);
+                       uint32_t nmsm, uint8_t* out96, CurveId cv = CurveId::Bn254);
+    // n Grumpkin points k_i * G (k_i = gen_scalars(seed, n, 0)) into a device
+    // buffer of n * 64 bytes
+    int32_t grumpkin_points_generate(uint64_t seed, uint32_t n, void* d_out);
 
     // device-resident base table (SRS)
     int32_t srs_upload(const uint8_t* points, uint32_t n);
@@ -151,7 +154,10 @@ class Engine {
         memcpy(ctab_batch_, o.ctab_batch_, sizeof(ctab_batch_));
         c_override_ = o.c_override_;
     }
-    std::vector<uint32_t> window_bits_for(const std::vector<uint32_t>& ns) const;
+    // per-MSM window widths: the override, else the tuned tables (BN254 G1
+    // only), else 0 = the cost model
+    std::vector<uint32_t> window_bits_for(const std::vector<uint32_t>& ns,
+                                          CurveId cv = CurveId::Bn254) const;
 
     void set_window_bits(uint32_t c) { c_override_ = c; }
     // graph mode: the device launch sequence of a call is captured into a
@@ -181,6 +187,7 @@ class Engine {
   private:
     Engine() = default;
     int32_t ensure_gtab();
+    int32_t ensure_gtab_grumpkin();
 
     int dev_ = 0;
     hipStream_t stream_ = nullptr;
@@ -197,7 +204,7 @@ class Engine {
 
     DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;
     DevBuf recAk_, recAp_, recBk_, rc_, winsum_, err_;  // recBk_: long-run list
-    DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, tmp_;
+    DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, gtab_gk_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
     // host image of the resident bases (64 B each) for the prefix check:

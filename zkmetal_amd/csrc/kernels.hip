@@ -1011,6 +1011,40 @@ __global__ __launch_bounds__(256) void k_gen_bases(const uint4* __restrict__ ks,
     store_table_pair(table, idmask, i, a);
 }
 
+// Affine points out[i] = k_i * G (64 B: x, y in the 2^256 Montgomery form,
+// (0,0) for k_i = 0) on the 29-bit arithmetic of field F, from a host table
+// gtab[w*256 + b] = b * 2^(8w) * G in the same form (entry 0 = (0,0)):
+// synthetic Grumpkin bases for benchmarks and tests.
+template <class F>
+__global__ __launch_bounds__(256) void k_gen_points(const uint4* __restrict__ ks,
+                                                    const uint4* __restrict__ gtab, uint64_t n,
+                                                    uint4* __restrict__ out) {
+    using A = Arith<F>;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 s0 = ks[2 * i], s1 = ks[2 * i + 1];
+    const uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    QXyzz acc = qxyzz_identity();
+    for (int w = 0; w < 32; w++) {
+        const uint32_t byte = (k[w >> 2] >> ((w & 3) * 8)) & 0xffu;
+        if (byte) {
+            const Aff q = load_aff(gtab, (uint64_t)w * 256 + byte);
+            A::qxyzz_madd(acc, A::fq_from_fp_canon(q.x), A::fq_from_fp_canon(q.y), false);
+        }
+    }
+    Aff a;
+    if (A::qxyzz_is_id(acc)) {
+        a.x = fp_zero();
+        a.y = fp_zero();
+    } else {
+        // 1/ZZ = ZZZ t and 1/ZZZ = ZZ t with t = 1/(ZZ ZZZ)
+        const Fq t = A::fq_inv(A::fq_mul(acc.ZZ, acc.ZZZ));
+        a.x = A::fq_to_fp(A::fq_mul(acc.X, A::fq_mul(t, acc.ZZZ)));
+        a.y = A::fq_to_fp(A::fq_mul(acc.Y, A::fq_mul(t, acc.ZZ)));
+    }
+    store_aff(out, i, a);
+}
+
 // ------------------------------------------------------ test entry points
 // Field / curve primitives exposed for the GPU unit tests.
 
@@ -1397,6 +1431,13 @@ void launch_test_xyzz(int op, const void* p, const void* q, void* out, uint64_t 
     if (!n) return;
     hipLaunchKernelGGL(k_test_xyzz, dim3(blocks_for(n, 64)), dim3(64), 0, st, op,
                        (const uint4*)p, (const uint4*)q, (uint4*)out, n);
+}
+
+void launch_gen_points_grumpkin(const void* ks, const void* gtab, uint64_t n, void* out,
+                                hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_gen_points<Bn254Fr>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                       (const uint4*)ks, (const uint4*)gtab, n, (uint4*)out);
 }
 
 void launch_test_glv(const void* scalars, void* out, uint64_t n, hipStream_t st) {

@@ -48,6 +48,9 @@ void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_w
                           const void* buckets, void* rc, void* bits, hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
+// out[i] = k_i * G on Grumpkin, 64-byte affine points (x, y in Fr Montgomery form)
+void launch_gen_points_grumpkin(const void* ks, const void* gtab, uint64_t n, void* out,
+                                hipStream_t st);
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,
                     hipStream_t st);
 void launch_test_xyzz(int op, const void* p, const void* q, void* out, uint64_t n,
