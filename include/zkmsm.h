@@ -97,6 +97,30 @@ ZkMetalStatus zkmsm_grumpkin_msm_batch_device(ZkMetalMSMEngine engine, const voi
 ZkMetalStatus zkmsm_grumpkin_points_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n,
                                              void* device_points);
 
+/* BN254 G2 (the reference's BN254G2MSM.msm,
+ * Sources/zkMetal/MSM/BN254G2MSMEngine.swift:289) on the same engine.
+ * points: n x 128 B affine, x.c0 | x.c1 | y.c0 | y.c1 (G2PointAffineGPU,
+ * BN254G2MSMEngine.swift:11-27: Fp Montgomery form, 8 x u32 LE each; all zero
+ * = identity); scalars: n x 32 B standard form, reduced mod r as the reference
+ * does (BN254G2MSMEngine.swift:214-232); result: 192 B Jacobian
+ * X | Y | Z, each c0 | c1 (G2PointProjectiveGPU, :29-43), normalised (Z = 1;
+ * the identity is ((1,0), (1,0), (0,0)), g2Identity).  The batch forms take
+ * concatenated inputs (results n_msms x 192 B, a zero count writes 192 zero
+ * bytes); _device takes bases and scalars already in device memory. */
+ZkMetalStatus zkmsm_bn254_g2_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                 const uint8_t* scalars, uint32_t n, uint8_t* result);
+ZkMetalStatus zkmsm_bn254_g2_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
+                                       const uint8_t* all_scalars, const uint32_t* counts,
+                                       uint32_t n_msms, uint8_t* results);
+ZkMetalStatus zkmsm_bn254_g2_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                              const void* device_scalars, const uint32_t* counts,
+                                              uint32_t n_msms, uint8_t* results);
+/* Synthetic G2 bases: n points k_i * G2 (128 B each) into device memory,
+ * k_i = zkmsm_gen_scalars(seed, n, 0), G2 the reference's generator
+ * (BN254Pairing.swift:389-396).  For benchmarks and tests. */
+ZkMetalStatus zkmsm_bn254_g2_points_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n,
+                                             void* device_points);
+
 /* Host-point calls (zkmetal_bn254_msm, _batch, and the small-scalar
  * variants) whose point slices are each byte-for-byte a prefix of the
  * resident SRS run on the resident table: no base upload and no table build
@@ -231,6 +255,13 @@ ZkMetalStatus zkmsm_test_fp(ZkMetalMSMEngine engine, int32_t op, const uint8_t* 
 /* op 0: p + q, 1: 2p, 2: p + affine(q.X, q.Y); XYZZ points of 128 B. */
 ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,
                               const uint8_t* q, uint8_t* out, uint32_t n);
+/* BN254 G2 primitives on 256-byte XYZZ points (X, Y, ZZ, ZZZ c0 parts, then
+ * the c1 parts; Fp Montgomery form).  op 0: p + q, 1: 2p, 2: p + affine
+ * (q.X, q.Y) (all zero = identity), 3: the bucket-loop addition p + (q.X, q.Y)
+ * (p the identity = a run start); field ops on X, result in X, the rest zero:
+ * 4: p.X * q.X, 5: p.X^2, 6: 1 / p.X. */
+ZkMetalStatus zkmsm_test_g2(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,
+                            const uint8_t* q, uint8_t* out, uint32_t n);
 /* GLV split of n scalars: out n x 48 B = k1 (16 B), k2 (16 B), neg1, neg2 (u32). */
 ZkMetalStatus zkmsm_test_glv(ZkMetalMSMEngine engine, const uint8_t* scalars, uint8_t* out,
                              uint32_t n);

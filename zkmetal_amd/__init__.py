@@ -116,6 +116,11 @@ def lib():
         "zkmsm_grumpkin_msm_batch": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_grumpkin_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_grumpkin_points_generate": (i32, [vp, u64, u32, vp]),
+        "zkmsm_bn254_g2_msm": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_bn254_g2_msm_batch": (i32, [vp, vp, vp, vp, u32, vp]),
+        "zkmsm_bn254_g2_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
+        "zkmsm_bn254_g2_points_generate": (i32, [vp, u64, u32, vp]),
+        "zkmsm_test_g2": (i32, [vp, i32, vp, vp, vp, u32]),
         "zkmsm_srs_detect_hits": (u64, [vp]),
         "zkmsm_inner_product_msm": (i32, [vp, vp, vp, u32, vp, u32, vp]),
     }
@@ -316,6 +321,54 @@ class MetalMSM:
             raise MSMError(ERR_INVALID_INPUT, "device buffer too small")
         _check(lib().zkmsm_grumpkin_points_generate(self.handle, seed, n, dpoints.ptr),
                "grumpkin_points_generate")
+
+    def g2_msm(self, points, scalars):
+        """zkmsm_bn254_g2_msm: G2 points (128 B: x.c0, x.c1, y.c0, y.c1, Fp
+        Montgomery) and scalars (standard form) -> 192-byte Jacobian
+        X | Y | Z (each c0 | c1), normalised."""
+        n = _count(points, 128)
+        if n == 0 or _count(scalars, 32) != n:
+            raise MSMError(ERR_INVALID_INPUT, "points/scalars")
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = np.zeros(192, dtype=np.uint8)
+        _check(lib().zkmsm_bn254_g2_msm(self.handle, pp, sp, n, ctypes.c_void_p(out.ctypes.data)), "g2_msm")
+        return out.tobytes()
+
+    def g2_msm_batch(self, points, scalars, counts):
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        pp, k1 = _buf(points)
+        sp, k2 = _buf(scalars)
+        out = np.zeros(len(counts) * 192, dtype=np.uint8)
+        _check(lib().zkmsm_bn254_g2_msm_batch(self.handle, pp, sp, ctypes.c_void_p(counts.ctypes.data),
+                                              len(counts), ctypes.c_void_p(out.ctypes.data)),
+               "g2_msm_batch")
+        return [out[i * 192:(i + 1) * 192].tobytes() for i in range(len(counts))]
+
+    def g2_msm_batch_device(self, dpoints, dscalars, counts):
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        out = np.zeros(len(counts) * 192, dtype=np.uint8)
+        _check(lib().zkmsm_bn254_g2_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
+                                                     ctypes.c_void_p(counts.ctypes.data), len(counts),
+                                                     ctypes.c_void_p(out.ctypes.data)),
+               "g2_msm_batch_device")
+        return [out[i * 192:(i + 1) * 192].tobytes() for i in range(len(counts))]
+
+    def g2_points_generate(self, seed, n, dpoints):
+        """n G2 bases k_i * G2 (k_i = gen_scalars(seed, n, 0)) into a
+        DeviceBuffer of at least n * 128 bytes."""
+        if dpoints.nbytes < 128 * n:
+            raise MSMError(ERR_INVALID_INPUT, "device buffer too small")
+        _check(lib().zkmsm_bn254_g2_points_generate(self.handle, seed, n, dpoints.ptr),
+               "g2_points_generate")
+
+    def test_g2(self, op, p, q):
+        n = _count(p, 256)
+        out = np.zeros(n * 256, dtype=np.uint8)
+        pp, k1 = _buf(p)
+        qp, k2 = _buf(q)
+        _check(lib().zkmsm_test_g2(self.handle, op, pp, qp, ctypes.c_void_p(out.ctypes.data), n), "test_g2")
+        return out.tobytes()
 
     def set_srs_detect(self, on):
         """Serve host-point calls whose bases are prefixes of the resident SRS

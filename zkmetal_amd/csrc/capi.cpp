@@ -180,6 +180,40 @@ ZkMetalStatus zkmsm_grumpkin_points_generate(ZkMetalMSMEngine engine, uint64_t s
     return e->grumpkin_points_generate(seed, n, device_points);
 }
 
+ZkMetalStatus zkmsm_bn254_g2_msm(ZkMetalMSMEngine engine, const uint8_t* points,
+                                 const uint8_t* scalars, uint32_t n, uint8_t* result) {
+    Engine* e = as_engine(engine);
+    if (!e || n == 0 || !points || !scalars || !result) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_batch_host(points, scalars, &n, 1, result, zk::CurveId::Bn254G2);
+}
+
+ZkMetalStatus zkmsm_bn254_g2_msm_batch(ZkMetalMSMEngine engine, const uint8_t* all_points,
+                                       const uint8_t* all_scalars, const uint32_t* counts,
+                                       uint32_t n_msms, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e || n_msms == 0) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_batch_host(all_points, all_scalars, counts, n_msms, results, zk::CurveId::Bn254G2);
+}
+
+ZkMetalStatus zkmsm_bn254_g2_msm_batch_device(ZkMetalMSMEngine engine, const void* device_points,
+                                              const void* device_scalars, const uint32_t* counts,
+                                              uint32_t n_msms, uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_device(device_points, device_scalars, counts, n_msms, results, zk::CurveId::Bn254G2);
+}
+
+ZkMetalStatus zkmsm_bn254_g2_points_generate(ZkMetalMSMEngine engine, uint64_t seed, uint32_t n,
+                                             void* device_points) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->g2_points_generate(seed, n, device_points);
+}
+
 ZkMetalStatus zkmsm_set_srs_detect(ZkMetalMSMEngine engine, int32_t on) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
@@ -426,6 +460,14 @@ ZkMetalStatus zkmsm_test_fp(ZkMetalMSMEngine engine, int32_t op, const uint8_t* 
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->test_fp(op, a, b, out, n);
+}
+
+ZkMetalStatus zkmsm_test_g2(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,
+                            const uint8_t* q, uint8_t* out, uint32_t n) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->test_g2(op, p, q, out, n);
 }
 
 ZkMetalStatus zkmsm_test_xyzz(ZkMetalMSMEngine engine, int32_t op, const uint8_t* p,

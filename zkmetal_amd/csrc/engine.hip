@@ -198,6 +198,7 @@ int32_t Engine::create(Engine** out) {
         return kGpuError;
     }
     e->acc_cap_ = accumulate_capacity(e->dev_);
+    e->acc_cap_g2_ = accumulate_capacity(e->dev_, CurveId::Bn254G2);
     memcpy(e->ctab_, kTunedPlain, sizeof(e->ctab_));
     memcpy(e->ctab_batch_, kTunedBatch, sizeof(e->ctab_batch_));
     for (auto& ev : e->ev_) {
@@ -252,28 +253,94 @@ static constexpr uint32_t kRunLimit = 8;
 // Pl_t for t < g, else Ph_{t-g}; U_w at t = 0): ~c W doublings and W c
 // additions per MSM; then the affine normalisation of all results with one
 // inversion.  hw: the window planes as copied back; H: the curve's field.
+// Host views of the window bit-plane sums: HostG1<H> for XYZZ over one field
+// (128-byte host points, 96-byte results), HostG2View for BN254 G2 (256-byte
+// host points: the c0 parts X, Y, ZZ, ZZZ, then the c1 parts; 192-byte
+// results X | Y | Z, each c0 | c1).  Results are normalised: Z = 1, the
+// identity is (1, 1, 0) (the reference's g2Identity for G2).
 template <class H>
-static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out96) {
+struct HostG1 {
+    using Pt = zkh::Xyzz;
+    static constexpr uint64_t kIn = 128, kOut = 96;
+    static Pt load(const uint8_t* b) {
+        Pt p;
+        memcpy(&p, b, 128);
+        return p;
+    }
+    static Pt identity() { return H::identity(); }
+    static bool is_id(const Pt& p) { return H::is_id(p); }
+    static Pt xdbl(const Pt& p) { return H::xdbl(p); }
+    static Pt xadd(const Pt& a, const Pt& b) { return H::xadd(a, b); }
+    static void finish(const std::vector<Pt>& res, uint8_t* out) {
+        std::vector<zkh::Fp> xs, ys;
+        H::batch_to_affine(res, xs, ys);
+        const zkh::Fp one = H::one(), zero = zkh::zero();
+        for (size_t j = 0; j < res.size(); j++) {
+            uint8_t* o = out + j * kOut;
+            const bool id = H::is_id(res[j]);
+            memcpy(o, (id ? one : xs[j]).v, 32);
+            memcpy(o + 32, (id ? one : ys[j]).v, 32);
+            memcpy(o + 64, (id ? zero : one).v, 32);
+        }
+    }
+};
+
+struct HostG2View {
+    using G = zkh::HostG2;
+    using Pt = zkh::Xyzz2;
+    static constexpr uint64_t kIn = 256, kOut = 192;
+    static Pt load(const uint8_t* b) {
+        Pt p;
+        zkh::Fp* c0[4] = {&p.X.c0, &p.Y.c0, &p.ZZ.c0, &p.ZZZ.c0};
+        zkh::Fp* c1[4] = {&p.X.c1, &p.Y.c1, &p.ZZ.c1, &p.ZZZ.c1};
+        for (int k = 0; k < 4; k++) {
+            memcpy(c0[k]->v, b + 32 * k, 32);
+            memcpy(c1[k]->v, b + 128 + 32 * k, 32);
+        }
+        return p;
+    }
+    static Pt identity() { return G::identity(); }
+    static bool is_id(const Pt& p) { return G::is_id(p); }
+    static Pt xdbl(const Pt& p) { return G::xdbl(p); }
+    static Pt xadd(const Pt& a, const Pt& b) { return G::xadd(a, b); }
+    static void finish(const std::vector<Pt>& res, uint8_t* out) {
+        const zkh::Fp2 one = G::one2(), zero = G::zero2();
+        for (size_t j = 0; j < res.size(); j++) {
+            uint8_t* o = out + j * kOut;
+            zkh::Fp2 x, y, z = one;
+            if (G::is_id(res[j])) {
+                x = one;
+                y = one;
+                z = zero;
+            } else {
+                G::to_affine(res[j], x, y);
+            }
+            const zkh::Fp2* c[3] = {&x, &y, &z};
+            for (int k = 0; k < 3; k++) {
+                memcpy(o + 64 * k, c[k]->c0.v, 32);
+                memcpy(o + 64 * k + 32, c[k]->c1.v, 32);
+            }
+        }
+    }
+};
+
+template <class HC>
+static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out) {
     const uint32_t nmsm = (uint32_t)pl.msms.size();
-    std::vector<zkh::Xyzz> res(nmsm);
+    std::vector<typename HC::Pt> res(nmsm);
     auto tail = [&](uint32_t jb, uint32_t je) {
         for (uint32_t j = jb; j < je; j++) {
             const MsmDesc& m = pl.msms[j];
             const uint32_t c = m.c, cb = c - 1, g = cb / 2;
-            zkh::Xyzz acc = H::identity();
+            typename HC::Pt acc = HC::identity();
             for (int b = (int)(c * (m.nwin - 1) + cb - 1); b >= 0; b--) {
-                if (!H::is_id(acc)) acc = H::xdbl(acc);
+                if (!HC::is_id(acc)) acc = HC::xdbl(acc);
                 const uint32_t w = (uint32_t)b / c, t = (uint32_t)b % c;
                 if (t >= cb) continue;  // bit c-1 of a window carries no plane
-                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * 128;
+                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * HC::kIn;
                 const uint32_t slot = t < g ? 1 + t : 1 + g + (t - g);
-                zkh::Xyzz q;
-                memcpy(&q, bs + (uint64_t)slot * 128, 128);
-                acc = H::xadd(acc, q);
-                if (t == 0) {
-                    memcpy(&q, bs, 128);  // U_w
-                    acc = H::xadd(acc, q);
-                }
+                acc = HC::xadd(acc, HC::load(bs + (uint64_t)slot * HC::kIn));
+                if (t == 0) acc = HC::xadd(acc, HC::load(bs));  // U_w
             }
             res[j] = acc;
         }
@@ -289,22 +356,7 @@ static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out96) {
     } else {
         tail(0, nmsm);
     }
-    std::vector<zkh::Fp> xs, ys;
-    H::batch_to_affine(res, xs, ys);
-    for (uint32_t j = 0; j < nmsm; j++) {
-        uint8_t* o = out96 + (uint64_t)j * 96;
-        if (H::is_id(res[j])) {
-            zkh::Fp one = H::one(), zero = zkh::zero();
-            memcpy(o, one.v, 32);
-            memcpy(o + 32, one.v, 32);
-            memcpy(o + 64, zero.v, 32);
-        } else {
-            zkh::Fp one = H::one();
-            memcpy(o, xs[j].v, 32);
-            memcpy(o + 32, ys[j].v, 32);
-            memcpy(o + 64, one.v, 32);
-        }
-    }
+    HC::finish(res, out);
 }
 
 int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
@@ -325,14 +377,15 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_ALLOC(hist_, (uint64_t)pl.tiles * 256 * 4);
     ZK_ALLOC(part_, (uint64_t)pl.chunks * 4);
     ZK_ALLOC(segvalid_, (uint64_t)nseg * 4);
-    ZK_ALLOC(buckets_, pl.buckets * kBucketBytes);
+    const uint64_t ptb = point_bytes(pl.curve), hpb = host_point_bytes(pl.curve);
+    ZK_ALLOC(buckets_, pl.buckets * ptb);
     ZK_ALLOC(recAk_, rec_cap * 4);
-    ZK_ALLOC(recAp_, rec_cap * kBucketBytes);
+    ZK_ALLOC(recAp_, rec_cap * ptb);
     ZK_ALLOC(recBk_, rec_cap * 4);
-    ZK_ALLOC(rc_, pl.groups * kBucketBytes);
-    ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * 128);
+    ZK_ALLOC(rc_, pl.groups * ptb);
+    ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * hpb);
     ZK_ALLOC(err_, 16);
-    ZK_ALLOC(hwin_, (uint64_t)nseg * (16 * 128 + 4) + 16);
+    ZK_ALLOC(hwin_, (uint64_t)nseg * (16 * hpb + 4) + 16);
 
     uint8_t* hb = (uint8_t*)hdesc_.p;
     memcpy(hb + L.msms, pl.msms.data(), pl.msms.size() * sizeof(MsmDesc));
@@ -357,7 +410,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     uint32_t* d_err = (uint32_t*)err_.p;
     hipStream_t st = stream_;
 
-    const uint64_t bits_bytes = (uint64_t)nseg * 16 * 128;
+    const uint64_t bits_bytes = (uint64_t)nseg * 16 * hpb;
     uint8_t* hw = (uint8_t*)hwin_.p;
     // the device part of the call: one launch sequence on the engine stream.
     // Stage events are recorded only outside graph capture.
@@ -376,7 +429,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
                          (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
                          (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
-        launch_fill_zero(buckets_.p, pl.buckets * kBucketBytes, st);
+        launch_fill_zero(buckets_.p, pl.buckets * ptb, st);
         ZK_TRY(mark(2));
         launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
                           (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
@@ -460,8 +513,9 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
 
     // host tail: Horner over windows, then affine normalisation
     auto t_tail0 = std::chrono::steady_clock::now();
-    if (pl.curve == CurveId::Grumpkin) host_tail<zkh::HR>(pl, hw, out96);
-    else host_tail<zkh::HP>(pl, hw, out96);
+    if (pl.curve == CurveId::Grumpkin) host_tail<HostG1<zkh::HR>>(pl, hw, out96);
+    else if (pl.curve == CurveId::Bn254G2) host_tail<HostG2View>(pl, hw, out96);
+    else host_tail<HostG1<zkh::HP>>(pl, hw, out96);
     auto t_end = std::chrono::steady_clock::now();
     timing_.host_tail_ms = std::chrono::duration<float, std::milli>(t_end - t_tail0).count();
     timing_.wall_ms = std::chrono::duration<float, std::milli>(t_end - t_wall0).count();
@@ -552,7 +606,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     uint64_t total = 0;
     for (uint32_t j = 0; j < nmsm; j++) {
         if (counts[j] == 0) {
-            memset(out96 + (uint64_t)j * 96, 0, 96);  // zkMetal_ffi.swift:1260-1264
+            memset(out96 + (uint64_t)j * result_bytes(cv), 0, result_bytes(cv));  // zkMetal_ffi.swift:1260-1264
             continue;
         }
         ns.push_back(counts[j]);
@@ -564,18 +618,19 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                       acc_cap_, cv);
+                       lane_cap(cv), cv);
     if (rc != kOk) return rc;
-    ZK_ALLOC(pts_in_, total * 64);
+    ZK_ALLOC(pts_in_, total * input_point_bytes(cv));
     ZK_ALLOC(ptable_, table_bytes(total, cv));
     ZK_ALLOC(scalars_, total * 32);
-    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * 64, hipMemcpyHostToDevice, stream_));
+    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * input_point_bytes(cv), hipMemcpyHostToDevice, stream_));
     ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
     launch_build_table(cv, pts_in_.p, ptable_.p, total, stream_);
-    std::vector<uint8_t> res(ns.size() * 96);
+    std::vector<uint8_t> res(ns.size() * result_bytes(cv));
     rc = run(pl, scalars_.p, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
-    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    const uint64_t rb = result_bytes(cv);
+    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * rb, res.data() + k * rb, rb);
     return kOk;
 }
 
@@ -603,7 +658,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     uint64_t total = 0;
     for (uint32_t j = 0; j < nmsm; j++) {
         if (counts[j] == 0) {
-            memset(out96 + (uint64_t)j * 96, 0, 96);
+            memset(out96 + (uint64_t)j * result_bytes(cv), 0, result_bytes(cv));
             continue;
         }
         ns.push_back(counts[j]);
@@ -615,14 +670,15 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                       acc_cap_, cv);
+                       lane_cap(cv), cv);
     if (rc != kOk) return rc;
     ZK_ALLOC(ptable_, table_bytes(total, cv));
     launch_build_table(cv, d_points, ptable_.p, total, stream_);
-    std::vector<uint8_t> res(ns.size() * 96);
+    std::vector<uint8_t> res(ns.size() * result_bytes(cv));
     rc = run(pl, d_scalars, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
-    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * 96, res.data() + k * 96, 96);
+    const uint64_t rb = result_bytes(cv);
+    for (size_t k = 0; k < idx.size(); k++) memcpy(out96 + (uint64_t)idx[k] * rb, res.data() + k * rb, rb);
     return kOk;
 }
 
@@ -742,6 +798,69 @@ int32_t Engine::ensure_gtab_grumpkin() {
     using H = zkh::HR;
     const zkh::Fp gy{{0x833fc48d823f272cULL, 0x2d270d45f1181294ULL, 0xcf135e7506a45d63ULL, 0x2ULL}};
     return build_gtab<H>(H::mul(zkh::Fp{{1, 0, 0, 0}}, H::r2()), H::mul(gy, H::r2()), gtab_gk_);
+}
+
+// BN254 G2 generator table: gtab[w*256 + b] = b * 2^(8w) * G2, affine
+// 128-byte points (x.c0, x.c1, y.c0, y.c1 in the 2^256 form; entry 0 all
+// zero), G2 the reference's generator (BN254Pairing.swift:389-396); one
+// inversion for all entries (prefix products)
+int32_t Engine::ensure_gtab_g2() {
+    if (gtab_g2_.p) return kOk;
+    using G = zkh::HostG2;
+    using H = zkh::HP;
+    auto mont = [](zkh::Fp v) { return H::mul(v, H::r2()); };
+    const zkh::Fp2 gx{mont({{0x46debd5cd992f6edULL, 0x674322d4f75edaddULL, 0x426a00665e5c4479ULL, 0x1800deef121f1e76ULL}}),
+                      mont({{0x97e485b7aef312c2ULL, 0xf1aa493335a9e712ULL, 0x7260bfb731fb5d25ULL, 0x198e9393920d483aULL}})};
+    const zkh::Fp2 gy{mont({{0x4ce6cc0166fa7daaULL, 0xe3d1e7690c43d37bULL, 0x4aab71808dcb408fULL, 0x12c85ea5db8c6debULL}}),
+                      mont({{0x55acdadcd122975bULL, 0xbc4b313370b38ef3ULL, 0xec9e99ad690c3395ULL, 0x090689d0585ff075ULL}})};
+    std::vector<zkh::Xyzz2> pts(32 * 256);
+    zkh::Xyzz2 base{gx, gy, G::one2(), G::one2()};
+    for (int w = 0; w < 32; w++) {
+        zkh::Xyzz2 acc = G::identity();
+        pts[w * 256] = acc;
+        for (int b = 1; b < 256; b++) {
+            acc = G::xadd(acc, base);
+            pts[w * 256 + b] = acc;
+        }
+        base = G::xadd(acc, base);  // 256 * base
+    }
+    // d_i = ZZ_i ZZZ_i (1 for the identity); 1/ZZ = ZZZ/d, 1/ZZZ = ZZ/d
+    const size_t n = pts.size();
+    std::vector<zkh::Fp2> pre(n + 1);
+    pre[0] = G::one2();
+    for (size_t i = 0; i < n; i++)
+        pre[i + 1] = G::is_id(pts[i]) ? pre[i] : G::mul2(pre[i], G::mul2(pts[i].ZZ, pts[i].ZZZ));
+    zkh::Fp2 acc = G::inv2(pre[n]);
+    std::vector<uint8_t> blob(n * 128, 0);
+    for (size_t i = n; i-- > 0;) {
+        if (G::is_id(pts[i])) continue;
+        const zkh::Fp2 dinv = G::mul2(acc, pre[i]);
+        acc = G::mul2(acc, G::mul2(pts[i].ZZ, pts[i].ZZZ));
+        const zkh::Fp2 x = G::mul2(pts[i].X, G::mul2(dinv, pts[i].ZZZ));
+        const zkh::Fp2 y = G::mul2(pts[i].Y, G::mul2(dinv, pts[i].ZZ));
+        const zkh::Fp* c[4] = {&x.c0, &x.c1, &y.c0, &y.c1};
+        for (int k = 0; k < 4; k++) memcpy(&blob[i * 128 + 32 * k], c[k]->v, 32);
+    }
+    DevBuf tmp;
+    if (tmp.ensure(blob.size()) != hipSuccess) return kAllocFailed;
+    ZK_TRY(hipMemcpy(tmp.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    std::swap(gtab_g2_.p, tmp.p);
+    std::swap(gtab_g2_.cap, tmp.cap);
+    return kOk;
+}
+
+int32_t Engine::g2_points_generate(uint64_t seed, uint32_t n, void* d_out) {
+    if (n == 0 || !d_out) return kInvalidInput;
+    int rc = ensure_gtab_g2();
+    if (rc != kOk) return rc;
+    std::vector<uint8_t> ks((uint64_t)n * 32);
+    zkh::gen_scalars(seed, n, 0, ks.data());
+    ZK_ALLOC(tmp_, (uint64_t)n * 32);
+    ZK_TRY(hipMemcpyAsync(tmp_.p, ks.data(), ks.size(), hipMemcpyHostToDevice, stream_));
+    launch_gen_points_g2(tmp_.p, gtab_g2_.p, n, d_out, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
+    return kOk;
 }
 
 int32_t Engine::grumpkin_points_generate(uint64_t seed, uint32_t n, void* d_out) {
@@ -962,6 +1081,21 @@ int32_t Engine::test_xyzz(int op, const uint8_t* p, const uint8_t* q, uint8_t* o
     launch_test_xyzz(op, dp.p, dq.p, dout.p, n, stream_);
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipMemcpy(out, dout.p, (uint64_t)n * 128, hipMemcpyDeviceToHost));
+    return kOk;
+}
+
+int32_t Engine::test_g2(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n) {
+    if (!p || !q || !out || n == 0 || op < 0 || op > 6) return kInvalidInput;
+    DevBuf dp, dq, dout;
+    const uint64_t bytes = (uint64_t)n * 256;
+    ZK_ALLOC(dp, bytes);
+    ZK_ALLOC(dq, bytes);
+    ZK_ALLOC(dout, bytes);
+    ZK_TRY(hipMemcpy(dp.p, p, bytes, hipMemcpyHostToDevice));
+    ZK_TRY(hipMemcpy(dq.p, q, bytes, hipMemcpyHostToDevice));
+    launch_test_g2(op, dp.p, dq.p, dout.p, n, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipMemcpy(out, dout.p, bytes, hipMemcpyDeviceToHost));
     return kOk;
 }
 

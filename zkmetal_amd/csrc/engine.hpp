@@ -112,6 +112,8 @@ class Engine {
     // n Grumpkin points k_i * G (k_i = gen_scalars(seed, n, 0)) into a device
     // buffer of n * 64 bytes
     int32_t grumpkin_points_generate(uint64_t seed, uint32_t n, void* d_out);
+    // n BN254 G2 points k_i * G2 (128 B each) into a device buffer
+    int32_t g2_points_generate(uint64_t seed, uint32_t n, void* d_out);
 
     // device-resident base table (SRS)
     int32_t srs_upload(const uint8_t* points, uint32_t n);
@@ -172,6 +174,7 @@ class Engine {
     // primitive tests (host in / host out)
     int32_t test_fp(int op, const uint8_t* a, const uint8_t* b, uint8_t* out, uint32_t n);
     int32_t test_xyzz(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n);
+    int32_t test_g2(int op, const uint8_t* p, const uint8_t* q, uint8_t* out, uint32_t n);
     int32_t test_glv(const uint8_t* scalars, uint8_t* out, uint32_t n);
     // measured: [0] v_mad_u64_u32 products/s, [1] FIPS accumulate steps/s,
     // [2] field multiplies/s (all at full occupancy); [3] ns per dependent
@@ -188,6 +191,7 @@ class Engine {
     Engine() = default;
     int32_t ensure_gtab();
     int32_t ensure_gtab_grumpkin();
+    int32_t ensure_gtab_g2();
 
     int dev_ = 0;
     hipStream_t stream_ = nullptr;
@@ -197,6 +201,8 @@ class Engine {
     uint32_t ctab_batch_[33] = {};
     bool graph_mode_ = false;
     uint64_t acc_cap_ = 1u << 17;  // resident k_accumulate lanes on this device
+    uint64_t acc_cap_g2_ = 1u << 16;  // the same for the G2 accumulation
+    uint64_t lane_cap(CurveId cv) const { return cv == CurveId::Bn254G2 ? acc_cap_g2_ : acc_cap_; }
     hipGraphExec_t graph_exec_ = nullptr;
     std::vector<uint64_t> graph_key_;
     uint64_t graph_captures_ = 0;
@@ -204,7 +210,7 @@ class Engine {
 
     DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;
     DevBuf recAk_, recAp_, recBk_, rc_, winsum_, err_;  // recBk_: long-run list
-    DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, gtab_gk_, tmp_;
+    DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, gtab_gk_, gtab_g2_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
     // host image of the resident bases (64 B each) for the prefix check:

@@ -225,4 +225,80 @@ struct HostArith {
 
 using HP = HostArith<ModP>;  // BN254 G1
 using HR = HostArith<ModR>;  // Grumpkin
+
+// BN254 G2: XYZZ over Fp2 = Fp[u]/(u^2 + 1), components in the 2^256
+// Montgomery form of p (Sources/zkMetal/Curve/BN254Pairing.swift:11-80)
+struct Fp2 {
+    Fp c0, c1;
+};
+
+struct Xyzz2 {
+    Fp2 X, Y, ZZ, ZZZ;
+};
+
+struct HostG2 {
+    using H = HP;
+    static Fp2 zero2() { return Fp2{zero(), zero()}; }
+    static Fp2 one2() { return Fp2{H::one(), zero()}; }
+    static bool is_zero2(const Fp2& a) { return is_zero(a.c0) && is_zero(a.c1); }
+    static Fp2 add2(const Fp2& a, const Fp2& b) { return Fp2{H::add(a.c0, b.c0), H::add(a.c1, b.c1)}; }
+    static Fp2 sub2(const Fp2& a, const Fp2& b) { return Fp2{H::sub(a.c0, b.c0), H::sub(a.c1, b.c1)}; }
+    static Fp2 dbl2(const Fp2& a) { return add2(a, a); }
+    static Fp2 mul2(const Fp2& a, const Fp2& b) {
+        const Fp t0 = H::mul(a.c0, b.c0), t1 = H::mul(a.c1, b.c1);
+        const Fp t2 = H::mul(H::add(a.c0, a.c1), H::add(b.c0, b.c1));
+        return Fp2{H::sub(t0, t1), H::sub(t2, H::add(t0, t1))};
+    }
+    // conj(a) / (a0^2 + a1^2)
+    static Fp2 inv2(const Fp2& a) {
+        const Fp ni = H::inv(H::add(H::mul(a.c0, a.c0), H::mul(a.c1, a.c1)));
+        return Fp2{H::mul(a.c0, ni), H::sub(zero(), H::mul(a.c1, ni))};
+    }
+
+    static Xyzz2 identity() { return Xyzz2{zero2(), zero2(), zero2(), zero2()}; }
+    static bool is_id(const Xyzz2& p) { return is_zero2(p.ZZ); }
+
+    static Xyzz2 xdbl(const Xyzz2& p) {
+        if (is_id(p)) return p;
+        const Fp2 U = dbl2(p.Y), V = mul2(U, U), W = mul2(U, V), S = mul2(p.X, V);
+        const Fp2 xx = mul2(p.X, p.X), M = add2(dbl2(xx), xx);
+        Xyzz2 r;
+        r.X = sub2(mul2(M, M), dbl2(S));
+        r.Y = sub2(mul2(M, sub2(S, r.X)), mul2(W, p.Y));
+        r.ZZ = mul2(V, p.ZZ);
+        r.ZZZ = mul2(W, p.ZZZ);
+        return r;
+    }
+
+    static Xyzz2 xadd(const Xyzz2& a, const Xyzz2& b) {
+        if (is_id(a)) return b;
+        if (is_id(b)) return a;
+        const Fp2 U1 = mul2(a.X, b.ZZ), U2 = mul2(b.X, a.ZZ);
+        const Fp2 S1 = mul2(a.Y, b.ZZZ), S2 = mul2(b.Y, a.ZZZ);
+        const Fp2 P = sub2(U2, U1), R = sub2(S2, S1);
+        if (is_zero2(P)) {
+            if (is_zero2(R)) return xdbl(a);
+            return identity();
+        }
+        const Fp2 PP = mul2(P, P), PPP = mul2(P, PP), Q = mul2(U1, PP);
+        Xyzz2 r;
+        r.X = sub2(sub2(mul2(R, R), PPP), dbl2(Q));
+        r.Y = sub2(mul2(R, sub2(Q, r.X)), mul2(S1, PPP));
+        r.ZZ = mul2(mul2(a.ZZ, b.ZZ), PP);
+        r.ZZZ = mul2(mul2(a.ZZZ, b.ZZZ), PPP);
+        return r;
+    }
+
+    // (x, y); the identity gives (0, 0)
+    static void to_affine(const Xyzz2& p, Fp2& x, Fp2& y) {
+        if (is_id(p)) {
+            x = zero2();
+            y = zero2();
+            return;
+        }
+        const Fp2 t = inv2(mul2(p.ZZ, p.ZZZ));
+        x = mul2(p.X, mul2(t, p.ZZZ));
+        y = mul2(p.Y, mul2(t, p.ZZ));
+    }
+};
 }  // namespace zkh

@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "curve29.hpp"
+#include "curve_g2.hpp"
 #include "glv.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
@@ -101,6 +102,55 @@ __device__ __forceinline__ QXyzz shfl_down_qxyzz(const QXyzz& a, uint32_t d) {
     return r;
 }
 
+// Curve policies of the bucket kernels (accumulation, merge, reduction): the
+// stored point type and its loads / stores, the affine table point, and the
+// additions.  G1Curve<F>: XYZZ over one field (BN254 G1 over Fp, Grumpkin
+// over Fr), one 128-byte table slot per entry, 144-byte points.  G2Curve:
+// XYZZ over Fp2 (BN254 G2), two slots per entry, two 144-byte halves per
+// point.  kSlots: table slots per entry.
+template <class F>
+struct G1Curve {
+    using Pt = QXyzz;
+    using Af = QAff;
+    static constexpr uint32_t kSlots = 1;
+    static constexpr uint64_t kPtBytes = kBucketBytes;
+    static __device__ __forceinline__ Pt identity() { return qxyzz_identity(); }
+    static __device__ __forceinline__ Pt run_start() { return Arith<F>::qxyzz_run_start(); }
+    static __device__ __forceinline__ void madd_run(Pt& a, bool& id, const Af& q) {
+        Arith<F>::qxyzz_madd_run(a, id, q.x, q.y);
+    }
+    static __device__ __forceinline__ void add(Pt& a, const Pt& b) { Arith<F>::qxyzz_add(a, b); }
+    static __device__ __forceinline__ Pt shfl_down(const Pt& a, uint32_t d) { return shfl_down_qxyzz(a, d); }
+    static __device__ __forceinline__ Af load_tab(const uint4* t, uint64_t e, bool neg) { return load_qtab(t, e, neg); }
+    static __device__ __forceinline__ Pt load(const uint4* b, uint64_t i) { return load_qxyzz(b, i); }
+    static __device__ __forceinline__ void store(uint4* b, uint64_t i, const Pt& p) { store_qxyzz(b, i, p); }
+    static __device__ __forceinline__ void store_host(uint4* b, uint64_t i, const Pt& p) {
+        store_xyzz(b, i, Arith<F>::qxyzz_to_xyzz(p));
+    }
+};
+
+struct G2Curve {
+    using Pt = QXyzz2;
+    using Af = QAff2;
+    static constexpr uint32_t kSlots = 2;
+    static constexpr uint64_t kPtBytes = 2 * kBucketBytes;
+    static __device__ __forceinline__ Pt identity() { return G2Arith::identity(); }
+    static __device__ __forceinline__ Pt run_start() { return G2Arith::run_start(); }
+    static __device__ __forceinline__ void madd_run(Pt& a, bool& id, const Af& q) {
+        G2Arith::madd_run(a, id, q.x, q.y);
+    }
+    static __device__ __forceinline__ void add(Pt& a, const Pt& b) { G2Arith::add(a, b); }
+    static __device__ __forceinline__ Pt shfl_down(const Pt& a, uint32_t d) {
+        const QXyzz x = shfl_down_qxyzz(QXyzz{a.X.c0, a.Y.c0, a.ZZ.c0, a.ZZZ.c0}, d);
+        const QXyzz y = shfl_down_qxyzz(QXyzz{a.X.c1, a.Y.c1, a.ZZ.c1, a.ZZZ.c1}, d);
+        return Pt{Fq2{x.X, y.X}, Fq2{x.Y, y.Y}, Fq2{x.ZZ, y.ZZ}, Fq2{x.ZZZ, y.ZZZ}};
+    }
+    static __device__ __forceinline__ Af load_tab(const uint4* t, uint64_t e, bool neg) { return load_qtab2(t, e, neg); }
+    static __device__ __forceinline__ Pt load(const uint4* b, uint64_t i) { return load_qxyzz2(b, i); }
+    static __device__ __forceinline__ void store(uint4* b, uint64_t i, const Pt& p) { store_qxyzz2(b, i, p); }
+    static __device__ __forceinline__ void store_host(uint4* b, uint64_t i, const Pt& p) { store_host_xyzz2(b, i, p); }
+};
+
 // Lane-sum then group tree with ONE addition site: steps 0..nload-1 add the
 // lane's own elements (load(e) returns the e-th, or the identity), the next
 // log2(P) steps add the partner's partial (shuffle tree over aligned groups of
@@ -112,36 +162,25 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-template <class F, class Load>
-__device__ __forceinline__ QXyzz sum_then_tree(uint32_t nload, uint32_t P, Load load) {
+template <class C, class Load>
+__device__ __forceinline__ typename C::Pt sum_then_tree(uint32_t nload, uint32_t P, Load load) {
     const uint32_t lane = __lane_id();
     uint32_t levels = 0;
     while ((1u << levels) < P) levels++;
-    QXyzz acc = qxyzz_identity();
+    typename C::Pt acc = C::identity();
 #pragma unroll 1
     for (uint32_t st = 0; st < nload + levels; st++) {
-        QXyzz q;
+        typename C::Pt q;
         if (st < nload) {
             q = load(st);
         } else {
             const uint32_t s = P >> (st - nload + 1);
-            q = shfl_down_qxyzz(acc, s);
-            if ((lane & (P - 1)) >= s) q = qxyzz_identity();
+            q = C::shfl_down(acc, s);
+            if ((lane & (P - 1)) >= s) q = C::identity();
         }
-        Arith<F>::qxyzz_add(acc, q);
+        C::add(acc, q);
     }
     return acc;
-}
-
-template <class F>
-__device__ __forceinline__ QXyzz group_sum(QXyzz v, uint32_t P) {
-    const uint32_t lane = __lane_id();
-#pragma unroll 1
-    for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
-        QXyzz o = shfl_down_qxyzz(v, s);
-        if ((lane & (P - 1)) < s) Arith<F>::qxyzz_add(v, o);
-    }
-    return v;
 }
 
 // -------------------------------------------------------- base table / phi
@@ -169,6 +208,20 @@ __device__ __forceinline__ void store_table_pair(uint4* table, uint64_t* idmask,
     Ap::store_qtab(table, 2 * i, q);
     q.x = Ap::fq_csub<1>(Ap::fq_mul(Ap::fq_beta(), q.x));
     Ap::store_qtab(table, 2 * i + 1, q);
+}
+
+// BN254 G2 table: point i (x.c0, x.c1, y.c0, y.c1 in the 2^256 form, 128 B)
+// -> slots 2i (the c0 parts) and 2i + 1 (the c1 parts), canonical; an
+// all-zero point is the identity (bitmap bit).
+__global__ __launch_bounds__(256) void k_build_table_g2(const uint4* __restrict__ pts,
+                                                        uint4* __restrict__ table,
+                                                        uint64_t* __restrict__ idmask, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4* p = pts + i * 8;
+    const QAff2 q{G2Arith::from_fp2(load_fp(p), load_fp(p + 2)), G2Arith::from_fp2(load_fp(p + 4), load_fp(p + 6))};
+    store_id_bit(idmask, i, qaff_is_id(QAff{q.x.c0, q.y.c0}) && qaff_is_id(QAff{q.x.c1, q.y.c1}));
+    store_qtab2(table, i, q);
 }
 
 // SRS import from an SRS file's G1 section (Sources/zkMetal/KZG/
@@ -295,6 +348,21 @@ __device__ __forceinline__ void reduce_mod_p(uint32_t k[8]) {
     }
 }
 
+// BN254 G2's group order is r: scalars are reduced mod r
+// (BN254G2MSMEngine.swift:214-232) and recoded like Grumpkin's.
+__device__ __forceinline__ void reduce_mod_r(uint32_t k[8]) {
+#pragma unroll 1
+    for (int it = 0; it < 6; it++) {
+        uint32_t d[8];
+        unsigned br = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) d[i] = __builtin_subc(k[i], r_limb(i), br, &br);
+        if (br) break;
+#pragma unroll
+        for (int i = 0; i < 8; i++) k[i] = d[i];
+    }
+}
+
 // c-bit window w of a 256-bit value, from 64-bit words by shifts (w is the
 // same in every lane, so the word choice is a uniform branch, no indexing)
 __device__ __forceinline__ uint32_t window_bits256(const uint32_t k[8], uint32_t w, uint32_t c) {
@@ -316,12 +384,13 @@ __device__ __forceinline__ uint32_t window_bits256(const uint32_t k[8], uint32_t
 // counts the tile's 256-bin histogram of key byte 0 for each window (LDS
 // atomics) and writes it where the scan expects it; the first pass then
 // needs no histogram kernel.  EPS = 2: BN254 G1, GLV halves k1 (entry 2i,
-// base P_i) and k2 (entry 2i + 1, phi(P_i)); EPS = 1: Grumpkin, one entry per
-// scalar over 255 bits.
+// base P_i) and k2 (entry 2i + 1, phi(P_i)); EPS = 1: one entry per scalar
+// over 255 bits, the scalar reduced mod r (kModR, BN254 G2) or mod p
+// (Grumpkin).
 constexpr int kDecThreads = 256;
 static_assert(kSortTile <= (1 << 15) && kSortTile / 2 / kDecThreads >= 1, "decomposition tiling");
 
-template <int EPS>
+template <int EPS, bool kModR>
 __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __restrict__ msms,
                                                           const uint64_t* __restrict__ msm_dtile0,
                                                           uint32_t nmsm,
@@ -356,7 +425,8 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
             for (int q = 0; q < 8; q++) k[q] = 0;  // identity base: no digits
         }
         if constexpr (EPS == 1) {
-            reduce_mod_p(k);
+            if constexpr (kModR) reduce_mod_r(k);
+            else reduce_mod_p(k);
             uint32_t carry = 0;
             for (uint32_t w = 0; w < m.nwin; w++) {
                 uint32_t d = window_bits256(k, w, c) + carry, neg = 0;
@@ -674,13 +744,14 @@ __global__ __launch_bounds__(256) void k_fill_zero(uint4* __restrict__ p, uint64
 // to the bucket array, the (at most two) runs that cross the range ends go to
 // merge records 2L (first run) and 2L+1 (last run).
 
+template <class C>
 __device__ __forceinline__ void put_record(uint32_t* keys, uint4* pts, uint64_t slot,
-                                           uint32_t key, bool valid, const QXyzz& v) {
+                                           uint32_t key, bool valid, const typename C::Pt& v) {
     keys[slot] = key | (valid ? 0u : ZK_REC_INVALID);
-    if (valid) store_qxyzz(pts, slot, v);
+    if (valid) C::store(pts, slot, v);
 }
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_accumulate(
     const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_lane0, uint32_t nseg,
     const MsmDesc* __restrict__ msms, const uint32_t* __restrict__ seg_valid,
@@ -700,37 +771,37 @@ __global__ __launch_bounds__(256) void k_accumulate(
         rec_keys[2 * L + 1] = ZK_REC_INVALID | ZK_REC_KEYMASK;
         return;
     }
-    const uint4* tb = table + msms[sg.msm].table0 * (kTableEntryBytes / 16);
+    const uint4* tb = table + msms[sg.msm].table0 * C::kSlots * (kTableEntryBytes / 16);
     uint32_t cur = keys[b];
     const bool head_open = (b > sg.ent0) && keys[b - 1] == cur;
     bool first = true;
-    QXyzz acc = Arith<F>::qxyzz_run_start();
+    typename C::Pt acc = C::run_start();
     bool acc_id = true;
     // software pipeline: the next entry's key, value and base are loaded
     // before the current mixed addition, hiding the random 128-byte gather
     uint32_t k = cur, v = vals[b];
-    QAff q = load_qtab(tb, v & 0x7fffffffu, v >> 31);
+    typename C::Af q = C::load_tab(tb, v & 0x7fffffffu, v >> 31);
     for (uint64_t p = b; p < e; p++) {
         const uint64_t pn = min(p + 1, e - 1);
         const uint32_t kn = keys[pn], vn = vals[pn];
-        const QAff qn = load_qtab(tb, vn & 0x7fffffffu, vn >> 31);
+        const typename C::Af qn = C::load_tab(tb, vn & 0x7fffffffu, vn >> 31);
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
             const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
             if (first && head_open) {
-                put_record(rec_keys, rec_pts, 2 * L, gk, true, acc);
+                put_record<C>(rec_keys, rec_pts, 2 * L, gk, true, acc);
             } else {
-                store_qxyzz(buckets, gk, acc);
-                if (first) put_record(rec_keys, rec_pts, 2 * L, gk, false, acc);
+                C::store(buckets, gk, acc);
+                if (first) put_record<C>(rec_keys, rec_pts, 2 * L, gk, false, acc);
             }
             first = false;
-            acc = Arith<F>::qxyzz_run_start();
+            acc = C::run_start();
             acc_id = true;
             cur = k;
         }
         // y is already negated for a negative digit; identity bases have no
         // digits (table bitmap), so q is never the identity
-        Arith<F>::qxyzz_madd_run(acc, acc_id, q.x, q.y);
+        C::madd_run(acc, acc_id, q);
         k = kn;
         v = vn;
         q = qn;
@@ -739,12 +810,12 @@ __global__ __launch_bounds__(256) void k_accumulate(
     const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
     if (first) {
         const bool open = head_open || tail_open;
-        if (!open) store_qxyzz(buckets, gk, acc);
-        put_record(rec_keys, rec_pts, 2 * L, gk, open, acc);
-        put_record(rec_keys, rec_pts, 2 * L + 1, gk, false, acc);
+        if (!open) C::store(buckets, gk, acc);
+        put_record<C>(rec_keys, rec_pts, 2 * L, gk, open, acc);
+        put_record<C>(rec_keys, rec_pts, 2 * L + 1, gk, false, acc);
     } else {
-        if (!tail_open) store_qxyzz(buckets, gk, acc);
-        put_record(rec_keys, rec_pts, 2 * L + 1, gk, tail_open, acc);
+        if (!tail_open) C::store(buckets, gk, acc);
+        put_record<C>(rec_keys, rec_pts, 2 * L + 1, gk, tail_open, acc);
     }
 }
 
@@ -759,7 +830,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
 // tree.  k_merge_long: one 256-lane workgroup per listed run, a shuffle tree
 // per wave, then one lane adds the four wave partials: depth len/256 + 9.
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     uint32_t limit, uint4* __restrict__ dest,
@@ -793,20 +864,20 @@ __global__ __launch_bounds__(256) void k_merge_runs(const uint32_t* __restrict__
             continue;
         }
         bool has = false;
-        QXyzz acc = qxyzz_identity();
+        typename C::Pt acc = C::identity();
         for (uint64_t p = i; p < j; p++) {
             if (!(keys[p] & ZK_REC_INVALID)) {
-                QXyzz q = load_qxyzz(pts, p);
-                if (has) Arith<F>::qxyzz_add(acc, q);
+                typename C::Pt q = C::load(pts, p);
+                if (has) C::add(acc, q);
                 else acc = q;
                 has = true;
             }
         }
-        if (has) store_qxyzz(dest, k, acc);
+        if (has) C::store(dest, k, acc);
     }
 }
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ keys,
                                                    const uint4* __restrict__ pts, uint64_t nrec,
                                                    const uint32_t* __restrict__ mid_count,
@@ -831,23 +902,23 @@ __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ 
                 has |= !(kr & ZK_REC_INVALID);
             }
         }
-        const QXyzz acc = sum_then_tree<F>(wave_max(cnt), 8, [&](uint32_t e) {
+        const typename C::Pt acc = sum_then_tree<C>(wave_max(cnt), 8, [&](uint32_t e) {
             const uint64_t p = start + sub + 8ull * e;
-            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
+            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? C::load(pts, p) : C::identity();
         });
         // the group's first lane stores if any lane of the group had a payload
         const uint64_t hb = __ballot(has);
-        if (run < n && sub == 0 && ((hb >> (lane & ~7u)) & 0xffull)) store_qxyzz(dest, k, acc);
+        if (run < n && sub == 0 && ((hb >> (lane & ~7u)) & 0xffull)) C::store(dest, k, acc);
     }
 }
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__ keys,
                                                     const uint4* __restrict__ pts, uint64_t nrec,
                                                     const uint32_t* __restrict__ long_count,
                                                     const uint32_t* __restrict__ long_list,
                                                     uint32_t long_cap, uint4* __restrict__ dest) {
-    __shared__ uint4 part[4][kBucketBytes / 16];
+    __shared__ uint4 part[4][C::kPtBytes / 16];
     __shared__ uint32_t part_has[4];
     const uint32_t n = min(*long_count, long_cap);
     const uint32_t t = threadIdx.x, wv = t >> 6, lane = __lane_id();
@@ -862,23 +933,23 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
             cnt++;
             has |= !(kr & ZK_REC_INVALID);
         }
-        QXyzz acc = sum_then_tree<F>(wave_max(cnt), 64, [&](uint32_t e) {
+        typename C::Pt acc = sum_then_tree<C>(wave_max(cnt), 64, [&](uint32_t e) {
             const uint64_t p = start + t + 256ull * e;
-            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? load_qxyzz(pts, p) : qxyzz_identity();
+            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? C::load(pts, p) : C::identity();
         });
         const bool wave_has = __ballot(has) != 0;
         if (lane == 0) {
-            store_qxyzz(&part[wv][0], 0, acc);
+            C::store(&part[wv][0], 0, acc);
             part_has[wv] = wave_has ? 1u : 0u;
         }
         __syncthreads();
         if (t == 0) {
 #pragma unroll 1
             for (uint32_t w = 1; w < 4; w++) {
-                const QXyzz o = load_qxyzz(&part[w][0], 0);
-                Arith<F>::qxyzz_add(acc, o);
+                const typename C::Pt o = C::load(&part[w][0], 0);
+                C::add(acc, o);
             }
-            if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) store_qxyzz(dest, k, acc);
+            if (part_has[0] | part_has[1] | part_has[2] | part_has[3]) C::store(dest, k, acc);
         }
         __syncthreads();  // LDS partials are reused by the next run
     }
@@ -906,7 +977,7 @@ __device__ __forceinline__ void seg_shape(uint32_t nbk, uint32_t& g, uint32_t& h
 
 __device__ __forceinline__ uint32_t lanes_per(uint32_t len, uint32_t per) { return len >= per ? len / per : 1; }
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
                                                    const uint64_t* __restrict__ seg_wave0,
                                                    const uint64_t* __restrict__ seg_rc0,
@@ -932,12 +1003,12 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     const uint32_t nitem = rows ? nrow : ncol;
     const uint32_t len = rows ? ncol : nrow;  // buckets per item
     const uint32_t cnt = len / P;  // buckets this lane sums
-    const QXyzz acc = sum_then_tree<F>(cnt, P, [&](uint32_t e) {
+    const typename C::Pt acc = sum_then_tree<C>(cnt, P, [&](uint32_t e) {
         const uint32_t x = part * cnt + e;
         const uint64_t kk = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
-        return item < nitem ? load_qxyzz(buckets, sg.bkt0 + kk) : qxyzz_identity();
+        return item < nitem ? C::load(buckets, sg.bkt0 + kk) : C::identity();
     });
-    if (item < nitem && part == 0) store_qxyzz(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
+    if (item < nitem && part == 0) C::store(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
 }
 
 // index of the i-th value with bit b set: insert a 1 at bit position b
@@ -945,7 +1016,7 @@ __device__ __forceinline__ uint32_t with_bit(uint32_t i, uint32_t b) {
     return ((i >> b) << (b + 1)) | (1u << b) | (i & ((1u << b) - 1u));
 }
 
-template <class F>
+template <class C>
 __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
                                                  const uint64_t* __restrict__ seg_rc0,
                                                  uint32_t nseg, const uint4* __restrict__ rc,
@@ -972,11 +1043,11 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
     }
     // count is a power of 2: lanes take elements lane, lane+64, ...
     const uint32_t P = count >= 64 ? 64u : count;
-    const QXyzz acc = sum_then_tree<F>((count + 63) / 64, P, [&](uint32_t e) {
+    const typename C::Pt acc = sum_then_tree<C>((count + 63) / 64, P, [&](uint32_t e) {
         const uint32_t i = lane + 64 * e;
-        return i < count ? load_qxyzz(rc, base + (masked ? with_bit(i, b) : i)) : qxyzz_identity();
+        return i < count ? C::load(rc, base + (masked ? with_bit(i, b) : i)) : C::identity();
     });
-    if (lane == 0) store_xyzz(bits, item, Arith<F>::qxyzz_to_xyzz(acc));  // host form
+    if (lane == 0) C::store_host(bits, item, acc);  // host form
 }
 
 // ------------------------------------------------ synthetic base table
@@ -1043,6 +1114,69 @@ __global__ __launch_bounds__(256) void k_gen_points(const uint4* __restrict__ ks
         a.y = A::fq_to_fp(A::fq_mul(acc.Y, A::fq_mul(t, acc.ZZ)));
     }
     store_aff(out, i, a);
+}
+
+// BN254 G2 points out[i] = k_i * G2 (128 B: x.c0, x.c1, y.c0, y.c1 in the
+// 2^256 form; all zero for k_i = 0) from a host table gtab[w*256 + b] =
+// b * 2^(8w) * G2 in the same form (entry 0 all zero): synthetic G2 bases.
+__global__ __launch_bounds__(256) void k_gen_points_g2(const uint4* __restrict__ ks,
+                                                       const uint4* __restrict__ gtab, uint64_t n,
+                                                       uint4* __restrict__ out) {
+    using G = G2Arith;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 s0 = ks[2 * i], s1 = ks[2 * i + 1];
+    const uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    QXyzz2 acc = G::identity();
+    for (int w = 0; w < 32; w++) {
+        const uint32_t byte = (k[w >> 2] >> ((w & 3) * 8)) & 0xffu;
+        if (byte) {
+            const uint4* q = gtab + ((uint64_t)w * 256 + byte) * 8;
+            G::madd(acc, G::from_fp2(load_fp(q), load_fp(q + 2)), G::from_fp2(load_fp(q + 4), load_fp(q + 6)), false);
+        }
+    }
+    uint4* o = out + i * 8;
+    if (G::is_id(acc)) {
+        for (int j = 0; j < 8; j++) o[j] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    // 1/ZZ = ZZZ t and 1/ZZZ = ZZ t with t = 1/(ZZ ZZZ)
+    const Fq2 t = G::inv2(G::mul2(acc.ZZ, acc.ZZZ));
+    const Fq2 x = G::mul2(acc.X, G::mul2(t, acc.ZZZ));
+    const Fq2 y = G::mul2(acc.Y, G::mul2(t, acc.ZZ));
+    store_fp(o, G::to_fp(x.c0));
+    store_fp(o + 2, G::to_fp(x.c1));
+    store_fp(o + 4, G::to_fp(y.c0));
+    store_fp(o + 6, G::to_fp(y.c1));
+}
+
+// BN254 G2 primitives on host-form XYZZ points (256 B: c0 record, c1 record).
+// op 0: P + Q, 1: 2P, 2: P + affine (Q.X, Q.Y) (all-zero = identity), 3: the
+// run addition P + (Q.X, Q.Y) from a run start (acc_id) or not; field ops on
+// the X coordinate: 4: X_P * X_Q, 5: X_P^2, 6: 1 / X_P (result in X, the rest
+// zero).
+__global__ __launch_bounds__(64) void k_test_g2(int op, const uint4* __restrict__ p,
+                                                const uint4* __restrict__ q, uint4* __restrict__ out,
+                                                uint64_t n) {
+    using G = G2Arith;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    QXyzz2 a = load_host_xyzz2(p, i), b = load_host_xyzz2(q, i);
+    const bool b_aff_id = G::is_zero2(b.X) && G::is_zero2(b.Y);
+    if (op == 0) G::add(a, b);
+    else if (op == 1) a = G::dbl(a);
+    else if (op == 2) G::madd(a, b.X, b.Y, b_aff_id);
+    else if (op == 3) {
+        bool id = G::is_id(a);
+        if (id) a = G::run_start();
+        G::madd_run(a, id, b.X, b.Y);
+        if (id) a = G::identity();
+    } else {
+        const Fq2 r = op == 4 ? G::mul2(a.X, b.X) : op == 5 ? G::sqr2(a.X) : G::inv2(a.X);
+        a = G::identity();
+        a.X = r;
+    }
+    store_host_xyzz2(out, i, a);
 }
 
 // ------------------------------------------------------ test entry points
@@ -1264,7 +1398,7 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) {
 
 static inline uint64_t* table_idmask(void* table, uint64_t n, CurveId cv) {
     return reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(table) +
-                                       n * entries_per_point(cv) * kTableEntryBytes);
+                                       n * table_slots_per_point(cv) * kTableEntryBytes);
 }
 
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st) {
@@ -1287,7 +1421,10 @@ void launch_table_export(CurveId cv, const void* table, uint64_t n, void* pts, h
 void launch_build_table(CurveId cv, const void* pts, void* table, uint64_t n, hipStream_t st) {
     if (!n) return;
     uint64_t* idm = table_idmask(table, n, cv);
-    if (cv == CurveId::Grumpkin)
+    if (cv == CurveId::Bn254G2)
+        hipLaunchKernelGGL(k_build_table_g2, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const uint4*)pts, (uint4*)table, idm, n);
+    else if (cv == CurveId::Grumpkin)
         hipLaunchKernelGGL((k_build_table<Bn254Fr, 1>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
                            (const uint4*)pts, (uint4*)table, idm, n);
     else
@@ -1302,10 +1439,13 @@ void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile
     if (!dtiles) return;
     const uint64_t* idm = table_idmask(const_cast<void*>(table), table_n, cv);
     if (cv == CurveId::Grumpkin)
-        hipLaunchKernelGGL(k_decompose<1>, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+        hipLaunchKernelGGL((k_decompose<1, false>), dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+                           nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
+    else if (cv == CurveId::Bn254G2)
+        hipLaunchKernelGGL((k_decompose<1, true>), dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
                            nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
     else
-        hipLaunchKernelGGL(k_decompose<2>, dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
+        hipLaunchKernelGGL((k_decompose<2, false>), dim3(dtiles), dim3(kDecThreads), 0, st, msms, msm_dtile0,
                            nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
 }
 
@@ -1336,11 +1476,13 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, (uint4*)p, n16);
 }
 
-uint64_t accumulate_capacity(int device) {
+uint64_t accumulate_capacity(int device, CurveId cv) {
     int cus = 0, blocks = 0;
+    const hipError_t eo = cv == CurveId::Bn254G2
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate<G2Curve>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate<G1Curve<Bn254Fp>>, 256, 0);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_accumulate<Bn254Fp>, 256, 0) != hipSuccess ||
-        cus <= 0 || blocks <= 0) {
+        eo != hipSuccess || cus <= 0 || blocks <= 0) {
         (void)hipGetLastError();
         return 1u << 17;
     }
@@ -1352,30 +1494,30 @@ void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
                        void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st) {
     if (!lanes) return;
-    if (cv == CurveId::Grumpkin)
-        hipLaunchKernelGGL(k_accumulate<Bn254Fr>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st,
-                           segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table,
-                           T, lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
-    else
-        hipLaunchKernelGGL(k_accumulate<Bn254Fp>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st,
-                           segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table,
-                           T, lanes, (uint4*)buckets, rec_keys, (uint4*)rec_pts);
+#define ZK_ACC(C)                                                                                  \
+    hipLaunchKernelGGL(k_accumulate<C>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st, segs,      \
+                       seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table, T, lanes, \
+                       (uint4*)buckets, rec_keys, (uint4*)rec_pts)
+    if (cv == CurveId::Grumpkin) ZK_ACC(G1Curve<Bn254Fr>);
+    else if (cv == CurveId::Bn254G2) ZK_ACC(G2Curve);
+    else ZK_ACC(G1Curve<Bn254Fp>);
+#undef ZK_ACC
 }
 
-template <class F>
+template <class C>
 static void merge_runs_t(const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
                          void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                          hipStream_t st) {
     uint32_t* long_list = lists;
     uint32_t* mid_list = lists + list_cap;
-    hipLaunchKernelGGL(k_merge_runs<F>, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
+    hipLaunchKernelGGL(k_merge_runs<C>, dim3(blocks_for((nrec + 1) / 2, 256)), dim3(256), 0, st, keys,
                        (const uint4*)pts, nrec, limit, (uint4*)dest, counts, long_list,
                        counts + 1, mid_list, list_cap);
     // grid-stride over the lists (their lengths are known on the device only)
-    hipLaunchKernelGGL(k_merge_mid<F>, dim3(std::min<uint32_t>(blocks_for(list_cap, 32), 1024)), dim3(256),
+    hipLaunchKernelGGL(k_merge_mid<C>, dim3(std::min<uint32_t>(blocks_for(list_cap, 32), 1024)), dim3(256),
                        0, st, keys, (const uint4*)pts, nrec, (const uint32_t*)(counts + 1),
                        (const uint32_t*)mid_list, list_cap, (uint4*)dest);
-    hipLaunchKernelGGL(k_merge_long<F>, dim3(std::min<uint32_t>(list_cap, 2048)), dim3(256), 0, st, keys,
+    hipLaunchKernelGGL(k_merge_long<C>, dim3(std::min<uint32_t>(list_cap, 2048)), dim3(256), 0, st, keys,
                        (const uint4*)pts, nrec, (const uint32_t*)counts,
                        (const uint32_t*)long_list, list_cap, (uint4*)dest);
 }
@@ -1385,19 +1527,21 @@ void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64
                        uint32_t list_cap, hipStream_t st) {
     if (!nrec || !list_cap) return;
     if (cv == CurveId::Grumpkin)
-        merge_runs_t<Bn254Fr>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
+        merge_runs_t<G1Curve<Bn254Fr>>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
+    else if (cv == CurveId::Bn254G2)
+        merge_runs_t<G2Curve>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
     else
-        merge_runs_t<Bn254Fp>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
+        merge_runs_t<G1Curve<Bn254Fp>>(keys, pts, nrec, limit, dest, counts, lists, list_cap, st);
 }
 
-template <class F>
+template <class C>
 static void bucket_reduce_t(const SegDesc* segs, const uint64_t* seg_wave0,
                             const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
                             const void* buckets, void* rc, void* bits, hipStream_t st) {
-    hipLaunchKernelGGL(k_br_rowcol<F>, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
+    hipLaunchKernelGGL(k_br_rowcol<C>, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
                        seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
     const uint64_t bit_items = (uint64_t)nseg * 16;
-    hipLaunchKernelGGL(k_br_bits<F>, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
+    hipLaunchKernelGGL(k_br_bits<C>, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
                        seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);
 }
 
@@ -1406,9 +1550,11 @@ void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_w
                           const void* buckets, void* rc, void* bits, hipStream_t st) {
     if (!rc_waves) return;
     if (cv == CurveId::Grumpkin)
-        bucket_reduce_t<Bn254Fr>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+        bucket_reduce_t<G1Curve<Bn254Fr>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+    else if (cv == CurveId::Bn254G2)
+        bucket_reduce_t<G2Curve>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
     else
-        bucket_reduce_t<Bn254Fp>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+        bucket_reduce_t<G1Curve<Bn254Fp>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
 }
 
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
@@ -1430,6 +1576,18 @@ void launch_test_xyzz(int op, const void* p, const void* q, void* out, uint64_t 
                       hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_test_xyzz, dim3(blocks_for(n, 64)), dim3(64), 0, st, op,
+                       (const uint4*)p, (const uint4*)q, (uint4*)out, n);
+}
+
+void launch_gen_points_g2(const void* ks, const void* gtab, uint64_t n, void* out, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_gen_points_g2, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                       (const uint4*)ks, (const uint4*)gtab, n, (uint4*)out);
+}
+
+void launch_test_g2(int op, const void* p, const void* q, void* out, uint64_t n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_test_g2, dim3(blocks_for(n, 64)), dim3(64), 0, st, op,
                        (const uint4*)p, (const uint4*)q, (uint4*)out, n);
 }
 

@@ -8,11 +8,11 @@
 namespace zk {
 
 // A table of n points of curve cv takes table_bytes(n, cv): n rows of
-// entries_per_point(cv) 128-byte entries, then a bitmap with bit i set when
+// table_slots_per_point(cv) 128-byte slots, then a bitmap with bit i set when
 // point i is the identity (written by the table builders, read by the
 // decomposition).
 inline uint64_t table_bytes(uint64_t n, CurveId cv = CurveId::Bn254) {
-    return n * entries_per_point(cv) * kTableEntryBytes + (n + 63) / 64 * 8;
+    return n * table_slots_per_point(cv) * kTableEntryBytes + (n + 63) / 64 * 8;
 }
 // BN254 G1 only: SRS import from big-endian standard-form coordinates
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
@@ -33,7 +33,7 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 // lanes of k_accumulate resident at once on the device (CUs x workgroups per
 // CU at its register use x 256): a lane count just above it leaves a second,
 // nearly empty round of waves
-uint64_t accumulate_capacity(int device);
+uint64_t accumulate_capacity(int device, CurveId cv = CurveId::Bn254);
 void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
                        const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
@@ -49,6 +49,10 @@ void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_w
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
 // out[i] = k_i * G on Grumpkin, 64-byte affine points (x, y in Fr Montgomery form)
+// out[i] = k_i * G2, 128-byte affine points (x.c0, x.c1, y.c0, y.c1, Fp Montgomery)
+void launch_gen_points_g2(const void* ks, const void* gtab, uint64_t n, void* out, hipStream_t st);
+// BN254 G2 primitives on 256-byte host-form XYZZ points (see k_test_g2)
+void launch_test_g2(int op, const void* p, const void* q, void* out, uint64_t n, hipStream_t st);
 void launch_gen_points_grumpkin(const void* ks, const void* gtab, uint64_t n, void* out,
                                 hipStream_t st);
 void launch_test_fp(int op, const void* a, const void* b, void* out, uint64_t n,

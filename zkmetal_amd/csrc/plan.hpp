@@ -12,21 +12,36 @@
 
 namespace zk {
 
-// The two curves of the BN254 cycle the engine runs MSMs on: BN254 G1
-// (coordinates in Fp, scalars mod r, GLV: two table entries per base, P and
-// phi(P), and two 128-bit half-scalars) and Grumpkin (y^2 = x^3 - 17 over Fr,
-// scalars mod p, no GLV: one entry per base, 255-bit scalars).
-enum class CurveId : uint32_t { Bn254 = 0, Grumpkin = 1 };
+// The curves the engine runs MSMs on: BN254 G1 (coordinates in Fp, scalars
+// mod r, GLV: two digit entries per scalar over the bases P and phi(P), two
+// 128-bit half-scalars), Grumpkin (y^2 = x^3 - 17 over Fr, scalars mod p, no
+// GLV: one entry per scalar, 255-bit windows) and BN254 G2 (the twist over
+// Fp2, scalars mod r, no GLV, coordinates twice as wide).
+enum class CurveId : uint32_t { Bn254 = 0, Grumpkin = 1, Bn254G2 = 2 };
+// digit entries per scalar
 constexpr uint32_t entries_per_point(CurveId cv) { return cv == CurveId::Bn254 ? 2u : 1u; }
 constexpr uint32_t scalar_window_span(CurveId cv) { return cv == CurveId::Bn254 ? 128u : 255u; }
 // windows per MSM the decomposition's histogram counters hold (c >= 2 for
-// BN254 G1, c >= 4 for Grumpkin)
+// BN254 G1, c >= 4 for the 255-bit windows)
 constexpr uint32_t kMaxWindows = 64;
 
-// bytes per base-table entry (one affine point in the kernels' field form)
+// bytes per base-table slot (one affine point in the kernels' field form, or
+// the c0 / c1 half of a G2 point)
 constexpr uint64_t kTableEntryBytes = 128;
-// bytes per stored XYZZ point (buckets, merge records, reduction partials)
+// bytes per stored XYZZ point over one field (buckets, merge records,
+// reduction partials); a G2 point is two of them
 constexpr uint64_t kBucketBytes = 144;
+
+// table slots per digit entry's base, and per point (P and phi(P) for G1)
+constexpr uint32_t slots_per_entry(CurveId cv) { return cv == CurveId::Bn254G2 ? 2u : 1u; }
+constexpr uint32_t table_slots_per_point(CurveId cv) { return cv == CurveId::Grumpkin ? 1u : 2u; }
+// stored point, host-form point (XYZZ in the 2^256 form) and result
+// (Jacobian X | Y | Z) sizes
+constexpr uint64_t point_bytes(CurveId cv) { return cv == CurveId::Bn254G2 ? 2 * kBucketBytes : kBucketBytes; }
+constexpr uint64_t host_point_bytes(CurveId cv) { return cv == CurveId::Bn254G2 ? 256u : 128u; }
+constexpr uint64_t result_bytes(CurveId cv) { return cv == CurveId::Bn254G2 ? 192u : 96u; }
+// affine input point: x, y (64 B), or x.c0, x.c1, y.c0, y.c1 for G2 (128 B)
+constexpr uint64_t input_point_bytes(CurveId cv) { return cv == CurveId::Bn254G2 ? 128u : 64u; }
 
 struct MsmDesc {
     uint64_t scalar0;  // first scalar of this MSM in the batch scalar buffer
