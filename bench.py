@@ -18,6 +18,7 @@ inputs, timed in the same run.
     python bench.py --config batched|skewed|serve   (the batched, skewed and
                                                      serving workloads)
     python bench.py --config grumpkin              (the same MSM on Grumpkin G1)
+    python bench.py --config g2                    (the same MSM on BN254 G2)
 """
 import argparse
 import json
@@ -65,6 +66,18 @@ def algorithmic_field_muls_grumpkin(n):
     for c in range(4, 25):
         w = math.ceil(254 / c)
         f = 10 * n * w + 28 * w * 2 ** (c - 1)
+        best = f if best is None else min(best, f)
+    return best
+
+
+def algorithmic_field_muls_g2(n):
+    """BN254 G2 (no GLV): Fp multiplies with Karatsuba Fp2 arithmetic, 28 per
+    mixed addition (8 Fp2 multiplies x 3 + 2 squarings x 2) over n entries
+    per window, 2 additions of 40 per bucket, ceil(254/c) windows."""
+    best = None
+    for c in range(4, 25):
+        w = math.ceil(254 / c)
+        f = 28 * n * w + 80 * w * 2 ** (c - 1)
         best = f if best is None else min(best, f)
     return best
 
@@ -279,6 +292,76 @@ def grumpkin(args, eng, peak):
     return line
 
 
+def g2(args, eng, peak):
+    """BN254 G2 MSM (Fp2 coordinates, 255-bit signed windows over n entries),
+    bases k_i * G2 generated on the device, scalars uniform in Fr, both
+    resident.  Accumulation roofline: 28 Fp multiplies per entry."""
+    n = 1 << args.logn
+    dpts = zk.DeviceBuffer(128 * n)
+    eng.g2_points_generate(args.seed, n, dpts)
+    scal = zk.gen_scalars(args.seed + 1, n, 0)
+    dsc = zk.DeviceBuffer(scal.nbytes)
+    dsc.upload(scal)
+    result = {}
+
+    def step():
+        result["r"] = eng.g2_msm_batch_device(dpts, dsc, [n])[0]
+        result.setdefault("t", []).append(eng.last_timing())
+
+    ms, samples = timed_steps(step, args.steps, args.warmup)
+    timings = result["t"][args.warmup:]
+    acc_ms = statistics.mean(t["accumulate_ms"] for t in timings)
+    valid = statistics.mean(t["valid_entries"] for t in timings)
+    stage = {k: round(statistics.mean(t[k] for t in timings), 4)
+             for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms",
+                       "copyout_ms", "device_ms", "host_tail_ms")}
+    achieved = valid * 28 * PRODUCTS_PER_FIELD_MUL / (acc_ms * 1e-3)
+    msm_products = algorithmic_field_muls_g2(n) * PRODUCTS_PER_FIELD_MUL
+    line = {
+        "metric": f"BN254 G2 MSM ms at n=2^{args.logn}",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain BN254 G2 MSM, n=2^{args.logn}, scalars uniform in Fr (seeded), "
+                        f"bases k_i*G2 (seeded, generated on the device), affine Fp2 Montgomery; "
+                        f"bases+scalars resident in HBM",
+            "n": n,
+            "window_bits": int(timings[-1]["window_bits"]),
+        },
+        "roofline": {
+            "bound": "valu_int32_mul",
+            "kernel": "k_accumulate",
+            "achieved": float("%.4g" % (achieved / 1e12)),
+            "peak": float("%.4g" % (peak / 1e12)),
+            "unit": "T int32 products/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": None,
+        },
+        "extra": {
+            "ms_p50": round(statistics.median(samples), 4),
+            "msms_per_s": round(1e3 / ms, 2),
+            "stage_ms": stage,
+            "msm_roofline_frac": round(msm_products / (ms * 1e-3) / peak, 4),
+            "msm_algorithmic_field_muls": algorithmic_field_muls_g2(n),
+        },
+        # the reference's G2 CPU Pippenger (g2PippengerMSM) is Swift; no C
+        # restatement of it is built here
+        "cpu_baseline": None,
+    }
+    dpts.free()
+    dsc.free()
+    return line
+
+
 def batched(args, eng, peak):
     """64 MSMs sharing a 2^20 resident table, sizes log-uniform in [2^12, 2^20]."""
     rnd = np.random.default_rng(args.seed)
@@ -452,7 +535,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=22)
     ap.add_argument("--seed", type=int, default=0x5EED)
-    ap.add_argument("--config", default="plain", choices=["plain", "batched", "skewed", "serve", "grumpkin"])
+    ap.add_argument("--config", default="plain", choices=["plain", "batched", "skewed", "serve", "grumpkin", "g2"])
     ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=2, help="serve: engine slots")
@@ -479,6 +562,8 @@ def main():
         line = serve(args, eng)
     elif args.config == "grumpkin":
         line = grumpkin(args, eng, peak)
+    elif args.config == "g2":
+        line = g2(args, eng, peak)
     else:
         line = batched(args, eng, peak)
     eng.close()
