@@ -171,6 +171,50 @@ def fq_mul_sub(k, a, b, c, sqr=False, borrow=1):
     return t
 
 
+def fq_mul_add(a, b, c, d):
+    """fq_mul_add: (a*b + c*d)/R with one Montgomery reduction (the kernel
+    splits each column over two accumulators; their sum is this one).
+    a, c normalised; b limbs < 1.5 * 2^30, d limbs < 2^30 (column sums below
+    9 * 1.5 * 2^59 + 9 * 2^59 + 9 * 2^58 = 2^63.75)."""
+    assert all(v < (1 << 29) for v in a + c)
+    assert all(v < (3 << 29) for v in b) and all(v < (1 << 30) for v in d)
+    s = val(a) * val(b) + val(c) * val(d)
+    assert s < P * R // 2
+    m = [0] * 9
+    t = [0] * 9
+    acc = 0
+
+    def mac(x, y):
+        nonlocal acc
+        acc += x * y
+        assert acc < U64, "column overflow"
+
+    for k in range(9):
+        for j in range(k):
+            mac(a[j], b[k - j])
+            mac(c[j], d[k - j])
+            mac(m[j], PL[k - j])
+        mac(a[k], b[0])
+        mac(c[k], d[0])
+        m[k] = ((acc & 0xFFFFFFFF) * PINV) & MASK
+        mac(m[k], PL[0])
+        assert acc & MASK == 0
+        acc >>= 29
+    for k in range(9, 17):
+        for j in range(k - 8, 9):
+            mac(a[j], b[k - j])
+            mac(c[j], d[k - j])
+            mac(m[j], PL[k - j])
+        t[k - 9] = acc & MASK
+        acc >>= 29
+    t[8] = acc
+    assert t[8] < (1 << 29)
+    r = val(t)
+    assert r % P == s * pow(R, -1, P) % P
+    assert r < s // R + P + 1
+    return t
+
+
 def fq_add(a, b):
     r, c = [], 0
     for i in range(9):
@@ -269,27 +313,30 @@ RUN_START = (limbs(0), limbs(0), ONE, ONE)
 
 def madd_run(acc, acc_id, x2, y2):
     """qxyzz_madd_run: the accumulation-loop addition, identity as a flag,
-    exceptional cases screened on limb 0 of PP"""
+    exceptional cases screened on limb 0 of PP.  Products in pairs (P, R),
+    (PP, RR), (PPP, Q), (ZZ3, ZZZ3) -- pairing is scheduling only -- and
+    Y3 = R (Q - X3 + 7p) + Y (2p - PPP) under one reduction."""
     X1, Y1, ZZ1, ZZZ1 = acc
     lt(X1, 6.5), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
     Pp = fq_mul_sub(7, x2, ZZ1, X1)
     Rr = fq_mul_sub(5, y2, ZZZ1, Y1)
     lt(Pp, 8.5), lt(Rr, 6.5)
     PP = fq_sqr(Pp)
-    lt(PP, 1.6)
+    RR = fq_sqr(Rr)
+    lt(PP, 1.6), lt(RR, 1.6)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(X1, PP)
     T = [2 * Q[i] + PPP[i] for i in range(9)]  # lazy: no carries
     lt(T, 4.5)
-    X3 = fq_mul_sub(5, Rr, Rr, T, sqr=True, borrow=3)
+    X3 = fq_sub(5, RR, T)
     lt(X3, 6.5)
-    Y3 = fq_mul_sub(2, Rr, sub_lazy(7, Q, X3), fq_mul(Y1, PPP))
-    lt(Y3, 3.5)
+    Y3 = fq_mul_add(Rr, sub_lazy(7, Q, X3), Y1, sub_lazy(2, limbs(0), PPP))
+    lt(Y3, 1.5)
     r = (X3, Y3, fq_mul(ZZ1, PP), fq_mul(ZZZ1, PPP))
     cancel = False
     if PP[0] in (0, PL[0]):  # the one-limb screen
         z = not acc_id and iz2(PP)
-        d = z and iz2(fq_sqr(Rr))
+        d = z and iz2(RR)
         cancel = z and not d
         if d:
             r = dbl_core(x2, y2)

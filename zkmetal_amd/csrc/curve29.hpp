@@ -248,6 +248,162 @@ struct Arith {
     }
 
     static __device__ __forceinline__ Fq fq_sqr(const Fq& a) { return fq_sqr_sub<0>(a, a); }
+
+    // Two independent products computed column by column side by side (two
+    // multiply-add chains per column, so each hides the other's latency):
+    // r1 = fq_mul_sub<K1, B1>(a1, b1, c1), r2 = fq_mul_sub<K2, B2>(a2, b2, c2),
+    // same operand conditions.
+    template <int K1, int B1, int K2, int B2>
+    static __device__ __forceinline__ void fq_mul_sub_x2(const Fq& a1, const Fq& b1, const Fq& c1, Fq& r1,
+                                                         const Fq& a2, const Fq& b2, const Fq& c2, Fq& r2) {
+        uint32_t m1[9], m2[9];
+        uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+            }
+            qmac(s1, a1.v[k], b1.v[0]);
+            qmac(s2, a2.v[k], b2.v[0]);
+            m1[k] = ((uint32_t)s1 * F::kPinv) & ZQ_MASK;
+            m2[k] = ((uint32_t)s2 * F::kPinv) & ZQ_MASK;
+            qmac(s1, m1[k], F::p(0));
+            qmac(s2, m2[k], F::p(0));
+            s1 >>= 29;
+            s2 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+            if (K1 > 0) s1 += q_kpr<K1, B1>(k - 9) - c1.v[k - 9];
+            if (K2 > 0) s2 += q_kpr<K2, B2>(k - 9) - c2.v[k - 9];
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+            }
+            r1.v[k - 9] = (uint32_t)s1 & ZQ_MASK;
+            r2.v[k - 9] = (uint32_t)s2 & ZQ_MASK;
+            s1 >>= 29;
+            s2 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+        r1.v[8] = (uint32_t)s1 + (K1 > 0 ? q_kpr<K1, B1>(8) - c1.v[8] : 0u);
+        r2.v[8] = (uint32_t)s2 + (K2 > 0 ? q_kpr<K2, B2>(8) - c2.v[8] : 0u);
+    }
+
+    // r1 = a1^2, r2 = a2^2 side by side (as fq_sqr)
+    static __device__ __forceinline__ void fq_sqr_x2(const Fq& a1, Fq& r1, const Fq& a2, Fq& r2) {
+        uint32_t d1[9], d2[9], m1[9], m2[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            d1[i] = a1.v[i] << 1;
+            d2[i] = a2.v[i] << 1;
+        }
+        uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int i = 0; 2 * i < k; i++) {
+                qmac(s1, d1[i], a1.v[k - i]);
+                qmac(s2, d2[i], a2.v[k - i]);
+            }
+            if ((k & 1) == 0) {
+                qmac(s1, a1.v[k / 2], a1.v[k / 2]);
+                qmac(s2, a2.v[k / 2], a2.v[k / 2]);
+            }
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+            }
+            m1[k] = ((uint32_t)s1 * F::kPinv) & ZQ_MASK;
+            m2[k] = ((uint32_t)s2 * F::kPinv) & ZQ_MASK;
+            qmac(s1, m1[k], F::p(0));
+            qmac(s2, m2[k], F::p(0));
+            s1 >>= 29;
+            s2 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+#pragma unroll
+            for (int i = k - 8; 2 * i < k; i++) {
+                qmac(s1, d1[i], a1.v[k - i]);
+                qmac(s2, d2[i], a2.v[k - i]);
+            }
+            if ((k & 1) == 0) {
+                qmac(s1, a1.v[k / 2], a1.v[k / 2]);
+                qmac(s2, a2.v[k / 2], a2.v[k / 2]);
+            }
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+            }
+            r1.v[k - 9] = (uint32_t)s1 & ZQ_MASK;
+            r2.v[k - 9] = (uint32_t)s2 & ZQ_MASK;
+            s1 >>= 29;
+            s2 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+        r1.v[8] = (uint32_t)s1;
+        r2.v[8] = (uint32_t)s2;
+    }
+
+    // (a b + c d) / 2^261 mod p with one Montgomery reduction: result <
+    // (ab + cd)/2^261 + p, normalised.  a, c normalised; b limbs < 1.5 * 2^30,
+    // d limbs < 2^30 (column sums < 2^63.75); ab + cd < p * 2^261 / 2.  Each
+    // column's products go to two accumulators (a b and half the m p in one,
+    // c d and the rest in the other), joined before m is taken.
+    static __device__ __forceinline__ Fq fq_mul_add(const Fq& a, const Fq& b, const Fq& c, const Fq& d) {
+        uint32_t m[9];
+        Fq t;
+        uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(s1, a.v[j], b.v[k - j]);
+                qmac(s2, c.v[j], d.v[k - j]);
+                qmac((j & 1) ? s2 : s1, m[j], F::p(k - j));
+            }
+            qmac(s1, a.v[k], b.v[0]);
+            qmac(s2, c.v[k], d.v[0]);
+            s1 += s2;
+            m[k] = ((uint32_t)s1 * F::kPinv) & ZQ_MASK;
+            qmac(s1, m[k], F::p(0));
+            s1 >>= 29;
+            s2 = 0;
+            carry_fence(s1);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(s1, a.v[j], b.v[k - j]);
+                qmac(s2, c.v[j], d.v[k - j]);
+                qmac((j & 1) ? s2 : s1, m[j], F::p(k - j));
+            }
+            s1 += s2;
+            t.v[k - 9] = (uint32_t)s1 & ZQ_MASK;
+            s1 >>= 29;
+            s2 = 0;
+            carry_fence(s1);
+        }
+        t.v[8] = (uint32_t)s1;
+        return t;
+    }
     // x * 2^256 mod p (the 32-bit-limb Montgomery form) -> x * 2^261 mod p
     static __device__ __forceinline__ Fq fq_from_fp(const Fp& a) { return fq_mul(fq_slice(a), F::from32()); }
 
@@ -420,23 +576,23 @@ struct Arith {
     // identity.
     static __device__ __forceinline__ void qxyzz_madd_run(QXyzz& acc, bool& acc_id, const Fq& x2,
                                                    const Fq& y2) {
-        const Fq P = fq_mul_sub<7>(x2, acc.ZZ, acc.X);   // U2 - X1 + 7p < 8.5p
-        const Fq R = fq_mul_sub<5>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 5p < 6.5p
-        const Fq PP = fq_sqr(P);
-        const Fq PPP = fq_mul(P, PP);
-        const Fq Q = fq_mul(acc.X, PP);
+        // the ten products in pairs of independent ones, and Y3's two under one
+        // reduction: Y3 = (R (Q - X3 + 7p) + Y1 (2p - PPP)) / 2^261
+        Fq P, R, PP, RR, PPP, Q;
+        fq_mul_sub_x2<7, 1, 5, 1>(x2, acc.ZZ, acc.X, P, y2, acc.ZZZ, acc.Y, R);  // < 8.5p, < 6.5p
+        fq_sqr_x2(P, PP, R, RR);                                                   // < 1.6p each
+        fq_mul_sub_x2<0, 1, 0, 1>(P, PP, P, PPP, acc.X, PP, P, Q);                 // < 1.5p each
         QXyzz r;
         Fq T;  // 2Q + PPP < 4.5p, limbs sums of three normalised limbs
 #pragma unroll
         for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
-        r.X = fq_sqr_sub<5, 3>(R, T);                                        // < 6.4p
-        r.Y = fq_mul_sub<2>(R, fq_sub_lazy<7>(Q, r.X), fq_mul(acc.Y, PPP));  // < 3.5p
-        r.ZZ = fq_mul(acc.ZZ, PP);
-        r.ZZZ = fq_mul(acc.ZZZ, PPP);
+        r.X = fq_sub<5>(RR, T);                                                     // < 6.3p
+        r.Y = fq_mul_add(R, fq_sub_lazy<7>(Q, r.X), acc.Y, fq_sub_lazy<2>(fq_zero(), PPP));  // < 1.4p
+        fq_mul_sub_x2<0, 1, 0, 1>(acc.ZZ, PP, P, r.ZZ, acc.ZZZ, PPP, P, r.ZZZ);
         bool cancel = false;
         if (__builtin_expect(PP.v[0] == 0u || PP.v[0] == F::p(0), 0)) {
             const bool z = !acc_id && fq_is_zero2(PP);
-            const bool d = z && fq_is_zero2(fq_sqr(R));  // R^2 < 1.5p
+            const bool d = z && fq_is_zero2(RR);
             cancel = z && !d;
             if (d) r = qxyzz_dbl_aff(x2, y2);
         }
