@@ -777,13 +777,18 @@ __global__ __launch_bounds__(256) void k_accumulate(
     bool first = true;
     typename C::Pt acc = C::run_start();
     bool acc_id = true;
-    // software pipeline: the next entry's key, value and base are loaded
-    // before the current mixed addition, hiding the random 128-byte gather
-    uint32_t k = cur, v = vals[b];
-    typename C::Af q = C::load_tab(tb, v & 0x7fffffffu, v >> 31);
+    // software pipeline: before entry p's mixed addition the base of entry
+    // p+1 is gathered (its value was loaded one iteration earlier, so the
+    // gather's address waits on nothing issued in this iteration) and the key
+    // and value of entry p+2 are loaded; indices clamp to the lane's last entry
+    uint32_t k = cur;
+    const uint32_t v0 = vals[b];
+    typename C::Af q = C::load_tab(tb, v0 & 0x7fffffffu, v0 >> 31);
+    const uint64_t p1 = min(b + 1, e - 1);
+    uint32_t kn = keys[p1], vn = vals[p1];
     for (uint64_t p = b; p < e; p++) {
-        const uint64_t pn = min(p + 1, e - 1);
-        const uint32_t kn = keys[pn], vn = vals[pn];
+        const uint64_t pnn = min(p + 2, e - 1);
+        const uint32_t knn = keys[pnn], vnn = vals[pnn];
         const typename C::Af qn = C::load_tab(tb, vn & 0x7fffffffu, vn >> 31);
         if (k != cur) {
             // run `cur` ended at p-1 and is not the lane's last run
@@ -803,7 +808,8 @@ __global__ __launch_bounds__(256) void k_accumulate(
         // digits (table bitmap), so q is never the identity
         C::madd_run(acc, acc_id, q);
         k = kn;
-        v = vn;
+        kn = knn;
+        vn = vnn;
         q = qn;
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
