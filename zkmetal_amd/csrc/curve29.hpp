@@ -300,6 +300,65 @@ struct Arith {
         r2.v[8] = (uint32_t)s2 + (K2 > 0 ? q_kpr<K2, B2>(8) - c2.v[8] : 0u);
     }
 
+    // Three independent products a_i b_i / 2^261 side by side (as fq_mul).
+    static __device__ __forceinline__ void fq_mul_x3(const Fq& a1, const Fq& b1, Fq& r1, const Fq& a2,
+                                                     const Fq& b2, Fq& r2, const Fq& a3, const Fq& b3,
+                                                     Fq& r3) {
+        uint32_t m1[9], m2[9], m3[9];
+        uint64_t s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(s3, a3.v[j], b3.v[k - j]);
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+                qmac(s3, m3[j], F::p(k - j));
+            }
+            qmac(s1, a1.v[k], b1.v[0]);
+            qmac(s2, a2.v[k], b2.v[0]);
+            qmac(s3, a3.v[k], b3.v[0]);
+            m1[k] = ((uint32_t)s1 * F::kPinv) & ZQ_MASK;
+            m2[k] = ((uint32_t)s2 * F::kPinv) & ZQ_MASK;
+            m3[k] = ((uint32_t)s3 * F::kPinv) & ZQ_MASK;
+            qmac(s1, m1[k], F::p(0));
+            qmac(s2, m2[k], F::p(0));
+            qmac(s3, m3[k], F::p(0));
+            s1 >>= 29;
+            s2 >>= 29;
+            s3 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+            carry_fence(s3);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(s3, a3.v[j], b3.v[k - j]);
+                qmac(s1, m1[j], F::p(k - j));
+                qmac(s2, m2[j], F::p(k - j));
+                qmac(s3, m3[j], F::p(k - j));
+            }
+            r1.v[k - 9] = (uint32_t)s1 & ZQ_MASK;
+            r2.v[k - 9] = (uint32_t)s2 & ZQ_MASK;
+            r3.v[k - 9] = (uint32_t)s3 & ZQ_MASK;
+            s1 >>= 29;
+            s2 >>= 29;
+            s3 >>= 29;
+            carry_fence(s1);
+            carry_fence(s2);
+            carry_fence(s3);
+        }
+        r1.v[8] = (uint32_t)s1;
+        r2.v[8] = (uint32_t)s2;
+        r3.v[8] = (uint32_t)s3;
+    }
+
     // r1 = a1^2, r2 = a2^2 side by side (as fq_sqr)
     static __device__ __forceinline__ void fq_sqr_x2(const Fq& a1, Fq& r1, const Fq& a2, Fq& r2) {
         uint32_t d1[9], d2[9], m1[9], m2[9];

@@ -49,18 +49,21 @@ struct G2Arith {
         return A::fq_is_zero2(a.c0) & A::fq_is_zero2(a.c1);
     }
 
-    // Karatsuba: (a0 b0 - a1 b1) + ((a0 + a1)(b0 + b1) - a0 b0 - a1 b1) u
+    // Karatsuba: (a0 b0 - a1 b1) + ((a0 + a1)(b0 + b1) - a0 b0 - a1 b1) u; the
+    // three products side by side (independent multiply-add chains: one wave
+    // per SIMD runs this code, so its own chains must hide each other's
+    // latency)
     static __device__ __forceinline__ Fq2 mul2(const Fq2& a, const Fq2& b) {
-        const Fq t0 = A::fq_mul(a.c0, b.c0);
-        const Fq t1 = A::fq_mul(a.c1, b.c1);
-        const Fq t2 = A::fq_mul(A::fq_add(a.c0, a.c1), A::fq_add(b.c0, b.c1));
+        Fq t0, t1, t2;
+        A::fq_mul_x3(a.c0, b.c0, t0, a.c1, b.c1, t1, A::fq_add(a.c0, a.c1), A::fq_add(b.c0, b.c1), t2);
         return Fq2{red4(A::fq_sub<2>(t0, t1)), red8(A::fq_sub<3>(t2, A::fq_add(t0, t1)))};
     }
 
-    // (a0 + a1)(a0 - a1) + 2 a0 a1 u
+    // (a0 + a1)(a0 - a1) + 2 a0 a1 u, the two products side by side
     static __device__ __forceinline__ Fq2 sqr2(const Fq2& a) {
-        const Fq c0 = A::fq_mul(A::fq_add(a.c0, a.c1), A::fq_sub<2>(a.c0, a.c1));
-        const Fq t = A::fq_mul(a.c0, a.c1);
+        Fq c0, t;
+        A::fq_mul_sub_x2<0, 1, 0, 1>(A::fq_add(a.c0, a.c1), A::fq_sub<2>(a.c0, a.c1), a.c0, c0, a.c0, a.c1,
+                                     a.c0, t);
         return Fq2{c0, red4(A::fq_add(t, t))};
     }
 
@@ -126,7 +129,9 @@ struct G2Arith {
     // identity); acc_id says acc is the identity (a run start, or a
     // cancellation earlier in the run).  Doubling and cancellation have P = 0;
     // the exact tests and the doubling run in a branch a wave takes only when
-    // one of its lanes needs it.
+    // one of its lanes needs it (a ballot makes the branch wave-uniform, so
+    // the compiler keeps it a branch instead of computing the doubling on
+    // every step and selecting).
     static __device__ __forceinline__ void madd_run(QXyzz2& acc, bool& acc_id, const Fq2& x2,
                                                     const Fq2& y2) {
         const Fq2 P = sub2(mul2(x2, acc.ZZ), acc.X);
@@ -140,10 +145,12 @@ struct G2Arith {
         r.ZZ = mul2(acc.ZZ, PP);
         r.ZZZ = mul2(acc.ZZZ, PPP);
         bool cancel = false;
-        if (__builtin_expect(!acc_id && is_zero2(P), 0)) {
-            const bool d = is_zero2(R);
-            cancel = !d;
-            if (d) r = dbl_aff(x2, y2);
+        const bool ex = !acc_id && is_zero2(P);
+        if (__builtin_expect(__ballot(ex) != 0, 0)) {
+            const bool d = ex && is_zero2(R);
+            cancel = ex && !d;
+            const QXyzz2 dd = dbl_aff(x2, y2);
+            r = select(d, dd, r);
         }
         const QXyzz2 q{x2, y2, one2(), one2()};
         acc = select(acc_id, q, r);
