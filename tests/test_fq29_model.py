@@ -171,15 +171,23 @@ def fq_mul_sub(k, a, b, c, sqr=False, borrow=1):
     return t
 
 
-def fq_mul_add(a, b, c, d):
-    """fq_mul_add: (a*b + c*d)/R with one Montgomery reduction (the kernel
-    splits each column over two accumulators; their sum is this one).
+def fq_mul_add(a, b, c, d, k=0, e=None, borrow=1):
+    """fq_mul_add_sub<K, B>: (a*b + c*d)/R - e + k*p with one Montgomery
+    reduction (k = 0: fq_mul_add, no subtraction), the limbs of k*p - e
+    entering the high columns (the kernel splits each column over two
+    accumulators; their sum is this one).
     a, c normalised; b limbs < 1.5 * 2^30, d limbs < 2^30 (column sums below
     9 * 1.5 * 2^59 + 9 * 2^59 + 9 * 2^58 = 2^63.75)."""
     assert all(v < (1 << 29) for v in a + c)
     assert all(v < (3 << 29) for v in b) and all(v < (1 << 30) for v in d)
     s = val(a) * val(b) + val(c) * val(d)
     assert s < P * R // 2
+    dd = [0] * 9
+    if k:
+        K = kpr(k, borrow)
+        assert all(v <= borrow * MASK for v in e)
+        dd = [K[i] - e[i] for i in range(9)]
+        assert all(0 <= x < (1 << 32) for x in dd), "negative limb of k*p - e"
     m = [0] * 9
     t = [0] * 9
     acc = 0
@@ -189,29 +197,31 @@ def fq_mul_add(a, b, c, d):
         acc += x * y
         assert acc < U64, "column overflow"
 
-    for k in range(9):
-        for j in range(k):
-            mac(a[j], b[k - j])
-            mac(c[j], d[k - j])
-            mac(m[j], PL[k - j])
-        mac(a[k], b[0])
-        mac(c[k], d[0])
-        m[k] = ((acc & 0xFFFFFFFF) * PINV) & MASK
-        mac(m[k], PL[0])
+    for kk in range(9):
+        for j in range(kk):
+            mac(a[j], b[kk - j])
+            mac(c[j], d[kk - j])
+            mac(m[j], PL[kk - j])
+        mac(a[kk], b[0])
+        mac(c[kk], d[0])
+        m[kk] = ((acc & 0xFFFFFFFF) * PINV) & MASK
+        mac(m[kk], PL[0])
         assert acc & MASK == 0
         acc >>= 29
-    for k in range(9, 17):
-        for j in range(k - 8, 9):
-            mac(a[j], b[k - j])
-            mac(c[j], d[k - j])
-            mac(m[j], PL[k - j])
-        t[k - 9] = acc & MASK
+    for kk in range(9, 17):
+        acc += dd[kk - 9]
+        for j in range(kk - 8, 9):
+            mac(a[j], b[kk - j])
+            mac(c[j], d[kk - j])
+            mac(m[j], PL[kk - j])
+        t[kk - 9] = acc & MASK
         acc >>= 29
-    t[8] = acc
+    t[8] = acc + dd[8]
     assert t[8] < (1 << 29)
     r = val(t)
-    assert r % P == s * pow(R, -1, P) % P
-    assert r < s // R + P + 1
+    ev = val(e) if k else 0
+    assert (r + ev - k * P) % P == s * pow(R, -1, P) % P
+    assert r < s // R + P + 1 - ev + k * P
     return t
 
 

@@ -10,8 +10,8 @@ the twist (oracle/bn254_g2.py)."""
 import random
 
 from oracle import bn254_g2 as G2
-from test_fq29_model import (P, R, fq_add, fq_csub, fq_mul, fq_sqr, fq_sub, iz2, limbs, lt, val,
-                             worst)
+from test_fq29_model import (P, R, fq_add, fq_csub, fq_mul, fq_mul_add, fq_mul_sub, fq_sqr, fq_sub,
+                             iz2, limbs, lt, sub_lazy, val, worst)
 
 
 def red4(a):
@@ -102,6 +102,64 @@ def madd_run(acc, acc_id, x2, y2):
     return (q if acc_id else r), cancel
 
 
+# ---- the bucket loop's addition (G2Arith::madd_run): schoolbook Fp2 products
+# with merged reductions, c0 = (a0 b0 + a1 (2p - b1))/R, c1 = (a0 b1 + a1 b0)/R,
+# subtractions folded into the high columns, one or two conditional
+# subtractions on the stored outputs only.
+
+def nl(k, v):
+    """k*p - v limb-wise (the borrow form), v normalised < k p"""
+    return sub_lazy(k, limbs(0), v)
+
+
+def lz(a, b):
+    r = [a[i] + b[i] for i in range(9)]
+    assert all(x < (1 << 30) for x in r)
+    return r
+
+
+def madd_run_lazy(acc, acc_id, x2, y2):
+    X, Y, ZZ, ZZZ = acc
+    for c in acc:
+        chk(c)
+    Pp = (fq_mul_add(x2[0], ZZ[0], x2[1], nl(2, ZZ[1]), 2, X[0]), fq_mul_add(x2[0], ZZ[1], x2[1], ZZ[0], 2, X[1]))
+    Rr = (fq_mul_add(y2[0], ZZZ[0], y2[1], nl(2, ZZZ[1]), 2, Y[0]), fq_mul_add(y2[0], ZZZ[1], y2[1], ZZZ[0], 2, Y[1]))
+    for v in Pp + Rr:
+        lt(v, 3.1)
+    # fq_mul_sub<0>: one operand normalised, the other's limbs < 1.5 * 2^30
+    PP = (fq_mul_sub(0, fq_add(Pp[0], Pp[1]), sub_lazy(4, Pp[0], Pp[1]), Pp[0]),
+          fq_mul_sub(0, lz(Pp[0], Pp[0]), Pp[1], Pp[0]))
+    lt(PP[0], 1.3), lt(PP[1], 1.3)
+    PPP = (fq_mul_add(Pp[0], PP[0], Pp[1], nl(2, PP[1])), fq_mul_add(Pp[0], PP[1], Pp[1], PP[0]))
+    Q = (fq_mul_add(X[0], PP[0], X[1], nl(2, PP[1])), fq_mul_add(X[0], PP[1], X[1], PP[0]))
+    for v in PPP + Q:
+        lt(v, 1.1)
+    T = tuple([2 * Q[j][i] + PPP[j][i] for i in range(9)] for j in range(2))
+    lt(T[0], 4), lt(T[1], 4)
+    X3 = (fq_csub(2, fq_csub(4, fq_mul_sub(4, fq_add(Rr[0], Rr[1]), sub_lazy(4, Rr[0], Rr[1]), T[0], borrow=3))),
+          fq_csub(2, fq_csub(4, fq_mul_sub(4, lz(Rr[0], Rr[0]), Rr[1], T[1], borrow=3))))
+    chk(X3)
+    M2 = (fq_mul_add(Y[0], PPP[0], Y[1], nl(2, PPP[1])), fq_mul_add(Y[0], PPP[1], Y[1], PPP[0]))
+    TT = (fq_sub(2, Q[0], X3[0]), fq_sub(2, Q[1], X3[1]))
+    lt(TT[0], 3.1), lt(TT[1], 3.1)
+    Y3 = (fq_csub(2, fq_mul_add(Rr[0], TT[0], Rr[1], nl(4, TT[1]), 2, M2[0])),
+          fq_csub(2, fq_mul_add(Rr[0], TT[1], Rr[1], TT[0], 2, M2[1])))
+    chk(Y3)
+    ZZ3 = (fq_mul_add(ZZ[0], PP[0], ZZ[1], nl(2, PP[1])), fq_mul_add(ZZ[0], PP[1], ZZ[1], PP[0]))
+    ZZZ3 = (fq_mul_add(ZZZ[0], PPP[0], ZZZ[1], nl(2, PPP[1])), fq_mul_add(ZZZ[0], PPP[1], ZZZ[1], PPP[0]))
+    r = (X3, Y3, chk(ZZ3), chk(ZZZ3))
+    cancel = False
+    if not acc_id and iz(PP):
+        d = iz((fq_csub(2, Rr[0]), fq_csub(2, Rr[1])))
+        cancel = not d
+        if d:
+            X3d, Y3d, V, W = dbl_core(x2, y2)
+            r = (X3d, Y3d, V, W)
+    if acc_id:
+        r = (x2, y2, ONE2, ONE2)
+    return r, cancel
+
+
 def add(acc, q):
     p_id, q_id = iz(acc[2]), iz(q[2])
     U1 = mul2(acc[0], q[2])
@@ -185,15 +243,40 @@ def test_g2_point_ops_against_the_oracle():
         assert affine(add(A, Bq)) == G2.add(a, b)
         assert affine(dbl(A)) == G2.add(a, a)
     # the run addition: a run over a few points with a repeat and a cancellation
+    # (both forms; the table holds y or p - y, canonical or p itself)
     run = [pts[1], pts[2], pts[2], G2.neg(G2.add(pts[1], G2.add(pts[2], pts[2]))), pts[3], pts[3]]
-    acc, acc_id, want = None, True, None
-    acc = (ZERO2, ZERO2, ONE2, ONE2)
-    for q in run:
+    for step in (madd_run, madd_run_lazy):
+        acc, acc_id, want = None, True, None
+        acc = (ZERO2, ZERO2, ONE2, ONE2)
+        for q in run:
+            qa = to2(q[0]), to2(q[1])
+            acc, acc_id = step(acc, acc_id, qa[0], qa[1])
+            want = G2.add(want, q)
+            got = None if acc_id else affine(acc)
+            assert got == want
+
+
+def test_g2_run_addition_at_extremes():
+    """the bucket loop's addition from accumulators at the top of their
+    bound (every component just below 2p where its residue allows), long
+    runs, and table operands equal to p"""
+    rnd = random.Random(13)
+    for _ in range(12):
+        a = G2.mul(G2.G2, rnd.randrange(1, G2.R_ORDER))
+        q = G2.mul(G2.G2, rnd.randrange(1, G2.R_ORDER))
+        A = xyzz_of(a, rnd)
         qa = to2(q[0]), to2(q[1])
-        acc, acc_id = madd_run(acc, acc_id, qa[0], qa[1])
-        want = G2.add(want, q)
-        got = None if acc_id else affine(acc)
-        assert got == want
+        out, f = madd_run_lazy(A, False, qa[0], qa[1])
+        assert not f and affine(out) == G2.add(a, q)
+        cur, want = out, G2.add(a, q)
+        for _ in range(6):
+            q2 = G2.mul(G2.G2, rnd.randrange(1, G2.R_ORDER))
+            cur, f = madd_run_lazy(cur, False, to2(q2[0]), to2(q2[1]))
+            want = G2.add(want, q2)
+            assert not f and affine(cur) == want
+    # a y component of 0 stored as p - 0 = p
+    y = (limbs(P), limbs(5))
+    assert val(sub2(y, (limbs(0), limbs(5)))[0]) % P == 0
     # the table's p - y form of y = 0 components is p itself (< 2p)
     ny = (limbs(P), limbs(P - 5))
     assert iz(sub2(ny, (limbs(0), limbs(P - 5))))

@@ -420,11 +420,78 @@ struct Arith {
         r2.v[8] = (uint32_t)s2;
     }
 
+    // Two merged products side by side (four multiply-add chains):
+    // r1 = fq_mul_add_sub<K1, B1>(a1, b1, c1, d1, e1), r2 likewise.
+    template <int K1, int B1, int K2, int B2>
+    static __device__ __forceinline__ void fq_mul_add_sub_x2(const Fq& a1, const Fq& b1, const Fq& c1,
+                                                             const Fq& d1, const Fq& e1, Fq& r1,
+                                                             const Fq& a2, const Fq& b2, const Fq& c2,
+                                                             const Fq& d2, const Fq& e2, Fq& r2) {
+        uint32_t m1[9], m2[9];
+        uint64_t s1 = 0, t1 = 0, s2 = 0, t2 = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+#pragma unroll
+            for (int j = 0; j < k; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(t1, c1.v[j], d1.v[k - j]);
+                qmac(t2, c2.v[j], d2.v[k - j]);
+                qmac((j & 1) ? t1 : s1, m1[j], F::p(k - j));
+                qmac((j & 1) ? t2 : s2, m2[j], F::p(k - j));
+            }
+            qmac(s1, a1.v[k], b1.v[0]);
+            qmac(s2, a2.v[k], b2.v[0]);
+            qmac(t1, c1.v[k], d1.v[0]);
+            qmac(t2, c2.v[k], d2.v[0]);
+            s1 += t1;
+            s2 += t2;
+            m1[k] = ((uint32_t)s1 * F::kPinv) & ZQ_MASK;
+            m2[k] = ((uint32_t)s2 * F::kPinv) & ZQ_MASK;
+            qmac(s1, m1[k], F::p(0));
+            qmac(s2, m2[k], F::p(0));
+            s1 >>= 29;
+            s2 >>= 29;
+            t1 = 0;
+            t2 = 0;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+#pragma unroll
+        for (int k = 9; k < 17; k++) {
+            if (K1 > 0) s1 += q_kpr<K1, B1>(k - 9) - e1.v[k - 9];
+            if (K2 > 0) s2 += q_kpr<K2, B2>(k - 9) - e2.v[k - 9];
+#pragma unroll
+            for (int j = k - 8; j < 9; j++) {
+                qmac(s1, a1.v[j], b1.v[k - j]);
+                qmac(s2, a2.v[j], b2.v[k - j]);
+                qmac(t1, c1.v[j], d1.v[k - j]);
+                qmac(t2, c2.v[j], d2.v[k - j]);
+                qmac((j & 1) ? t1 : s1, m1[j], F::p(k - j));
+                qmac((j & 1) ? t2 : s2, m2[j], F::p(k - j));
+            }
+            s1 += t1;
+            s2 += t2;
+            r1.v[k - 9] = (uint32_t)s1 & ZQ_MASK;
+            r2.v[k - 9] = (uint32_t)s2 & ZQ_MASK;
+            s1 >>= 29;
+            s2 >>= 29;
+            t1 = 0;
+            t2 = 0;
+            carry_fence(s1);
+            carry_fence(s2);
+        }
+        r1.v[8] = (uint32_t)s1 + (K1 > 0 ? q_kpr<K1, B1>(8) - e1.v[8] : 0u);
+        r2.v[8] = (uint32_t)s2 + (K2 > 0 ? q_kpr<K2, B2>(8) - e2.v[8] : 0u);
+    }
+
     // (a b + c d) / 2^261 mod p with one Montgomery reduction: result <
     // (ab + cd)/2^261 + p, normalised.  a, c normalised; b limbs < 1.5 * 2^30,
     // d limbs < 2^30 (column sums < 2^63.75); ab + cd < p * 2^261 / 2.  Each
     // column's products go to two accumulators (a b and half the m p in one,
-    // c d and the rest in the other), joined before m is taken.
+    // c d and the rest in the other), joined before m is taken.  With K > 0
+    // (fq_mul_add_sub_x2) the limbs of K p - e enter the high columns as in
+    // fq_mul_sub: result < (ab + cd)/2^261 + p + K p - e.
     static __device__ __forceinline__ Fq fq_mul_add(const Fq& a, const Fq& b, const Fq& c, const Fq& d) {
         uint32_t m[9];
         Fq t;

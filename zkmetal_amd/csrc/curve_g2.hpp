@@ -75,6 +75,31 @@ struct G2Arith {
         return Fq2{red4(A::fq_add(a.c0, a.c0)), red4(A::fq_add(a.c1, a.c1))};
     }
 
+    // ---- the bucket loop's products: schoolbook Fp2 with merged reductions,
+    // c0 = (a0 b0 + a1 (KB p - b1)) / 2^261, c1 = (a0 b1 + a1 b0) / 2^261
+    // (the same 4 products of 81 multiply-adds and 2 reductions as Karatsuba's
+    // 3 products, with normalised outputs and no conditional subtraction),
+    // each component minus e + K p folded into its high columns.  a, b
+    // components normalised, b1 < KB p; results < (|a| |b| + |a| KB p)/2^261
+    // + p + K p - e.
+    template <int K>
+    static __device__ __forceinline__ Fq nl(const Fq& v) { return A::fq_sub_lazy<K>(fq_zero(), v); }
+
+    template <int K, int KB = 2>
+    static __device__ __forceinline__ Fq2 mul2m(const Fq2& a, const Fq2& b, const Fq2& e) {
+        Fq2 r;
+        A::fq_mul_add_sub_x2<K, 1, K, 1>(a.c0, b.c0, a.c1, nl<KB>(b.c1), e.c0, r.c0, a.c0, b.c1, a.c1, b.c0,
+                                         e.c1, r.c1);
+        return r;
+    }
+
+    static __device__ __forceinline__ Fq lzadd(const Fq& a, const Fq& b) {
+        Fq r;
+#pragma unroll
+        for (int i = 0; i < 9; i++) r.v[i] = a.v[i] + b.v[i];
+        return r;
+    }
+
     static __device__ __forceinline__ Fq2 triple2(const Fq2& a) {
         return Fq2{red8(A::fq_add(A::fq_add(a.c0, a.c0), a.c0)),
                    red8(A::fq_add(A::fq_add(a.c1, a.c1), a.c1))};
@@ -132,22 +157,42 @@ struct G2Arith {
     // one of its lanes needs it (a ballot makes the branch wave-uniform, so
     // the compiler keeps it a branch instead of computing the doubling on
     // every step and selecting).
+    // Products: mul2m (merged reductions); bounds (multiples of p): acc < 2p
+    // per component, P, R < 3.1p, PP < 1.3p, PPP, Q < 1.1p, T = 2Q + PPP < 4p,
+    // X3 < 5.3p and Y3 < 3.2p before their conditional subtractions, every
+    // stored component < 2p after them.
     static __device__ __forceinline__ void madd_run(QXyzz2& acc, bool& acc_id, const Fq2& x2,
                                                     const Fq2& y2) {
-        const Fq2 P = sub2(mul2(x2, acc.ZZ), acc.X);
-        const Fq2 R = sub2(mul2(y2, acc.ZZZ), acc.Y);
-        const Fq2 PP = sqr2(P);
-        const Fq2 PPP = mul2(P, PP);
-        const Fq2 Q = mul2(acc.X, PP);
+        const Fq2 P = mul2m<2>(x2, acc.ZZ, acc.X);   // U2 - X1 + 2p
+        const Fq2 R = mul2m<2>(y2, acc.ZZZ, acc.Y);  // S2 - Y1 + 2p
+        // PP = (P0 + P1)(P0 - P1 + 4p) + 2 P0 P1 u
+        Fq2 PP;
+        A::fq_mul_sub_x2<0, 1, 0, 1>(A::fq_add(P.c0, P.c1), A::fq_sub_lazy<4>(P.c0, P.c1), P.c0, PP.c0,
+                                     lzadd(P.c0, P.c0), P.c1, P.c0, PP.c1);
+        const Fq2 PPP = mul2m<0>(P, PP, P);
+        const Fq2 Q = mul2m<0>(acc.X, PP, P);
+        Fq2 T;  // 2Q + PPP, limbs sums of three normalised limbs
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            T.c0.v[i] = Q.c0.v[i] + Q.c0.v[i] + PPP.c0.v[i];
+            T.c1.v[i] = Q.c1.v[i] + Q.c1.v[i] + PPP.c1.v[i];
+        }
         QXyzz2 r;
-        r.X = sub2(sub2(sub2(sqr2(R), PPP), Q), Q);
-        r.Y = sub2(mul2(R, sub2(Q, r.X)), mul2(acc.Y, PPP));
-        r.ZZ = mul2(acc.ZZ, PP);
-        r.ZZZ = mul2(acc.ZZZ, PPP);
+        // X3 = R^2 - T: (R0 + R1)(R0 - R1 + 4p) - T0 + 4p, 2 R0 R1 - T1 + 4p
+        A::fq_mul_sub_x2<4, 3, 4, 3>(A::fq_add(R.c0, R.c1), A::fq_sub_lazy<4>(R.c0, R.c1), T.c0, r.X.c0,
+                                     lzadd(R.c0, R.c0), R.c1, T.c1, r.X.c1);
+        r.X = Fq2{red8(r.X.c0), red8(r.X.c1)};
+        // Y3 = R (Q - X3) - Y PPP
+        const Fq2 M2 = mul2m<0>(acc.Y, PPP, P);
+        const Fq2 TT{A::fq_sub<2>(Q.c0, r.X.c0), A::fq_sub<2>(Q.c1, r.X.c1)};  // < 3.1p
+        r.Y = mul2m<2, 4>(R, TT, M2);
+        r.Y = Fq2{red4(r.Y.c0), red4(r.Y.c1)};
+        r.ZZ = mul2m<0>(acc.ZZ, PP, P);
+        r.ZZZ = mul2m<0>(acc.ZZZ, PPP, P);
         bool cancel = false;
-        const bool ex = !acc_id && is_zero2(P);
+        const bool ex = !acc_id && is_zero2(PP);  // P = 0 iff PP = 0
         if (__builtin_expect(__ballot(ex) != 0, 0)) {
-            const bool d = ex && is_zero2(R);
+            const bool d = ex && is_zero2(Fq2{red4(R.c0), red4(R.c1)});
             cancel = ex && !d;
             const QXyzz2 dd = dbl_aff(x2, y2);
             r = select(d, dd, r);
