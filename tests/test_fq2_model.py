@@ -160,6 +160,44 @@ def madd_run_lazy(acc, acc_id, x2, y2):
     return r, cancel
 
 
+def m2(a, b, k=0, e=None, kb=2):
+    """mul2m<K, KB>: schoolbook Fp2 product, merged reductions, -e + k p"""
+    return (fq_mul_add(a[0], b[0], a[1], nl(kb, b[1]), k, e and e[0]),
+            fq_mul_add(a[0], b[1], a[1], b[0], k, e and e[1]))
+
+
+def add_lazy(acc, q):
+    """G2Arith::add on merged reductions: both XYZZ, components < 2p"""
+    for c in acc + q:
+        chk(c)
+    p_id, q_id = iz(acc[2]), iz(q[2])
+    U1 = m2(acc[0], q[2])
+    S1 = m2(acc[1], q[3])
+    Pp = m2(q[0], acc[2], 2, U1)
+    Rr = m2(q[1], acc[3], 2, S1)
+    for v in Pp + Rr:
+        lt(v, 3.1)
+    PP = (fq_mul_sub(0, fq_add(Pp[0], Pp[1]), sub_lazy(4, Pp[0], Pp[1]), Pp[0]),
+          fq_mul_sub(0, lz(Pp[0], Pp[0]), Pp[1], Pp[0]))
+    PPP = m2(Pp, PP)
+    Q = m2(U1, PP)
+    T = tuple([2 * Q[j][i] + PPP[j][i] for i in range(9)] for j in range(2))
+    lt(T[0], 4), lt(T[1], 4)
+    X3 = (fq_csub(2, fq_csub(4, fq_mul_sub(4, fq_add(Rr[0], Rr[1]), sub_lazy(4, Rr[0], Rr[1]), T[0], borrow=3))),
+          fq_csub(2, fq_csub(4, fq_mul_sub(4, lz(Rr[0], Rr[0]), Rr[1], T[1], borrow=3))))
+    M2 = m2(S1, PPP)
+    TT = (fq_sub(2, Q[0], X3[0]), fq_sub(2, Q[1], X3[1]))
+    Y3 = m2(Rr, TT, 2, M2, kb=4)
+    Y3 = (fq_csub(2, Y3[0]), fq_csub(2, Y3[1]))
+    r = (chk(X3), chk(Y3), chk(m2(m2(acc[2], q[2]), PP)), chk(m2(m2(acc[3], q[3]), PPP)))
+    d = iz(PP) and iz((fq_csub(2, Rr[0]), fq_csub(2, Rr[1]))) and not p_id and not q_id
+    if d:
+        r = dbl(acc)
+    if p_id:
+        r = q
+    return acc if q_id else r
+
+
 def add(acc, q):
     p_id, q_id = iz(acc[2]), iz(q[2])
     U1 = mul2(acc[0], q[2])
@@ -241,6 +279,7 @@ def test_g2_point_ops_against_the_oracle():
     for a, b in cases:
         A, Bq = xyzz_of(a, rnd), xyzz_of(b, rnd)
         assert affine(add(A, Bq)) == G2.add(a, b)
+        assert affine(add_lazy(A, Bq)) == G2.add(a, b)
         assert affine(dbl(A)) == G2.add(a, a)
     # the run addition: a run over a few points with a repeat and a cancellation
     # (both forms; the table holds y or p - y, canonical or p itself)
@@ -274,6 +313,13 @@ def test_g2_run_addition_at_extremes():
             cur, f = madd_run_lazy(cur, False, to2(q2[0]), to2(q2[1]))
             want = G2.add(want, q2)
             assert not f and affine(cur) == want
+        # the general addition chained on its own outputs
+        s, sw = cur, want
+        for _ in range(4):
+            s = add_lazy(s, A)
+            sw = G2.add(sw, a)
+            assert affine(s) == sw
+        assert affine(add_lazy(s, s)) == G2.add(sw, sw)
     # a y component of 0 stored as p - 0 = p
     y = (limbs(P), limbs(5))
     assert val(sub2(y, (limbs(0), limbs(5)))[0]) % P == 0

@@ -202,25 +202,37 @@ struct G2Arith {
         acc_id = cancel;
     }
 
-    // acc += q (both XYZZ)
+    // acc += q (both XYZZ, components < 2p), on the merged-reduction products
+    // of madd_run with the same bounds (U1, S1 < 1.1p)
     static __device__ __forceinline__ void add(QXyzz2& acc, const QXyzz2& q) {
         const bool p_id = is_id(acc);
         const bool q_id = is_id(q);
-        const Fq2 U1 = mul2(acc.X, q.ZZ);
-        const Fq2 U2 = mul2(q.X, acc.ZZ);
-        const Fq2 S1 = mul2(acc.Y, q.ZZZ);
-        const Fq2 S2 = mul2(q.Y, acc.ZZZ);
-        const Fq2 P = sub2(U2, U1);
-        const Fq2 R = sub2(S2, S1);
-        const bool d = is_zero2(P) && is_zero2(R) && !p_id && !q_id;
-        const Fq2 PP = sqr2(P);
-        const Fq2 PPP = mul2(P, PP);
-        const Fq2 Q = mul2(U1, PP);
+        const Fq2 U1 = mul2m<0>(acc.X, q.ZZ, acc.X);
+        const Fq2 S1 = mul2m<0>(acc.Y, q.ZZZ, acc.X);
+        const Fq2 P = mul2m<2>(q.X, acc.ZZ, U1);    // U2 - U1 + 2p
+        const Fq2 R = mul2m<2>(q.Y, acc.ZZZ, S1);  // S2 - S1 + 2p
+        Fq2 PP;
+        A::fq_mul_sub_x2<0, 1, 0, 1>(A::fq_add(P.c0, P.c1), A::fq_sub_lazy<4>(P.c0, P.c1), P.c0, PP.c0,
+                                     lzadd(P.c0, P.c0), P.c1, P.c0, PP.c1);
+        const Fq2 PPP = mul2m<0>(P, PP, P);
+        const Fq2 Q = mul2m<0>(U1, PP, P);
+        Fq2 T;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            T.c0.v[i] = Q.c0.v[i] + Q.c0.v[i] + PPP.c0.v[i];
+            T.c1.v[i] = Q.c1.v[i] + Q.c1.v[i] + PPP.c1.v[i];
+        }
         QXyzz2 r;
-        r.X = sub2(sub2(sub2(sqr2(R), PPP), Q), Q);
-        r.Y = sub2(mul2(R, sub2(Q, r.X)), mul2(S1, PPP));
-        r.ZZ = mul2(mul2(acc.ZZ, q.ZZ), PP);
-        r.ZZZ = mul2(mul2(acc.ZZZ, q.ZZZ), PPP);
+        A::fq_mul_sub_x2<4, 3, 4, 3>(A::fq_add(R.c0, R.c1), A::fq_sub_lazy<4>(R.c0, R.c1), T.c0, r.X.c0,
+                                     lzadd(R.c0, R.c0), R.c1, T.c1, r.X.c1);
+        r.X = Fq2{red8(r.X.c0), red8(r.X.c1)};
+        const Fq2 M2 = mul2m<0>(S1, PPP, P);
+        const Fq2 TT{A::fq_sub<2>(Q.c0, r.X.c0), A::fq_sub<2>(Q.c1, r.X.c1)};
+        r.Y = mul2m<2, 4>(R, TT, M2);
+        r.Y = Fq2{red4(r.Y.c0), red4(r.Y.c1)};
+        r.ZZ = mul2m<0>(mul2m<0>(acc.ZZ, q.ZZ, P), PP, P);
+        r.ZZZ = mul2m<0>(mul2m<0>(acc.ZZZ, q.ZZZ, P), PPP, P);
+        const bool d = is_zero2(PP) && is_zero2(Fq2{red4(R.c0), red4(R.c1)}) && !p_id && !q_id;
         if (__builtin_expect(d, 0)) r = dbl(acc);
         r = select(p_id, q, r);
         acc = select(q_id, acc, r);
