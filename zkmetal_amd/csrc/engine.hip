@@ -452,15 +452,21 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
         return kOk;
     };
     if (graph_mode_) {
-        // replay the captured sequence when the call has the same shape and
-        // every buffer it touches is where it was at capture
+        // replay the captured sequence when the call has the same shape (the
+        // curve -- its kernels -- and every launch dimension the capture
+        // baked in) and every buffer it touches is where it was at capture
         std::vector<uint64_t> key;
-        key.reserve(pl.msms.size() * 3 + 24);
+        key.reserve(pl.msms.size() * 3 + 40);
+        key.push_back((uint64_t)pl.curve);
         for (const MsmDesc& m : pl.msms) {
             key.push_back(m.n);
             key.push_back(m.c);
             key.push_back(m.table0);
         }
+        for (uint64_t v : {pl.scalars, pl.entries, pl.buckets, pl.lanes, pl.groups, pl.rc_waves,
+                           (uint64_t)pl.tiles, (uint64_t)pl.chunks, (uint64_t)pl.T, (uint64_t)pl.dtiles,
+                           (uint64_t)pl.G, (uint64_t)nseg})
+            key.push_back(v);
         key.push_back(table_n);  // places the table's identity bitmap
         for (const void* p : std::initializer_list<const void*>{d_scalars, d_table, desc_.p, hdesc_.p, digits_.p, keysA_.p, valsA_.p,
                               keysB_.p, valsB_.p, hist_.p, part_.p, segvalid_.p, buckets_.p,
