@@ -10,9 +10,9 @@
 
 namespace zk {
 
-static const uint32_t kTunedPlain[33] = {0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 8, 8, 12, 10, 10, 12, 15,
+static const uint32_t kTunedPlain[33] = {0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 8, 8, 10, 10, 10, 12, 15,
                                          15, 16, 16, 16, 16, 16, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-static const uint32_t kTunedBatch[33] = {0, 0, 0, 0, 0, 0, 0, 0, 6, 7, 8, 10, 10, 10, 11, 12, 13,
+static const uint32_t kTunedBatch[33] = {0, 0, 0, 0, 0, 0, 0, 0, 6, 7, 8, 8, 9, 10, 11, 12, 13,
                                          13, 16, 16, 16, 16, 16, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 }  // namespace zk
