@@ -360,20 +360,27 @@ def madd_run(acc, acc_id, x2, y2):
 def add(acc, q):
     X1, Y1, ZZ1, ZZZ1 = acc
     X2, Y2, ZZ2, ZZZ2 = q
+    """qxyzz_add: subtractions folded into the products (U2 - U1 + 2p,
+    S2 - S1 + 2p), Y3's two products under one reduction"""
+    # stored points come straight from the accumulation loop: X < 6.5p
+    lt(X1, 6.5), lt(Y1, 4), lt(ZZ1, 2), lt(ZZZ1, 2)
+    lt(X2, 6.5), lt(Y2, 4), lt(ZZ2, 2), lt(ZZZ2, 2)
     p_id, q_id = iz2(ZZ1), iz2(ZZ2)
     U1 = fq_mul(X1, ZZ2)
-    U2 = fq_mul(X2, ZZ1)
     S1 = fq_mul(Y1, ZZZ2)
-    S2 = fq_mul(Y2, ZZZ1)
-    Pp = fq_sub(2, U2, U1)
-    Rr = fq_sub(2, S2, S1)
+    Pp = fq_mul_sub(2, X2, ZZ1, U1)
+    Rr = fq_mul_sub(2, Y2, ZZZ1, S1)
+    lt(Pp, 3.5), lt(Rr, 3.5)
     PP = fq_sqr(Pp)
+    RR = fq_sqr(Rr)
     PPP = fq_mul(Pp, PP)
     Q = fq_mul(U1, PP)
-    RR = fq_sqr(Rr)
     dbl = iz2(PP) and iz2(RR) and not p_id and not q_id
-    X3 = fq_csub(3, fq_sub(5, RR, fq_add(fq_add(Q, Q), PPP)))
-    Y3 = fq_sub(2, fq_mul(Rr, fq_sub(4, Q, X3)), fq_mul(S1, PPP))
+    T = [2 * Q[i] + PPP[i] for i in range(9)]
+    X3 = fq_csub(3, fq_sub(5, RR, T))
+    lt(X3, 4)
+    Y3 = fq_mul_add(Rr, sub_lazy(4, Q, X3), S1, sub_lazy(2, limbs(0), PPP))
+    lt(Y3, 1.5)
     r = (X3, Y3, fq_mul(fq_mul(ZZ1, ZZ2), PP), fq_mul(fq_mul(ZZZ1, ZZZ2), PPP))
     if dbl:
         X3, Y3, V, W = dbl_core(X1, Y1)

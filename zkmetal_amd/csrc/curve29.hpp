@@ -731,26 +731,27 @@ struct Arith {
         acc_id = cancel;
     }
 
-    // acc += q (both XYZZ)
+    // acc += q (both XYZZ; X < 6.5p as stored by the accumulation loop,
+    // Y < 4p, ZZ, ZZZ < 2p).  Products in independent pairs, U2 - U1 and
+    // S2 - S1 folded into their products, Y3's two products under one
+    // reduction (as in qxyzz_madd_run): X3 < 4p, Y3 < 1.5p, ZZ3, ZZZ3 < 1.5p.
     static __device__ __forceinline__ void qxyzz_add(QXyzz& acc, const QXyzz& q) {
         const bool p_id = qxyzz_is_id(acc);
         const bool q_id = qxyzz_is_id(q);
-        const Fq U1 = fq_mul(acc.X, q.ZZ);
-        const Fq U2 = fq_mul(q.X, acc.ZZ);
-        const Fq S1 = fq_mul(acc.Y, q.ZZZ);
-        const Fq S2 = fq_mul(q.Y, acc.ZZZ);
-        const Fq P = fq_sub<2>(U2, U1);  // < 3.5p
-        const Fq R = fq_sub<2>(S2, S1);
-        const Fq PP = fq_sqr(P);
-        const Fq PPP = fq_mul(P, PP);
-        const Fq Q = fq_mul(U1, PP);
-        const Fq RR = fq_sqr(R);
+        Fq U1, S1, P, R, PP, RR, PPP, Q, ZZ12, ZZZ12;
+        fq_mul_sub_x2<0, 1, 0, 1>(acc.X, q.ZZ, acc.X, U1, acc.Y, q.ZZZ, acc.X, S1);
+        fq_mul_sub_x2<2, 1, 2, 1>(q.X, acc.ZZ, U1, P, q.Y, acc.ZZZ, S1, R);  // < 3.5p
+        fq_sqr_x2(P, PP, R, RR);
+        fq_mul_sub_x2<0, 1, 0, 1>(P, PP, P, PPP, U1, PP, P, Q);
         const bool dbl = fq_is_zero2(PP) && fq_is_zero2(RR) && !p_id && !q_id;
         QXyzz r;
-        r.X = fq_csub<3>(fq_sub<5>(RR, fq_add(fq_add(Q, Q), PPP)));
-        r.Y = fq_sub<2>(fq_mul(R, fq_sub<4>(Q, r.X)), fq_mul(S1, PPP));
-        r.ZZ = fq_mul(fq_mul(acc.ZZ, q.ZZ), PP);
-        r.ZZZ = fq_mul(fq_mul(acc.ZZZ, q.ZZZ), PPP);
+        Fq T;  // 2Q + PPP, limbs sums of three normalised limbs
+#pragma unroll
+        for (int i = 0; i < 9; i++) T.v[i] = Q.v[i] + Q.v[i] + PPP.v[i];
+        r.X = fq_csub<3>(fq_sub<5>(RR, T));
+        r.Y = fq_mul_add(R, fq_sub_lazy<4>(Q, r.X), S1, fq_sub_lazy<2>(fq_zero(), PPP));
+        fq_mul_sub_x2<0, 1, 0, 1>(acc.ZZ, q.ZZ, P, ZZ12, acc.ZZZ, q.ZZZ, P, ZZZ12);
+        fq_mul_sub_x2<0, 1, 0, 1>(ZZ12, PP, P, r.ZZ, ZZZ12, PPP, P, r.ZZZ);
         if (__builtin_expect(dbl, 0)) r = qxyzz_dbl(acc);
         r = qxyzz_select(p_id, q, r);
         acc = qxyzz_select(q_id, acc, r);
