@@ -547,9 +547,9 @@ int32_t Engine::msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t
     return msm_batch_host(points, scalars, counts, 1, out96);
 }
 
-// SURVEY-listed follow-up of zkMetal_ffi.swift:1241-1298: a batch whose
-// point slices are each a prefix of the resident SRS runs on the resident
-// table.  The check is exact: one 64-byte compare against the cached first
+// Host-point calls (zkMetal_ffi.swift:1241-1298 takes points on every call):
+// a batch whose point slices are each a prefix of the resident SRS runs on
+// the resident table.  The check is exact: one 64-byte compare against the cached first
 // SRS base rejects unrelated inputs at once; otherwise every slice is compared
 // with the host image of the SRS (in parallel), which is far cheaper than
 // sending the bases over PCIe and rebuilding their table.
