@@ -70,6 +70,16 @@ def test_null_engine_is_invalid_input():
     buf = (ctypes.c_uint8 * 96)()
     assert L.zkmetal_bn254_msm(None, buf, buf, 1, buf, buf, buf) == zk.ERR_INVALID_INPUT
     assert L.zkmetal_bn254_msm_batch(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    # the sibling-curve entry points reject a null engine the same way
+    assert L.zkmsm_grumpkin_msm(None, buf, buf, 1, buf, buf, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_grumpkin_msm_batch(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_grumpkin_msm_batch_device(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_grumpkin_points_generate(None, 1, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_bn254_g2_msm(None, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_bn254_g2_msm_batch(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_bn254_g2_msm_batch_device(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_bn254_g2_points_generate(None, 1, 1, buf) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_test_g2(None, 0, buf, buf, buf, 1) == zk.ERR_INVALID_INPUT
     L.zkmetal_msm_engine_destroy(None)
 
 
