@@ -17,6 +17,10 @@ What pins what:
     against sums of G2, doubling against the chord rule's tangent case;
   * MSM: closed form (sum s_i k_i mod r) * G2 for bases k_i * G2, against
     brute force on small cases.
+The C G2 Pippenger in oracle/bn254_ref.c is pinned against this module
+(tests/test_oracle_g2.py): its point operations at random Z against the
+affine group law here, including the doubling, inverse and identity cases,
+and its MSM against the closed form and brute force.
 """
 from . import bn254
 

@@ -31,6 +31,11 @@
  * algorithm over the field r with group order p; only the constants differ.
  * The GLV routines are BN254-only and are left out of that build.
  *
+ * The last block (BN254 builds only) is a G2 Pippenger over Fp2 = Fp[u] /
+ * (u^2 + 1) with the reference's G2 point formulas (BN254Pairing.swift:
+ * 45-70, 314-359), the G2 parity checker at sizes the Python oracle cannot
+ * reach and the timed G2 CPU baseline.
+ *
  * Parity pins: see oracle/bn254.py's header (EIP-196 vectors, closed form,
  * brute force, algebraic invariants) and tests/test_oracle.py.
  */
@@ -634,3 +639,317 @@ void oracle_signed_digits(const uint32_t k[8], int c, int nwin, int32_t *digits,
     }
     *carry_out = (int)carry;
 }
+
+#ifndef ORACLE_GRUMPKIN
+/* ---------------- BN254 G2 ----------------
+ *
+ * Fp2 = Fp[u] / (u^2 + 1), an element c0 + c1 u as 8 limbs (c0 | c1), both
+ * in the Fp Montgomery form above: Karatsuba multiply, (a0+a1)(a0-a1) square
+ * and norm inverse as BN254Pairing.swift:45-70.  Points on the twist
+ * y^2 = x^3 + 3/(9+u) in Jacobian (X, Y, Z) over Fp2, 24 limbs, identity
+ * Z == 0 stored as (1, 1, 0): doubling and full addition as g2Double /
+ * g2Add (BN254Pairing.swift:314-359), the same a = 0 formulas as the G1
+ * code above, with the mixed addition derived from them (Z2 = 1).  Affine
+ * bases are 16 limbs (x.c0, x.c1, y.c0, y.c1), all-zero = the identity.
+ *
+ * The reference's CPU G2 MSM (BN254G2MSMEngine.swift:542-560,
+ * Groth16Prover.swift:374-402) is bit-serial Straus, about 127 n additions
+ * for n bases; the MSM here is the G1 Pippenger above (unsigned c-bit
+ * windows over 256 bits, buckets by mixed addition, one inversion per
+ * window, running sums, Horner) over G2, which is what a CPU baseline at
+ * n = 2^22 can run.  Scalars are not reduced mod r: the bases lie in the
+ * subgroup of order r, so s P = (s mod r) P. */
+
+static int f2_is_zero(const uint64_t a[8]) { return f_is_zero(a) && f_is_zero(a + 4); }
+static void f2_add(const uint64_t a[8], const uint64_t b[8], uint64_t r[8]) {
+    f_add(a, b, r);
+    f_add(a + 4, b + 4, r + 4);
+}
+static void f2_sub(const uint64_t a[8], const uint64_t b[8], uint64_t r[8]) {
+    f_sub(a, b, r);
+    f_sub(a + 4, b + 4, r + 4);
+}
+static void f2_dbl(const uint64_t a[8], uint64_t r[8]) { f2_add(a, a, r); }
+static void f2_mul(const uint64_t a[8], const uint64_t b[8], uint64_t r[8]) {
+    uint64_t t0[4], t1[4], sa[4], sb[4], m[4];
+    f_mul(a, b, t0);
+    f_mul(a + 4, b + 4, t1);
+    f_add(a, a + 4, sa);
+    f_add(b, b + 4, sb);
+    f_mul(sa, sb, m);
+    f_sub(t0, t1, r);
+    f_add(t0, t1, t0);
+    f_sub(m, t0, r + 4);
+}
+static void f2_sqr(const uint64_t a[8], uint64_t r[8]) {
+    uint64_t t[4], s[4], d[4];
+    f_mul(a, a + 4, t);
+    f_add(a, a + 4, s);
+    f_sub(a, a + 4, d);
+    f_mul(s, d, r);
+    f_dbl(t, r + 4);
+}
+static void f2_inv(const uint64_t a[8], uint64_t r[8]) {
+    static const uint64_t zero[4] = {0, 0, 0, 0};
+    uint64_t n0[4], n1[4], ni[4], t[4];
+    f_sqr(a, n0);
+    f_sqr(a + 4, n1);
+    f_add(n0, n1, n0);
+    f_inv(n0, ni);
+    f_mul(a, ni, r);
+    f_mul(a + 4, ni, t);
+    f_sub(zero, t, r + 4);
+}
+
+static void g2_set_id(uint64_t p[24]) {
+    memset(p, 0, 192);
+    memcpy(p, MONE, 32);
+    memcpy(p + 8, MONE, 32);
+}
+static int g2_is_id(const uint64_t p[24]) { return f2_is_zero(p + 16); }
+static int g2a_is_id(const uint64_t q[16]) {
+    uint64_t acc = 0;
+    for (int i = 0; i < 16; i++) acc |= q[i];
+    return acc == 0;
+}
+
+static void g2_dbl(const uint64_t p[24], uint64_t r[24]) {
+    if (g2_is_id(p)) { memcpy(r, p, 192); return; }
+    uint64_t a[8], b[8], c[8], d[8], e[8], f[8], t[8], u[8], out[24];
+    f2_sqr(p, a);
+    f2_sqr(p + 8, b);
+    f2_sqr(b, c);
+    f2_add(p, b, t);           /* d = 2((x+b)^2 - a - c) */
+    f2_sqr(t, t);
+    f2_sub(t, a, t);
+    f2_sub(t, c, t);
+    f2_dbl(t, d);
+    f2_dbl(a, t);              /* e = 3a */
+    f2_add(t, a, e);
+    f2_sqr(e, f);
+    f2_dbl(d, t);              /* X3 = f - 2d */
+    f2_sub(f, t, out);
+    f2_sub(d, out, t);         /* Y3 = e (d - X3) - 8c */
+    f2_mul(e, t, u);
+    f2_dbl(c, t);
+    f2_dbl(t, t);
+    f2_dbl(t, t);
+    f2_sub(u, t, out + 8);
+    f2_add(p + 8, p + 16, t);  /* Z3 = (y+z)^2 - b - z^2 */
+    f2_sqr(t, t);
+    f2_sub(t, b, t);
+    f2_sqr(p + 16, u);
+    f2_sub(t, u, out + 16);
+    memcpy(r, out, 192);
+}
+
+static void g2_add(const uint64_t p[24], const uint64_t q[24], uint64_t r[24]) {
+    if (g2_is_id(p)) { memcpy(r, q, 192); return; }
+    if (g2_is_id(q)) { memcpy(r, p, 192); return; }
+    uint64_t z1z1[8], z2z2[8], u1[8], u2[8], s1[8], s2[8], h[8], rr[8], ii[8], jj[8],
+        v[8], t[8], out[24];
+    f2_sqr(p + 16, z1z1);
+    f2_sqr(q + 16, z2z2);
+    f2_mul(p, z2z2, u1);
+    f2_mul(q, z1z1, u2);
+    f2_mul(q + 16, z2z2, t);
+    f2_mul(p + 8, t, s1);
+    f2_mul(p + 16, z1z1, t);
+    f2_mul(q + 8, t, s2);
+    f2_sub(u2, u1, h);
+    f2_sub(s2, s1, t);
+    f2_dbl(t, rr);
+    if (f2_is_zero(h)) {
+        if (f2_is_zero(rr)) { g2_dbl(p, r); return; }
+        g2_set_id(r);
+        return;
+    }
+    f2_dbl(h, t);
+    f2_sqr(t, ii);
+    f2_mul(h, ii, jj);
+    f2_mul(u1, ii, v);
+    f2_sqr(rr, out);           /* X3 = rr^2 - jj - 2v */
+    f2_sub(out, jj, out);
+    f2_dbl(v, t);
+    f2_sub(out, t, out);
+    f2_sub(v, out, t);         /* Y3 = rr (v - X3) - 2 s1 jj */
+    f2_mul(rr, t, out + 8);
+    f2_mul(s1, jj, t);
+    f2_dbl(t, t);
+    f2_sub(out + 8, t, out + 8);
+    f2_add(p + 16, q + 16, t); /* Z3 = ((z1+z2)^2 - z1z1 - z2z2) h */
+    f2_sqr(t, t);
+    f2_sub(t, z1z1, t);
+    f2_sub(t, z2z2, t);
+    f2_mul(t, h, out + 16);
+    memcpy(r, out, 192);
+}
+
+/* Jacobian + affine: g2_add with Z2 = 1 (z2z2 = 1, u1 = X1, s1 = Y1) */
+static void g2_add_aff(const uint64_t p[24], const uint64_t q[16], uint64_t r[24]) {
+    if (g2a_is_id(q)) { memcpy(r, p, 192); return; }
+    if (g2_is_id(p)) {
+        memset(r, 0, 192);
+        memcpy(r, q, 128);
+        memcpy(r + 16, MONE, 32);
+        return;
+    }
+    uint64_t z1z1[8], u2[8], s2[8], h[8], rr[8], ii[8], jj[8], v[8], t[8], hh[8], out[24];
+    f2_sqr(p + 16, z1z1);
+    f2_mul(q, z1z1, u2);
+    f2_mul(p + 16, z1z1, t);
+    f2_mul(q + 8, t, s2);
+    f2_sub(u2, p, h);
+    f2_sub(s2, p + 8, t);
+    f2_dbl(t, rr);
+    if (f2_is_zero(h)) {
+        if (f2_is_zero(rr)) { g2_dbl(p, r); return; }
+        g2_set_id(r);
+        return;
+    }
+    f2_dbl(h, t);
+    f2_sqr(t, ii);
+    f2_mul(h, ii, jj);
+    f2_mul(p, ii, v);
+    f2_sqr(rr, out);
+    f2_sub(out, jj, out);
+    f2_dbl(v, t);
+    f2_sub(out, t, out);
+    f2_sub(v, out, t);
+    f2_mul(rr, t, out + 8);
+    f2_mul(p + 8, jj, t);
+    f2_dbl(t, t);
+    f2_sub(out + 8, t, out + 8);
+    f2_add(p + 16, h, t);      /* Z3 = (z1 + h)^2 - z1z1 - h^2 = 2 z1 h */
+    f2_sqr(t, t);
+    f2_sub(t, z1z1, t);
+    f2_sqr(h, hh);
+    f2_sub(t, hh, out + 16);
+    memcpy(r, out, 192);
+}
+
+/* n Jacobian G2 points -> affine with one Fp2 inversion; identity -> 0 */
+static void g2_batch_affine(const uint64_t *proj, uint64_t *aff, int n) {
+    if (n <= 0) return;
+    uint64_t *pre = (uint64_t *)malloc((size_t)n * 64);
+    uint64_t one2[8] = {0};
+    memcpy(one2, MONE, 32);
+    int first = -1;
+    for (int i = 0; i < n; i++) {
+        const uint64_t *pt = proj + (size_t)i * 24;
+        if (g2_is_id(pt)) {
+            if (i == 0) memcpy(pre, one2, 64);
+            else memcpy(pre + (size_t)i * 8, pre + (size_t)(i - 1) * 8, 64);
+        } else if (first < 0) {
+            first = i;
+            memcpy(pre + (size_t)i * 8, pt + 16, 64);
+        } else {
+            f2_mul(pre + (size_t)(i - 1) * 8, pt + 16, pre + (size_t)i * 8);
+        }
+    }
+    if (first < 0) {
+        memset(aff, 0, (size_t)n * 128);
+        free(pre);
+        return;
+    }
+    uint64_t inv[8];
+    f2_inv(pre + (size_t)(n - 1) * 8, inv);
+    for (int i = n - 1; i >= 0; i--) {
+        const uint64_t *pt = proj + (size_t)i * 24;
+        uint64_t *o = aff + (size_t)i * 16;
+        if (g2_is_id(pt)) { memset(o, 0, 128); continue; }
+        uint64_t zi[8], zi2[8], zi3[8];
+        if (i > first) {
+            f2_mul(inv, pre + (size_t)(i - 1) * 8, zi);
+            f2_mul(inv, pt + 16, inv);
+        } else {
+            memcpy(zi, inv, 64);
+        }
+        f2_sqr(zi, zi2);
+        f2_mul(zi2, zi, zi3);
+        f2_mul(pt, zi2, o);
+        f2_mul(pt + 8, zi3, o + 8);
+    }
+    free(pre);
+}
+
+static void g2_one_window(const uint64_t *pts, const uint32_t *sc, int n, int c, int w,
+                          uint64_t sum[24]) {
+    if (c < 1 || c > 24) return;
+    int nb = (1 << c) - 1;
+    uint64_t *bk = (uint64_t *)malloc((size_t)(nb + 1) * 192);
+    for (int b = 0; b <= nb; b++) g2_set_id(bk + (size_t)b * 24);
+    for (int i = 0; i < n; i++) {
+        uint32_t d = window_digit(sc + (size_t)i * 8, w, c);
+        if (d) g2_add_aff(bk + (size_t)d * 24, pts + (size_t)i * 16, bk + (size_t)d * 24);
+    }
+    uint64_t *aff = (uint64_t *)malloc((size_t)nb * 128);
+    g2_batch_affine(bk + 24, aff, nb);
+    uint64_t run[24], acc[24];
+    g2_set_id(run);
+    g2_set_id(acc);
+    for (int j = nb - 1; j >= 0; j--) {
+        if (!g2a_is_id(aff + (size_t)j * 16)) g2_add_aff(run, aff + (size_t)j * 16, run);
+        g2_add(acc, run, acc);
+    }
+    memcpy(sum, acc, 192);
+    free(bk);
+    free(aff);
+}
+
+static void *g2_window_thread(void *arg) {
+    job_t *jb = (job_t *)arg;
+    for (int w = jb->first; w < jb->nwin; w += jb->stride)
+        g2_one_window(jb->pts, jb->sc, jb->n, jb->c, w, jb->sums + (size_t)w * 24);
+    return NULL;
+}
+
+/* G2 MSM over n affine bases (16 u64 each, layout above) and n standard-form
+ * 256-bit scalars (8 u32 each); window_bits <= 0 selects
+ * oracle_window_bits(n), threads <= 0 one thread per window.  result:
+ * Jacobian (X, Y, Z), each c0 | c1 in Fp Montgomery form. */
+void oracle_g2_pippenger_msm(const uint64_t *points, const uint32_t *scalars, int n,
+                             int window_bits, int threads, uint64_t result[24]) {
+    if (n <= 0) { g2_set_id(result); return; }
+    int c = window_bits > 0 ? window_bits : oracle_window_bits(n);
+    int nwin = (256 + c - 1) / c;
+    uint64_t *sums = (uint64_t *)malloc((size_t)nwin * 192);
+    int nt = threads > 0 ? threads : nwin;
+    if (nt > nwin) nt = nwin;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nt);
+    job_t *jobs = (job_t *)malloc(sizeof(job_t) * (size_t)nt);
+    for (int t = 0; t < nt; t++) {
+        job_t j = {points, scalars, n, c, nwin, nt, t, sums};
+        jobs[t] = j;
+    }
+    for (int t = 1; t < nt; t++) pthread_create(&th[t], NULL, g2_window_thread, &jobs[t]);
+    g2_window_thread(&jobs[0]);
+    for (int t = 1; t < nt; t++) pthread_join(th[t], NULL);
+    uint64_t acc[24];
+    memcpy(acc, sums + (size_t)(nwin - 1) * 24, 192);
+    for (int w = nwin - 2; w >= 0; w--) {
+        for (int s = 0; s < c; s++) g2_dbl(acc, acc);
+        g2_add(acc, sums + (size_t)w * 24, acc);
+    }
+    memcpy(result, acc, 192);
+    free(sums);
+    free(th);
+    free(jobs);
+}
+
+void oracle_g2_jacobian_to_affine(const uint64_t p[24], uint64_t aff[16]) {
+    if (g2_is_id(p)) { memset(aff, 0, 128); return; }
+    uint64_t zi[8], zi2[8], zi3[8];
+    f2_inv(p + 16, zi);
+    f2_sqr(zi, zi2);
+    f2_mul(zi2, zi, zi3);
+    f2_mul(p, zi2, aff);
+    f2_mul(p + 8, zi3, aff + 8);
+}
+
+void oracle_g2_jac_add(const uint64_t p[24], const uint64_t q[24], uint64_t r[24]) { g2_add(p, q, r); }
+void oracle_g2_jac_dbl(const uint64_t p[24], uint64_t r[24]) { g2_dbl(p, r); }
+void oracle_g2_jac_add_affine(const uint64_t p[24], const uint64_t q[16], uint64_t r[24]) {
+    g2_add_aff(p, q, r);
+}
+#endif /* ORACLE_GRUMPKIN */

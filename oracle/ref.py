@@ -4,7 +4,8 @@ Wraps the C restatement of the reference CPU Pippenger
 (Sources/NeonFieldOps/bn254_msm.c:1145-1187) plus the helper routines the
 tests use.  `curve="grumpkin"` selects oracle/liboracle_grumpkin.so, the same
 source built over Grumpkin's field (grumpkin_ops.c:528-560); its field and
-point helpers then work in BN254 Fr and it has no GLV routines.  Build with
+point helpers then work in BN254 Fr and it has no GLV routines.  The g2_*
+functions wrap the BN254 G2 Pippenger of the same source (liboracle.so only).  Build with
 `make -C oracle`.
 """
 import ctypes
@@ -40,6 +41,12 @@ def lib(curve="bn254"):
         L.oracle_jac_add.argtypes = [u64p, u64p, u64p]
         L.oracle_jac_dbl.argtypes = [u64p, u64p]
         L.oracle_jac_add_affine.argtypes = [u64p, u64p, u64p]
+        if curve == "bn254":
+            L.oracle_g2_pippenger_msm.argtypes = [u64p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, u64p]
+            L.oracle_g2_jacobian_to_affine.argtypes = [u64p, u64p]
+            L.oracle_g2_jac_add.argtypes = [u64p, u64p, u64p]
+            L.oracle_g2_jac_dbl.argtypes = [u64p, u64p]
+            L.oracle_g2_jac_add_affine.argtypes = [u64p, u64p, u64p]
         _LIBS[curve] = L
     return _LIBS[curve]
 
@@ -56,6 +63,35 @@ def pippenger(points_u8, scalars_u8, n, window_bits=0, threads=0, curve="bn254")
     sc = np.ascontiguousarray(sc)
     out = np.zeros(12, dtype=np.uint64)
     lib(curve).oracle_pippenger_msm(_p(pts, ctypes.c_uint64), _p(sc, ctypes.c_uint32), int(n), int(window_bits), int(threads), _p(out, ctypes.c_uint64))
+    return out.tobytes()
+
+
+def g2_pippenger(points_u8, scalars_u8, n, window_bits=0, threads=0):
+    """BN254 G2: points_u8 n*128 bytes (x.c0, x.c1, y.c0, y.c1, Fp Montgomery),
+    scalars_u8 n*32 bytes -> Jacobian (X, Y, Z), each c0 | c1, 192 bytes."""
+    pts = np.ascontiguousarray(np.frombuffer(bytes(points_u8), dtype=np.uint64)
+                               if not isinstance(points_u8, np.ndarray) else points_u8.view(np.uint64))
+    sc = np.ascontiguousarray(np.frombuffer(bytes(scalars_u8), dtype=np.uint32)
+                              if not isinstance(scalars_u8, np.ndarray) else scalars_u8.view(np.uint32))
+    out = np.zeros(24, dtype=np.uint64)
+    lib().oracle_g2_pippenger_msm(_p(pts, ctypes.c_uint64), _p(sc, ctypes.c_uint32), int(n),
+                                  int(window_bits), int(threads), _p(out, ctypes.c_uint64))
+    return out.tobytes()
+
+
+def g2_jacobian_to_affine_bytes(jac192):
+    j = np.frombuffer(bytes(jac192), dtype=np.uint64).copy()
+    out = np.zeros(16, dtype=np.uint64)
+    lib().oracle_g2_jacobian_to_affine(_p(j, ctypes.c_uint64), _p(out, ctypes.c_uint64))
+    return out.tobytes()
+
+
+def g2_point_op(name, *args):
+    """oracle_g2_jac_add / _dbl / _add_affine on byte strings (192-byte
+    Jacobian, 128-byte affine) -> 192-byte Jacobian."""
+    ins = [np.frombuffer(bytes(a), dtype=np.uint64).copy() for a in args]
+    out = np.zeros(24, dtype=np.uint64)
+    getattr(lib(), name)(*[_p(x, ctypes.c_uint64) for x in ins], _p(out, ctypes.c_uint64))
     return out.tobytes()
 
 

@@ -8,7 +8,8 @@ over bases i * G2: 16 points with scalars 1..16, 256 points with the harness
 LCG scalars, 128 points at window width 4.  Those cases are here, with the
 expected value from the closed form (sum s_i k_i mod r) * G2, plus edge
 scalars, identity bases, repeats, cancellation, every window width, batches
-with a zero count, on-device generated bases and 2^16 / 2^18 sizes."""
+with a zero count, on-device generated bases and 2^16 / 2^18 sizes, and a
+skewed-scalar case against the C G2 Pippenger (oracle/bn254_ref.c)."""
 import random
 
 import pytest
@@ -304,3 +305,24 @@ def test_invalid_inputs(engine):
         engine.g2_msm(b"", b"")
     with pytest.raises(zk.MSMError):
         engine.g2_msm(bytes(128), bytes(64))
+
+
+def test_skewed_scalars_against_the_c_pippenger_at_2_14(engine):
+    """Every scalar from a pool of 5 values (a few buckets per window hold
+    thousands of entries each): the G2 MSM against the C restatement of the
+    G2 Pippenger on the downloaded bases."""
+    import numpy as np
+    from oracle import ref
+    n = 1 << 14
+    d = _gen_device(engine, 0x88, n)
+    try:
+        pts = d.download(128 * n)
+    finally:
+        d.free()
+    rnd = random.Random(37)
+    pool = [rnd.randrange(R) for _ in range(4)] + [R - 1]
+    ss = [pool[rnd.randrange(5)] for _ in range(n)]
+    sc = np.frombuffer(sc_bytes(ss), dtype=np.uint8)
+    got = engine.g2_msm(pts, sc.tobytes())
+    want = ref.g2_pippenger(pts, sc.tobytes(), n, 0, 8)
+    assert G.jacobian_bytes_to_affine(got) == G.jacobian_bytes_to_affine(want)
