@@ -221,7 +221,7 @@ def plain(args, eng, peak):
             "peak": float("%.4g" % (peak / 1e12)),
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
-            "traffic": args.traffic,
+            "traffic": _traffic_from_profiles("accumulate_n22_pmc.md") if args.logn == 22 else None,
         },
         "extra": extra,
     }
@@ -283,7 +283,7 @@ def grumpkin(args, eng, peak):
             "peak": float("%.4g" % (peak / 1e12)),
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": _traffic_from_profiles("grumpkin_accumulate_n22_pmc.md") if args.logn == 22 else None,
         },
         "extra": {
             "ms_p50": round(statistics.median(samples), 4),
@@ -353,7 +353,7 @@ def g2(args, eng, peak):
             "peak": float("%.4g" % (peak / 1e12)),
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": _traffic_from_profiles("g2_accumulate_n22_pmc.md") if args.logn == 22 else None,
         },
         "extra": {
             "ms_p50": round(statistics.median(samples), 4),
@@ -561,7 +561,6 @@ def main():
               f"(a multi-GPU MSM would shard points across GPUs and gather one point per GPU)",
               file=sys.stderr)
         sys.exit(2)
-    args.traffic = _traffic_from_profiles()
     eng = zk.MetalMSM()
     args.rates = eng.probe_rates()
     peak = PEAK_INT32_MUL
@@ -579,11 +578,11 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def _traffic_from_profiles():
-    """HBM bytes per k_accumulate launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), recorded in
-    profiles/accumulate_n22_pmc.md as a line 'traffic_bytes_per_launch: N'."""
-    path = os.path.join(ROOT, "profiles", "accumulate_n22_pmc.md")
+def _traffic_from_profiles(name):
+    """HBM bytes per k_accumulate launch at n = 2^22 measured with rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration),
+    recorded in profiles/<name> as a line 'traffic_bytes_per_launch: N'."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             for line in fh:

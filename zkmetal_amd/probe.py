@@ -8,6 +8,8 @@ and written), then --reps plain MSMs of 2^logn points with bases and scalars
 resident in device memory, and prints the per-stage HIP-event timings.
 With --batched: --reps calls of the batched workload instead (64 MSMs over a
 resident 2^20 table, sizes log-uniform in [2^12, 2^20], one call each).
+With --curve grumpkin or g2: the same plain MSMs on Grumpkin G1 or BN254 G2,
+bases generated on the device (k_i * G, k_i * G2).
 """
 import argparse
 import json
@@ -38,6 +40,7 @@ def main():
     ap.add_argument("--window-bits", type=int, default=0)
     ap.add_argument("--dist", type=int, default=0)
     ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--curve", default="bn254", choices=["bn254", "grumpkin", "g2"])
     a = ap.parse_args()
     eng = zk.MetalMSM()
     copy_ms = eng.probe_copy(a.copy_mib << 20)
@@ -48,17 +51,27 @@ def main():
         eng.close()
         return
     n = 1 << a.logn
-    eng.srs_generate(0x5EED, n)
-    bases = eng.srs_download(n)
+    if a.curve == "bn254":
+        eng.srs_generate(0x5EED, n)
+        bases = eng.srs_download(n)
+        dp = zk.DeviceBuffer(bases.nbytes)
+        dp.upload(bases)
+        run = eng.msm_batch_device
+    elif a.curve == "grumpkin":
+        dp = zk.DeviceBuffer(64 * n)
+        eng.grumpkin_points_generate(0x5EED, n, dp)
+        run = eng.grumpkin_msm_batch_device
+    else:
+        dp = zk.DeviceBuffer(128 * n)
+        eng.g2_points_generate(0x5EED, n, dp)
+        run = eng.g2_msm_batch_device
     sc = zk.gen_scalars(0x5EEE, n, a.dist)
-    dp = zk.DeviceBuffer(bases.nbytes)
-    dp.upload(bases)
     ds = zk.DeviceBuffer(sc.nbytes)
     ds.upload(sc)
     if a.window_bits:
         eng.set_window_bits(a.window_bits)
     for r in range(a.reps):
-        eng.msm_batch_device(dp, ds, [n])
+        run(dp, ds, [n])
         print(json.dumps({"rep": r, **{k: round(v, 4) for k, v in eng.last_timing().items()}}), flush=True)
     dp.free()
     ds.free()
