@@ -504,41 +504,36 @@ static void store_s(const uint64_t v[4], uint32_t *s) {
         s[2 * i + 1] = (uint32_t)(v[i] >> 32);
     }
 }
-static void red_r(uint64_t v[4]) {
-    while (geq4(v, RORD)) sub4(v, v, RORD);
-}
-/* (a * b) mod r by shift-and-add over the bits of b, a and b reduced first */
-static void mulmod_r(const uint64_t a0[4], const uint64_t b0[4], uint64_t out[4]) {
-    uint64_t a[4], b[4], acc[4] = {0, 0, 0, 0};
-    memcpy(a, a0, 32);
-    memcpy(b, b0, 32);
-    red_r(a);
-    red_r(b);
-    for (int bit = 255; bit >= 0; bit--) {
-        uint64_t t[4];
-        uint64_t c = add4(t, acc, acc);
-        if (c || geq4(t, RORD)) sub4(t, t, RORD);
-        memcpy(acc, t, 32);
-        if ((b[bit >> 6] >> (bit & 63)) & 1) {
-            c = add4(t, acc, a);
-            if (c || geq4(t, RORD)) sub4(t, t, RORD);
-            memcpy(acc, t, 32);
-        }
-    }
-    memcpy(out, acc, 32);
-}
-
-/* sum_i s_i * k_i mod r, for the closed form of MSMs over bases k_i*G */
+/* sum_i s_i * k_i mod r, for the closed form of MSMs over bases k_i*G: the
+ * 512-bit products summed exactly in 9 words (n < 2^31 adds at most 31
+ * bits), then one reduction of the sum, bit by bit from the top. */
 void oracle_scalar_dot(const uint32_t *s, const uint32_t *k, int n, uint32_t out[8]) {
-    uint64_t acc[4] = {0, 0, 0, 0};
+    uint64_t sum[9] = {0};
     for (int i = 0; i < n; i++) {
-        uint64_t a[4], b[4], p[4], t[4];
+        uint64_t a[4], b[4];
         load_s(s + (size_t)i * 8, a);
         load_s(k + (size_t)i * 8, b);
-        mulmod_r(a, b, p);
-        uint64_t c = add4(t, acc, p);
+        for (int x = 0; x < 4; x++) {
+            uint64_t carry = 0;
+            for (int y = 0; y < 4; y++) {
+                u128 t = (u128)a[x] * b[y] + sum[x + y] + carry;
+                sum[x + y] = (uint64_t)t;
+                carry = (uint64_t)(t >> 64);
+            }
+            for (int z = x + 4; carry && z < 9; z++) {
+                u128 t = (u128)sum[z] + carry;
+                sum[z] = (uint64_t)t;
+                carry = (uint64_t)(t >> 64);
+            }
+        }
+    }
+    uint64_t acc[4] = {0, 0, 0, 0};
+    for (int bit = 9 * 64 - 1; bit >= 0; bit--) {
+        uint64_t t[4], one[4] = {(sum[bit >> 6] >> (bit & 63)) & 1, 0, 0, 0};
+        uint64_t c = add4(t, acc, acc);
         if (c || geq4(t, RORD)) sub4(t, t, RORD);
-        memcpy(acc, t, 32);
+        c = add4(acc, t, one);
+        if (c || geq4(acc, RORD)) sub4(acc, acc, RORD);
     }
     store_s(acc, out);
 }
@@ -571,7 +566,7 @@ void oracle_glv_decompose(const uint32_t kin[8], uint32_t k1o[8], uint32_t k2o[8
                           int *neg2) {
     uint64_t k[4];
     load_s(kin, k);
-    red_r(k);
+    while (geq4(k, RORD)) sub4(k, k, RORD);
     uint64_t p7[7], p6[6];
     mulw(k, 4, G1C, 3, p7);
     mulw(k, 4, G2C, 2, p6);

@@ -1,6 +1,7 @@
 """Write the golden MSM fixtures under tests/golden/ (pure-Python big integers).
 
     python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py --configs   (configs.json, ~1 min)
 
 Each fixture holds the raw ABI inputs (64 B Montgomery affine bases, 32 B
 standard-form scalars) and the expected result in canonical affine
@@ -19,6 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "..", ".."))
 
 from oracle import bn254 as B  # noqa: E402
+from oracle import bn254_g2 as G2  # noqa: E402
+from oracle import grumpkin as GK  # noqa: E402
 
 
 def lcg_scalars(seed, n):
@@ -54,7 +57,45 @@ def fixture(name, points, scalars, note, ks=None):
     }
 
 
+def config_fixtures(seed=0x5EED):
+    """Expected results of the benchmarked configurations, from their seeds:
+    bases k_i * G (G2, Grumpkin G) with k_i = gen_scalar(seed, i, 0), scalars
+    gen_scalar(seed + 1, i, 0), i < n.  Element i does not depend on n, so
+    one pass gives every size as a prefix; the integer sum sum(s_i k_i) is
+    reduced mod r (BN254 G1, G2) or mod p (Grumpkin's group order)."""
+    sizes = {1 << 16: None, 1 << 20: None, 1 << 22: None}
+    acc = 0
+    for i in range(1 << 22):
+        acc += B.uniform_fr(seed, i) * B.uniform_fr(seed + 1, i)
+        if i + 1 in sizes:
+            sizes[i + 1] = acc
+    cases = []
+
+    def g1(pt):
+        return {"inf": pt is None} if pt is None else {"inf": False, "x": "%064x" % pt[0], "y": "%064x" % pt[1]}
+
+    for n, dot in sizes.items():
+        cases.append({"curve": "bn254", "n": n, "expected": g1(B.mul(B.G, dot % B.R_ORDER))})
+    dot = sizes[1 << 22]
+    cases.append({"curve": "grumpkin", "n": 1 << 22, "expected": g1(GK.mul(GK.G, dot % GK.ORDER))})
+    q = G2.mul(G2.G2, dot % G2.R_ORDER)
+    cases.append({"curve": "g2", "n": 1 << 22,
+                  "expected": {"inf": False, "x": ["%064x" % v for v in q[0]], "y": ["%064x" % v for v in q[1]]}})
+    return {
+        "note": "plain MSMs as bench.py runs them: bases k_i * G with k_i = gen_scalars(seed_bases), "
+                "scalars gen_scalars(seed_scalars), uniform in Fr; expected canonical affine (not Montgomery)",
+        "seed_bases": seed,
+        "seed_scalars": seed + 1,
+        "cases": cases,
+    }
+
+
 def main():
+    if "--configs" in sys.argv:
+        with open(os.path.join(HERE, "configs.json"), "w") as fh:
+            json.dump(config_fixtures(), fh, indent=1)
+        print("wrote configs.json")
+        return
     fx = []
     seed_k, seed_s = 0x5EED0001, 0x5EED0002
     ks = [B.uniform_fr(seed_k, i) for i in range(256)]

@@ -169,3 +169,8 @@ def test_c_oracle_scalar_mul_and_dot():
     sb = b"".join(B.scalar_to_bytes(s) for s in ss)
     kb = b"".join(B.scalar_to_bytes(k) for k in ks)
     assert ref.scalar_dot(sb, kb, 40) == sum(s * k for s, k in zip(ss, ks)) % B.R_ORDER
+    # the exact 576-bit sum's carries: 1000 products of 2^256 - 1 with itself,
+    # and the empty sum
+    top = B.scalar_to_bytes(2**256 - 1) * 1000
+    assert ref.scalar_dot(top, top, 1000) == 1000 * (2**256 - 1) ** 2 % B.R_ORDER
+    assert ref.scalar_dot(b"", b"", 0) == 0
