@@ -60,3 +60,21 @@ def closed_form(ks, ss):
     for s, k in zip(ss, ks):
         acc += s * k
     return B.mul(B.G, acc % B.R_ORDER)
+
+
+def config_fixture():
+    """tests/golden/configs.json: seeds and expected affine results of the
+    benchmarked configurations."""
+    with open(os.path.join(GOLDEN, "configs.json")) as fh:
+        return json.load(fh)
+
+
+def config_expected_point(case):
+    """Expected affine point of one configs.json case (None = identity); G2
+    coordinates as (c0, c1) pairs."""
+    e = case["expected"]
+    if e["inf"]:
+        return None
+    if case["curve"] == "g2":
+        return (tuple(int(v, 16) for v in e["x"]), tuple(int(v, 16) for v in e["y"]))
+    return (int(e["x"], 16), int(e["y"], 16))

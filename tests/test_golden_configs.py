@@ -4,9 +4,6 @@ seeds) recomputed a second way on the CPU: the library's host scalar
 generator (zkmsm_gen_scalars) and the C oracle's sum(s_i k_i) mod the group
 order, then one scalar multiple of the generator.  The GPU side of the same
 fixture is in tests/test_gpu_configs.py."""
-import json
-import os
-
 import pytest
 
 import zkmetal_amd as zk
@@ -14,22 +11,7 @@ from oracle import bn254 as B
 from oracle import bn254_g2 as G2
 from oracle import grumpkin as GK
 from oracle import ref
-
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "configs.json")
-
-
-def load():
-    with open(GOLDEN) as fh:
-        return json.load(fh)
-
-
-def expected_point(case):
-    e = case["expected"]
-    if e["inf"]:
-        return None
-    if case["curve"] == "g2":
-        return (tuple(int(v, 16) for v in e["x"]), tuple(int(v, 16) for v in e["y"]))
-    return (int(e["x"], 16), int(e["y"], 16))
+from helpers import config_expected_point as expected_point, config_fixture as load
 
 
 def closed_form(case, seed_bases, seed_scalars):

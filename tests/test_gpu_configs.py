@@ -7,8 +7,7 @@ import pytest
 import zkmetal_amd as zk
 from oracle import bn254_g2 as G2
 from oracle import grumpkin as GK
-from helpers import result_affine
-from test_golden_configs import expected_point, load
+from helpers import config_expected_point as expected_point, config_fixture as load, result_affine
 
 pytestmark = pytest.mark.gpu
 
