@@ -19,6 +19,9 @@ inputs, timed in the same run.
                                                      serving workloads)
     python bench.py --config grumpkin              (the same MSM on Grumpkin G1)
     python bench.py --config g2                    (the same MSM on BN254 G2)
+    python bench.py --profile [--config ...] --steps K  (profiling run for
+                                                     rocprofv3: copy-kernel
+                                                     calibration, then K calls)
 """
 import argparse
 import json
@@ -222,6 +225,9 @@ def plain(args, eng, peak):
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
             "traffic": _traffic_from_profiles("accumulate_n22_pmc.md") if args.logn == 22 else None,
+            # traffic is not measured in this run: it is the figure recorded in
+            # the named profile (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)
+            "traffic_source": "profiles/accumulate_n22_pmc.md" if args.logn == 22 else None,
         },
         "extra": extra,
     }
@@ -284,6 +290,9 @@ def grumpkin(args, eng, peak):
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
             "traffic": _traffic_from_profiles("grumpkin_accumulate_n22_pmc.md") if args.logn == 22 else None,
+            # traffic is not measured in this run: it is the figure recorded in
+            # the named profile (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)
+            "traffic_source": "profiles/grumpkin_accumulate_n22_pmc.md" if args.logn == 22 else None,
         },
         "extra": {
             "ms_p50": round(statistics.median(samples), 4),
@@ -354,6 +363,9 @@ def g2(args, eng, peak):
             "unit": "T int32 products/s",
             "frac": round(achieved / peak, 4),
             "traffic": _traffic_from_profiles("g2_accumulate_n22_pmc.md") if args.logn == 22 else None,
+            # traffic is not measured in this run: it is the figure recorded in
+            # the named profile (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)
+            "traffic_source": "profiles/g2_accumulate_n22_pmc.md" if args.logn == 22 else None,
         },
         "extra": {
             "ms_p50": round(statistics.median(samples), 4),
@@ -371,22 +383,42 @@ def g2(args, eng, peak):
     return line
 
 
-def batched(args, eng, peak):
-    """64 MSMs sharing a 2^20 resident table, sizes log-uniform in [2^12, 2^20]."""
+def batched(args, eng, peak, kind=None):
+    """kind "batched": 64 MSMs sharing a 2^20 resident table, sizes log-uniform
+    in [2^12, 2^20], uniform scalars; "skewed": 32 MSMs of 2^18 witness-like
+    scalars.  Each step is one call holding every MSM, replayed from a
+    captured hipGraph (graph mode; --no-graph: eager launches), against the
+    same MSMs as back-to-back plain calls (eager, one MSM per call).  Returns
+    the result line and what the closed-form check needs."""
+    kind = kind or args.config
     rnd = np.random.default_rng(args.seed)
     n_tab = 1 << 20
-    counts = [int(2 ** u) for u in rnd.uniform(12, 20, size=64)] if args.config == "batched" \
+    counts = [int(2 ** u) for u in rnd.uniform(12, 20, size=64)] if kind == "batched" \
         else [1 << 18] * 32
-    dist = 0 if args.config == "batched" else 1
+    dist = 0 if kind == "batched" else 1
     eng.srs_generate(args.seed, n_tab)
     scal = zk.gen_scalars(args.seed + 2, sum(counts), dist)
     dsc = zk.DeviceBuffer(scal.nbytes)
     dsc.upload(scal)
-    ms_b, _ = timed_steps(lambda: eng.msm_batch_resident(dsc, counts), args.steps, args.warmup)
-    batched_res = eng.msm_batch_resident(dsc, counts)
+    # stage split from one eager call (a graph replay has one device span)
+    eng.set_graph_mode(False)
+    eager_res = eng.msm_batch_resident(dsc, counts)
     tb = eng.last_timing()
     stage = {k: round(tb[k], 4) for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms",
                                            "reduce_ms", "copyout_ms", "device_ms", "host_tail_ms")}
+    graph = not args.no_graph
+    eng.set_graph_mode(graph)
+    caps0 = eng.graph_captures()
+    res = {}
+
+    def step():
+        res["r"] = eng.msm_batch_resident(dsc, counts)
+
+    ms_b, samples = timed_steps(step, args.steps, args.warmup)
+    batched_res = res["r"]
+    captures = eng.graph_captures() - caps0
+    device_ms = eng.last_timing()["device_ms"]
+    eng.set_graph_mode(False)
     # back-to-back plain launches of the same MSMs on the same device
     offs = np.concatenate([[0], np.cumsum(counts)])
     views = []
@@ -403,9 +435,9 @@ def batched(args, eng, peak):
     for v in views:
         v.free()
     dsc.free()
-    same = seq_res == batched_res
-    return {
-        "metric": f"{'batched' if args.config == 'batched' else 'skewed batched'} BN254 G1 MSMs/s",
+    same = seq_res == batched_res and eager_res == batched_res
+    line = {
+        "metric": f"{'batched' if kind == 'batched' else 'skewed batched'} BN254 G1 MSMs/s",
         "value": round(len(counts) / (ms_b * 1e-3), 2),
         "unit": "MSMs/s",
         "n_gpus": 1,
@@ -418,14 +450,20 @@ def batched(args, eng, peak):
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"{len(counts)} MSMs over a resident 2^20 table, sizes {min(counts)}..{max(counts)}, "
-                               f"scalars {'uniform Fr' if dist == 0 else 'witness-like mix'}",
-                   "msms": len(counts), "points": int(sum(counts))},
-        "extra": {"stage_ms": stage, "entries_per_lane": tb["entries_per_lane"],
+                               f"scalars {'uniform Fr' if dist == 0 else 'witness-like mix'}, "
+                               f"{'one hipGraph replay' if graph else 'one eager launch sequence'} per call",
+                   "msms": len(counts), "points": int(sum(counts)), "graph": graph},
+        "extra": {"ms_p50": round(statistics.median(samples), 4),
+                  "device_ms": round(device_ms, 4), "graph_captures": captures,
+                  "stage_ms_eager": stage, "entries_per_lane": tb["entries_per_lane"],
                   "back_to_back_ms": round(ms_s, 3),
                   "back_to_back_msms_per_s": round(len(counts) / (ms_s * 1e-3), 2),
                   "speedup_vs_back_to_back": round(ms_s / ms_b, 3),
                   "results_identical": bool(same)},
     }
+    check = {"srs_seed": args.seed, "srs_n": n_tab, "counts": counts, "scalars": scal,
+             "results": batched_res}
+    return line, check
 
 
 def serve(args, eng):
@@ -497,12 +535,14 @@ def serve(args, eng):
     # bit-exact: a sample of requests recomputed alone equals the served result
     check = sorted(rnd.choice(nreq, size=min(12, nreq), replace=False).tolist())
     same = all(eng.msm_batch_resident(views[i], [sizes[i]])[0] == results[i][0] for i in check)
+    closed = {"srs_seed": args.seed, "srs_n": n_srs, "pool_seed": args.seed + 3,
+              "requests": [(offs[i], sizes[i], results[i][0]) for i in check]}
     lat = sorted(r[1] for r in results)
     big = [r[1] for r, n in zip(results, sizes) if n >= (1 << 20)]
     del views, pool2d
     pool.free()
     return {
-        "metric": "served BN254 G1 MSMs/s at 200 req/s offered (p50/p99 latency)",
+        "metric": f"served BN254 G1 MSMs/s at {args.rate:g} req/s offered (p50/p99 latency)",
         "value": round(st["completed"] / wall, 2),
         "unit": "MSMs/s",
         "n_gpus": 1,
@@ -534,7 +574,59 @@ def serve(args, eng):
                                  "mean_batch": round(bst["mean_batch"], 3),
                                  "p50_ms": round(bst["p50_ms"], 3), "p99_ms": round(bst["p99_ms"], 3),
                                  "results_identical": bool(burst_same)}},
-    }
+    }, closed
+
+
+def profile(args, eng):
+    """Profiling run for rocprofv3 (kernel traces, PMC counters): one plain
+    copy kernel over --copy-mib MiB (known traffic: that many bytes read and
+    written, the HBM counters' calibration), then --steps calls of the
+    workload with their stage timings, one JSON line each.  --config plain,
+    grumpkin or g2: plain MSMs of 2^logn points, bases and scalars resident;
+    batched: the 64-MSM call over a resident 2^20 table."""
+    copy_ms = eng.probe_copy(args.copy_mib << 20)
+    print(json.dumps({"copy_bytes": args.copy_mib << 20, "copy_ms": round(copy_ms, 4)}), flush=True)
+    n = 1 << args.logn
+    bufs = []
+    if args.config in ("batched", "skewed"):
+        rnd = np.random.default_rng(args.seed)
+        counts = [int(2 ** u) for u in rnd.uniform(12, 20, size=64)] if args.config == "batched" \
+            else [1 << 18] * 32
+        eng.srs_generate(args.seed, 1 << 20)
+        sc = zk.gen_scalars(args.seed + 2, sum(counts), 0 if args.config == "batched" else 1)
+        ds = zk.DeviceBuffer(sc.nbytes)
+        ds.upload(sc)
+        bufs.append(ds)
+
+        def run():
+            eng.msm_batch_resident(ds, counts)
+    else:
+        if args.config == "grumpkin":
+            dp = zk.DeviceBuffer(64 * n)
+            eng.grumpkin_points_generate(args.seed, n, dp)
+            fn = eng.grumpkin_msm_batch_device
+        elif args.config == "g2":
+            dp = zk.DeviceBuffer(128 * n)
+            eng.g2_points_generate(args.seed, n, dp)
+            fn = eng.g2_msm_batch_device
+        else:
+            eng.srs_generate(args.seed, n)
+            bases = eng.srs_download(n)
+            dp = zk.DeviceBuffer(bases.nbytes)
+            dp.upload(bases)
+            fn = eng.msm_batch_device
+        sc = zk.gen_scalars(args.seed + 1, n, 0)
+        ds = zk.DeviceBuffer(sc.nbytes)
+        ds.upload(sc)
+        bufs += [dp, ds]
+
+        def run():
+            fn(dp, ds, [n])
+    for r in range(args.steps):
+        run()
+        print(json.dumps({"step": r, **{k: round(v, 4) for k, v in eng.last_timing().items()}}), flush=True)
+    for b in bufs:
+        b.free()
 
 
 def main():
@@ -555,6 +647,16 @@ def main():
     ap.add_argument("--skip-2-20", action="store_true")
     ap.add_argument("--skip-host-calls", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling run (rocprofv3): a calibrated copy kernel, then --steps calls of "
+                         "the workload with their stage timings; no JSON result line")
+    ap.add_argument("--copy-mib", type=int, default=256, help="profile: copy kernel size")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="plain: skip the batched / skewed / served legs")
+    ap.add_argument("--no-graph", action="store_true", help="batched calls as eager launches")
+    ap.add_argument("--leg-steps", type=int, default=5, help="timed calls of the batched legs")
+    ap.add_argument("--leg-requests", type=int, default=300, help="requests of the served leg")
+    ap.add_argument("--leg-burst", type=int, default=200, help="saturation requests of the served leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
@@ -562,20 +664,79 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     eng = zk.MetalMSM()
+    if args.profile:
+        profile(args, eng)
+        eng.close()
+        return
     args.rates = eng.probe_rates()
     peak = PEAK_INT32_MUL
+    checks = []
     if args.config == "plain":
         line = plain(args, eng, peak)
+        if not args.no_legs:
+            # the three system workloads, measured in the same run: batched and
+            # skewed calls replayed from captured hipGraphs, and a served stream
+            legs = {}
+            la = argparse.Namespace(**vars(args))
+            la.steps, la.warmup = args.leg_steps, 2
+            for kind in ("batched", "skewed"):
+                leg, chk = batched(la, eng, peak, kind)
+                legs[kind] = _leg_summary(leg)
+                checks.append((kind, chk))
+            la.requests, la.burst = args.leg_requests, args.leg_burst
+            leg, chk = serve(la, eng)
+            legs["serve"] = _leg_summary(leg)
+            checks.append(("serve", chk))
+            line["extra"]["system_legs"] = legs
     elif args.config == "serve":
-        line = serve(args, eng)
+        line, chk = serve(args, eng)
+        checks.append(("serve", chk))
     elif args.config == "grumpkin":
         line = grumpkin(args, eng, peak)
     elif args.config == "g2":
         line = g2(args, eng, peak)
     else:
-        line = batched(args, eng, peak)
+        line, chk = batched(args, eng, peak)
+        checks.append((args.config, chk))
     eng.close()
+    if checks and not args.no_cpu:
+        # part of the CPU leg: the system workloads' results against the closed
+        # form, after every timed region
+        line.setdefault("cpu_baseline", {})["closed_form_checks"] = closed_form_checks(checks)
     print(json.dumps(line), flush=True)
+
+
+def _leg_summary(leg):
+    keep = ("metric", "value", "unit", "steps", "warmup", "ms_per_step", "config", "extra")
+    return {k: leg[k] for k in keep if k in leg}
+
+
+def closed_form_checks(checks):
+    """Checker (CPU leg, after the timed regions): each checked result of the
+    system workloads against (sum s_i k_i mod r) * G of the resident bases
+    k_i * G, with the oracle's exact dot product and scalar multiplication."""
+    from oracle import bn254 as B
+    from oracle import ref
+
+    out = {}
+    t0 = time.perf_counter()
+    for kind, chk in checks:
+        ks = zk.gen_scalars(chk["srs_seed"], chk["srs_n"], 0).tobytes()
+        if kind == "serve":
+            pool = zk.gen_scalars(chk["pool_seed"], chk["srs_n"], 1).tobytes()
+            items = [(pool[32 * o:32 * (o + n)], n, r) for o, n, r in chk["requests"]]
+        else:
+            sb = chk["scalars"].tobytes()
+            offs = np.concatenate([[0], np.cumsum(chk["counts"])])
+            items = [(sb[32 * int(offs[j]):32 * int(offs[j + 1])], n, chk["results"][j])
+                     for j, n in enumerate(chk["counts"])]
+        ok = 0
+        for sc, n, r in items:
+            want = B.mul(B.G, ref.scalar_dot(sc, ks[:32 * n], n))
+            ok += B.jacobian_bytes_to_affine(r[:32], r[32:64], r[64:96]) == want
+        out[kind] = {"checked": len(items), "equal_closed_form": int(ok)}
+    out["seconds"] = round(time.perf_counter() - t0, 2)
+    return out
 
 
 def _traffic_from_profiles(name):

@@ -200,11 +200,23 @@ ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32
  * in zkmsm_last_timing are then 0 except the device total.  A call of a new
  * shape captures again.  zkmsm_graph_captures counts the captures. */
 ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
+
+/* Cap on the device memory an engine's buffers may grow to (its resident SRS
+ * included; 0 = no cap, the default), e.g. to share one GPU between
+ * engines.  A call that would need more returns ZKMETAL_ERR_ALLOC_FAILED and
+ * leaves the engine usable; zkmsm_workspace_bytes reports the bytes held.
+ * No counterpart in the reference (its engine grows its buffers without a
+ * bound, Sources/zkMetal/MSM/MSMEngine.swift:232-277). */
+ZkMetalStatus zkmsm_set_workspace_limit(ZkMetalMSMEngine engine, uint64_t bytes);
+uint64_t zkmsm_workspace_bytes(ZkMetalMSMEngine engine);
 uint64_t zkmsm_graph_captures(ZkMetalMSMEngine engine);
 
 /* ---- request server ------------------------------------------------------ */
-/* Continuous batching of MSM requests over the resident table of an engine
- * (which must keep that table and outlive the server).  `slots` engines, each
+/* Continuous batching of MSM requests over the resident table of an engine.
+ * While a server exists, that engine's zkmsm_srs_upload / _generate /
+ * _load_ptau[_file] return ZKMETAL_ERR_INVALID_INPUT (the table stays as the
+ * server sees it), and zkmetal_msm_engine_destroy on it defers the release
+ * until the last server is destroyed.  `slots` engines, each
  * with its own stream and workspace, take queued requests in arrival order
  * (at most max_batch per launch) while their estimated device footprint fits
  * the free HBM budget; batches on different slots overlap on the device.
@@ -246,6 +258,12 @@ uint64_t zkmsm_server_footprint(uint32_t n);
 /* dist 0: uniform in Fr; 1: 40% 0, 30% < 2^16, 10% in {1, r-1}, 20% uniform;
  * 2: uniform 256-bit.  out: n x 32 B. */
 void zkmsm_gen_scalars(uint64_t seed, uint64_t n, int32_t dist, uint8_t* out);
+
+/* ---- DIAGNOSTICS: not part of the MSM API ------------------------------- */
+/* The zkmsm_test_* and zkmsm_probe_* entry points below exist for the test
+ * suite and for measurement (primitive parity tests on host buffers, the
+ * roofline peak and the HBM counter calibration).  A caller of the MSM path
+ * never needs them; their signatures may change between versions. */
 
 /* ---- primitive probes for GPU unit tests (host buffers) ------------------ */
 /* op 0 mul, 1 add, 2 sub, 3 inverse(a), 4 negate(a), 5 canonicalise(a);

@@ -77,44 +77,113 @@ static const uint64_t RORD[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL,
 
 /* ---------------- Fp ---------------- */
 
-static void f_mul(const uint64_t a[4], const uint64_t b[4], uint64_t out[4]) {
-    /* CIOS: for each word of b, t += a*b_i, then t = (t + m*p) / 2^64 */
-    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 4; i++) {
-        uint64_t carry = 0;
-        for (int j = 0; j < 4; j++) {
-            u128 acc = (u128)a[j] * b[i] + t[j] + carry;
-            t[j] = (uint64_t)acc;
-            carry = (uint64_t)(acc >> 64);
-        }
-        u128 top = (u128)t[4] + carry;
-        t[4] = (uint64_t)top;
-        t[5] = (uint64_t)(top >> 64);
-        uint64_t m = t[0] * QINV;
-        u128 acc = (u128)m * Q[0] + t[0];
-        carry = (uint64_t)(acc >> 64);
-        for (int j = 1; j < 4; j++) {
-            acc = (u128)m * Q[j] + t[j] + carry;
-            t[j - 1] = (uint64_t)acc;
-            carry = (uint64_t)(acc >> 64);
-        }
-        top = (u128)t[4] + carry;
-        t[3] = (uint64_t)top;
-        t[4] = t[5] + (uint64_t)(top >> 64);
-    }
-    /* conditional subtraction of p */
+/* t (5 words, value < 2p) -> t mod p in out; keeps t when t < p */
+static inline __attribute__((always_inline)) void f_final_sub(const uint64_t t[5], uint64_t out[4]) {
     uint64_t d[4];
     uint64_t borrow = 0;
+#pragma GCC unroll 4
     for (int j = 0; j < 4; j++) {
         u128 s = (u128)t[j] - Q[j] - borrow;
         d[j] = (uint64_t)s;
         borrow = (uint64_t)(s >> 127) & 1;
     }
-    if (t[4] || !borrow) memcpy(out, d, 32);
-    else memcpy(out, t, 32);
+    const int keep = !t[4] && borrow;
+#pragma GCC unroll 4
+    for (int j = 0; j < 4; j++) out[j] = keep ? t[j] : d[j];
 }
 
-static void f_sqr(const uint64_t a[4], uint64_t out[4]) { f_mul(a, a, out); }
+/* Montgomery product a*b/2^256 mod p, CIOS with the row loops fully unrolled
+ * (bn254_msm.c:38-154 unrolls the same four rounds by hand).  Round i adds
+ * a*b_i to the running value and then m*p with m = t_0 * (-p^-1) mod 2^64,
+ * shifting one word down; the running value stays below 2p (p < 2^254), so
+ * one spare word t[4] holds every carry. */
+static inline __attribute__((always_inline)) void f_mul(const uint64_t a[4], const uint64_t b[4],
+                                                        uint64_t out[4]) {
+    uint64_t t[5] = {0, 0, 0, 0, 0};
+#pragma GCC unroll 4
+    for (int i = 0; i < 4; i++) {
+        const uint64_t bi = b[i];
+        u128 acc = (u128)a[0] * bi + t[0];
+        t[0] = (uint64_t)acc;
+#pragma GCC unroll 3
+        for (int j = 1; j < 4; j++) {
+            acc = (u128)a[j] * bi + t[j] + (uint64_t)(acc >> 64);
+            t[j] = (uint64_t)acc;
+        }
+        const uint64_t hi = t[4] + (uint64_t)(acc >> 64);
+        const uint64_t m = t[0] * QINV;
+        acc = (u128)m * Q[0] + t[0];
+#pragma GCC unroll 3
+        for (int j = 1; j < 4; j++) {
+            acc = (u128)m * Q[j] + t[j] + (uint64_t)(acc >> 64);
+            t[j - 1] = (uint64_t)acc;
+        }
+        acc = (u128)hi + (uint64_t)(acc >> 64);
+        t[3] = (uint64_t)acc;
+        t[4] = (uint64_t)(acc >> 64);
+    }
+    f_final_sub(t, out);
+}
+
+/* Montgomery square a^2/2^256 mod p with its own product schedule (the
+ * reference has a dedicated squaring too, bn254_msm.c:156-267): the six
+ * cross products a_i a_j (i < j) once, the 512-bit sum doubled by a one-bit
+ * shift, the four squares a_i^2 added on the diagonal (10 word products
+ * instead of 16), then four word-by-word Montgomery reductions of the
+ * 8-word value. */
+static inline __attribute__((always_inline)) void f_sqr(const uint64_t a[4], uint64_t out[4]) {
+    uint64_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    /* cross products, one row per i: w[i+j] += a_i a_j for j > i */
+#pragma GCC unroll 3
+    for (int i = 0; i < 3; i++) {
+        uint64_t c = 0;
+#pragma GCC unroll 3
+        for (int j = i + 1; j < 4; j++) {
+            const u128 x = (u128)a[i] * a[j] + w[i + j] + c;
+            w[i + j] = (uint64_t)x;
+            c = (uint64_t)(x >> 64);
+        }
+        w[i + 4] = c;
+    }
+    /* double */
+    w[7] = w[6] >> 63;
+#pragma GCC unroll 5
+    for (int k = 6; k > 1; k--) w[k] = (w[k] << 1) | (w[k - 1] >> 63);
+    w[1] <<= 1;
+    /* diagonal */
+    uint64_t c = 0;
+#pragma GCC unroll 4
+    for (int i = 0; i < 4; i++) {
+        const u128 sq = (u128)a[i] * a[i];
+        u128 x = (u128)w[2 * i] + (uint64_t)sq + c;
+        w[2 * i] = (uint64_t)x;
+        x = (u128)w[2 * i + 1] + (uint64_t)(sq >> 64) + (uint64_t)(x >> 64);
+        w[2 * i + 1] = (uint64_t)x;
+        c = (uint64_t)(x >> 64);
+    }
+    /* reduce: w[i..] += m_i p, then the low word is zero */
+    uint64_t top = c;  /* 0: a^2 < p^2 < 2^508 */
+#pragma GCC unroll 4
+    for (int i = 0; i < 4; i++) {
+        const uint64_t m = w[i] * QINV;
+        u128 x = (u128)m * Q[0] + w[i];
+#pragma GCC unroll 3
+        for (int j = 1; j < 4; j++) {
+            x = (u128)m * Q[j] + w[i + j] + (uint64_t)(x >> 64);
+            w[i + j] = (uint64_t)x;
+        }
+        /* ripple the carry into the upper words */
+        uint64_t cy = (uint64_t)(x >> 64);
+        for (int k = i + 4; k < 8 && cy; k++) {
+            const u128 y = (u128)w[k] + cy;
+            w[k] = (uint64_t)y;
+            cy = (uint64_t)(y >> 64);
+        }
+        top += cy;
+    }
+    uint64_t t[5] = {w[4], w[5], w[6], w[7], top};
+    f_final_sub(t, out);
+}
 
 static int f_is_zero(const uint64_t a[4]) { return (a[0] | a[1] | a[2] | a[3]) == 0; }
 
@@ -445,6 +514,7 @@ void oracle_jacobian_to_affine(const uint64_t p[12], uint64_t aff[8]) {
 /* ---------------- helpers used by tests ---------------- */
 
 void oracle_fp_mul(const uint64_t a[4], const uint64_t b[4], uint64_t r[4]) { f_mul(a, b, r); }
+void oracle_fp_sqr(const uint64_t a[4], uint64_t r[4]) { f_sqr(a, r); }
 void oracle_fp_add(const uint64_t a[4], const uint64_t b[4], uint64_t r[4]) { f_add(a, b, r); }
 void oracle_fp_sub(const uint64_t a[4], const uint64_t b[4], uint64_t r[4]) { f_sub(a, b, r); }
 void oracle_fp_inv(const uint64_t a[4], uint64_t r[4]) { f_inv(a, r); }

@@ -36,7 +36,7 @@ def lib(curve="bn254"):
         L.oracle_window_bits.argtypes = [ctypes.c_int]
         for name in ("oracle_fp_mul", "oracle_fp_add", "oracle_fp_sub"):
             getattr(L, name).argtypes = [u64p, u64p, u64p]
-        for name in ("oracle_fp_inv", "oracle_to_mont", "oracle_from_mont"):
+        for name in ("oracle_fp_inv", "oracle_fp_sqr", "oracle_to_mont", "oracle_from_mont"):
             getattr(L, name).argtypes = [u64p, u64p]
         L.oracle_jac_add.argtypes = [u64p, u64p, u64p]
         L.oracle_jac_dbl.argtypes = [u64p, u64p]

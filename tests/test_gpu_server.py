@@ -91,3 +91,47 @@ def test_invalid_requests(srs_engine):
     with pytest.raises(zk.MSMError):
         srv.wait(123456)
     srv.close()
+
+
+def test_srs_replacement_refused_while_a_server_reads_it():
+    """A server reads its owner's resident table: while it exists, the calls
+    that would replace that table are refused and the server keeps serving
+    the original bases; after it is closed they work again."""
+    e = zk.MetalMSM()
+    try:
+        e.srs_generate(SEED, 1 << 12)
+        bases = e.srs_download(1 << 12)
+        sc = zk.gen_scalars(0x11FE, 1000, 0)
+        want = e.msm_batch_resident(sc, [1000])[0]
+        srv = zk.Server(e, slots=1)
+        try:
+            for fn in (lambda: e.srs_generate(SEED + 1, 1 << 13),
+                       lambda: e.srs_upload(bases[:64 * 100]),
+                       lambda: e.srs_load_ptau(data=b"ptau" + bytes(64))):
+                with pytest.raises(zk.MSMError) as ei:
+                    fn()
+                assert ei.value.status == zk.ERR_INVALID_INPUT
+            r, _ = srv.wait(srv.submit(sc))
+            assert r == want
+        finally:
+            srv.close()
+        e.srs_generate(SEED + 1, 1 << 12)  # no server: allowed again
+        assert e.msm_batch_resident(sc, [1000])[0] != want
+    finally:
+        e.close()
+
+
+def test_engine_handle_released_before_its_server():
+    """Destroying the SRS engine's handle while a server is attached defers
+    the release: the server still computes over the table."""
+    e = zk.MetalMSM()
+    e.srs_generate(SEED, 1 << 12)
+    sc = zk.gen_scalars(0x11FF, 3000, 1)
+    want = e.msm_batch_resident(sc, [3000])[0]
+    srv = zk.Server(e, slots=2)
+    e.close()
+    try:
+        r, _ = srv.wait(srv.submit(sc))
+        assert r == want
+    finally:
+        srv.close()

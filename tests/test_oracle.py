@@ -113,6 +113,21 @@ def test_c_field_ops_match_python():
         assert ref.fp_op("oracle_from_mont", am) == a
 
 
+def test_c_squaring_and_multiply_edges():
+    """The C squaring (its own product schedule) and the unrolled CIOS product
+    against Python integers, on random values and on the values that stress
+    the carries: 0, 1, p-1, p-2, all-ones words below p, powers of two."""
+    rnd = random.Random(12)
+    edges = [0, 1, 2, B.P - 1, B.P - 2, (B.P - 1) // 2, (1 << 253), (1 << 254) - 1 - ((1 << 254) - B.P),
+             int("ffffffffffffffff", 16), int("ffffffffffffffff" * 3, 16)]
+    vals = [v % B.P for v in edges] + [rnd.randrange(B.P) for _ in range(300)]
+    for a in vals:
+        am = B.to_mont(a)
+        assert ref.fp_op("oracle_fp_sqr", am) == B.to_mont(a * a % B.P)
+        b = rnd.choice(vals)
+        assert ref.fp_op("oracle_fp_mul", am, B.to_mont(b)) == B.to_mont(a * b % B.P)
+
+
 def _c_msm_affine(fx, window_bits=0, threads=0):
     pts = bytes.fromhex(fx["points"])
     sc = bytes.fromhex(fx["scalars"])

@@ -86,6 +86,7 @@ def test_c_field_is_fr():
         a, b = rnd.randrange(G.R), rnd.randrange(G.R)
         got = ref.fp_op("oracle_fp_mul", G.to_mont(a), G.to_mont(b), curve="grumpkin")
         assert got == G.to_mont(a * b % G.R)
+        assert ref.fp_op("oracle_fp_sqr", G.to_mont(a), curve="grumpkin") == G.to_mont(a * a % G.R)
         assert ref.fp_op("oracle_fp_sub", a, b, curve="grumpkin") == (a - b) % G.R
     # R mod r and R^2 mod r agree with the reference's literal one (grumpkin_ops.c:22-25)
     assert ref.fp_op("oracle_to_mont", 1, curve="grumpkin") == B.int_from_limbs(G.REF_ONE_LIMBS64, bits=64)

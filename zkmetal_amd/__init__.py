@@ -92,6 +92,8 @@ def lib():
         "zkmsm_test_fp": (i32, [vp, i32, vp, vp, vp, u32]),
         "zkmsm_test_xyzz": (i32, [vp, i32, vp, vp, vp, u32]),
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
+        "zkmsm_set_workspace_limit": (i32, [vp, u64]),
+        "zkmsm_workspace_bytes": (u64, [vp]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_set_window_table": (i32, [vp, vp, u32]),
@@ -147,6 +149,31 @@ def _buf(x):
 def _check(rc, what=""):
     if rc != SUCCESS:
         raise MSMError(rc, what)
+
+
+def _nbytes(x):
+    """bytes held by a host buffer (bytes-like / numpy) or a DeviceBuffer"""
+    if isinstance(x, DeviceBuffer):
+        return x.nbytes
+    if isinstance(x, np.ndarray):
+        return x.nbytes
+    return memoryview(x).nbytes
+
+
+def _counts(counts):
+    """counts as a contiguous uint32 array; negative or >= 2^32 is an error"""
+    c = np.asarray(counts, dtype=np.int64).reshape(-1)
+    if c.size and (c.min() < 0 or c.max() >= 1 << 32):
+        raise MSMError(ERR_INVALID_INPUT, "counts out of range")
+    return np.ascontiguousarray(c.astype(np.uint32))
+
+
+def _need(buf, need, what):
+    """The C ABI reads sum(counts) elements from each buffer: refuse a call
+    whose buffer is shorter, before any device work is launched."""
+    have = _nbytes(buf)
+    if need > have:
+        raise MSMError(ERR_INVALID_INPUT, f"{what}: buffer holds {have} bytes, the counts need {need}")
 
 
 def gpu_available():
@@ -260,7 +287,10 @@ class MetalMSM:
         return bytes(out)
 
     def msm_small(self, points, scalars, width):
+        if width not in (1, 2, 4):
+            raise MSMError(ERR_INVALID_INPUT, "msm_small: width is 1, 2 or 4 bytes")
         n = _count(points, 64)
+        _need(scalars, n * width, "msm_small scalars")
         pp, k1 = _buf(points)
         sp, k2 = _buf(scalars)
         out = (ctypes.c_uint8 * 96)()
@@ -271,7 +301,10 @@ class MetalMSM:
 
     def msm_batch(self, points, scalars, counts):
         """zkmetal_bn254_msm_batch: concatenated inputs -> list of 96-byte results."""
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(points, total * 64, "msm_batch points")
+        _need(scalars, total * 32, "msm_batch scalars")
         pp, k1 = _buf(points)
         sp, k2 = _buf(scalars)
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
@@ -295,7 +328,10 @@ class MetalMSM:
         return bytes(out)
 
     def grumpkin_msm_batch(self, points, scalars, counts):
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(points, total * 64, "grumpkin_msm_batch points")
+        _need(scalars, total * 32, "grumpkin_msm_batch scalars")
         pp, k1 = _buf(points)
         sp, k2 = _buf(scalars)
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
@@ -306,7 +342,10 @@ class MetalMSM:
 
     def grumpkin_msm_batch_device(self, dpoints, dscalars, counts):
         """Grumpkin MSMs with bases and scalars in DeviceBuffers."""
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(dpoints, total * 64, "grumpkin_msm_batch_device points")
+        _need(dscalars, total * 32, "grumpkin_msm_batch_device scalars")
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
         _check(lib().zkmsm_grumpkin_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
                                                      ctypes.c_void_p(counts.ctypes.data), len(counts),
@@ -336,7 +375,10 @@ class MetalMSM:
         return out.tobytes()
 
     def g2_msm_batch(self, points, scalars, counts):
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(points, total * 128, "g2_msm_batch points")
+        _need(scalars, total * 32, "g2_msm_batch scalars")
         pp, k1 = _buf(points)
         sp, k2 = _buf(scalars)
         out = np.zeros(len(counts) * 192, dtype=np.uint8)
@@ -346,7 +388,10 @@ class MetalMSM:
         return [out[i * 192:(i + 1) * 192].tobytes() for i in range(len(counts))]
 
     def g2_msm_batch_device(self, dpoints, dscalars, counts):
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(dpoints, total * 128, "g2_msm_batch_device points")
+        _need(dscalars, total * 32, "g2_msm_batch_device scalars")
         out = np.zeros(len(counts) * 192, dtype=np.uint8)
         _check(lib().zkmsm_bn254_g2_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
                                                      ctypes.c_void_p(counts.ctypes.data), len(counts),
@@ -408,7 +453,10 @@ class MetalMSM:
 
     def msm_batch_device(self, dpoints, dscalars, counts):
         """Plain MSMs with bases and scalars in DeviceBuffers."""
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        total = int(counts.sum(dtype=np.uint64))
+        _need(dpoints, total * 64, "msm_batch_device points")
+        _need(dscalars, total * 32, "msm_batch_device scalars")
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
         _check(lib().zkmsm_msm_batch_device(self.handle, dpoints.ptr, dscalars.ptr,
                                             ctypes.c_void_p(counts.ctypes.data), len(counts),
@@ -430,7 +478,8 @@ class MetalMSM:
         return out
 
     def msm_batch_resident(self, scalars, counts):
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        _need(scalars, int(counts.sum(dtype=np.uint64)) * 32, "msm_batch_resident scalars")
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
         if isinstance(scalars, DeviceBuffer):
             rc = lib().zkmsm_msm_batch_resident_device(self.handle, scalars.ptr, ctypes.c_void_p(counts.ctypes.data),
@@ -458,7 +507,8 @@ class MetalMSM:
         """KZG commitments over the resident SRS (KZGEngine.batchCommit): each
         polynomial's coefficients are Fr elements in Montgomery form (32 B LE),
         concatenated; one 96-byte X|Y|Z result per polynomial."""
-        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+        counts = _counts(counts)
+        _need(coeffs_mont, int(counts.sum(dtype=np.uint64)) * 32, "kzg_commit scalars")
         out = np.zeros(len(counts) * 96, dtype=np.uint8)
         if isinstance(coeffs_mont, DeviceBuffer):
             rc = lib().zkmsm_kzg_commit_device(self.handle, coeffs_mont.ptr, ctypes.c_void_p(counts.ctypes.data),
@@ -481,6 +531,14 @@ class MetalMSM:
         """Capture each call's device launch sequence in a hipGraph and replay
         it for later calls of the same shape."""
         _check(lib().zkmsm_set_graph_mode(self.handle, 1 if on else 0), "set_graph_mode")
+
+    def set_workspace_limit(self, nbytes):
+        """Cap the device memory this engine's buffers may grow to (0 = none);
+        a call needing more raises MSMError(ERR_ALLOC_FAILED)."""
+        _check(lib().zkmsm_set_workspace_limit(self.handle, nbytes), "set_workspace_limit")
+
+    def workspace_bytes(self):
+        return int(lib().zkmsm_workspace_bytes(self.handle))
 
     def graph_captures(self):
         return int(lib().zkmsm_graph_captures(self.handle))

@@ -88,7 +88,9 @@ ZkMetalStatus zkmetal_msm_engine_create(ZkMetalMSMEngine* out) {
     return rc;
 }
 
-void zkmetal_msm_engine_destroy(ZkMetalMSMEngine engine) { delete as_engine(engine); }
+// a request server created on this engine keeps it alive until the server is
+// destroyed (Engine::release)
+void zkmetal_msm_engine_destroy(ZkMetalMSMEngine engine) { Engine::release(as_engine(engine)); }
 
 ZkMetalStatus zkmetal_bn254_msm(ZkMetalMSMEngine engine, const uint8_t* points,
                                 const uint8_t* scalars, uint32_t n_points, uint8_t* result_x,
@@ -508,6 +510,21 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
     return e->set_window_table(c_by_log2n, len, false);
 }
 
+ZkMetalStatus zkmsm_set_workspace_limit(ZkMetalMSMEngine engine, uint64_t bytes) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    e->set_workspace_limit(bytes);
+    return ZKMETAL_SUCCESS;
+}
+
+uint64_t zkmsm_workspace_bytes(ZkMetalMSMEngine engine) {
+    Engine* e = as_engine(engine);
+    if (!e) return 0;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->workspace_bytes();
+}
+
 ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
@@ -544,11 +561,8 @@ ZkMetalStatus zkmsm_server_create(ZkMetalMSMEngine srs_engine, const ZkmsmServer
         if (cfg->hbm_budget_bytes) sc.hbm_budget = cfg->hbm_budget_bytes;
     }
     std::lock_guard<std::mutex> lk(e->mutex());
-    const void* table = e->srs_table();
-    const uint32_t n = e->srs_size();
-    if (!table || n == 0) return ZKMETAL_ERR_INVALID_INPUT;
     zk::Server* s = nullptr;
-    int32_t rc = zk::Server::create(table, n, sc, &s, e);  // slots take e's window tables
+    int32_t rc = zk::Server::create(e, sc, &s);  // slots take e's window tables
     *out = s;
     return rc;
 }

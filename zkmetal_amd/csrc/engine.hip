@@ -35,8 +35,9 @@ namespace zk {
 
 hipError_t DevBuf::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
-    release();
     size_t want = std::max<size_t>(bytes, 256);
+    if (acct && acct->limit && acct->used - cap + want > acct->limit) return hipErrorOutOfMemory;
+    release();
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
@@ -45,11 +46,13 @@ hipError_t DevBuf::ensure(size_t bytes) {
         return e;
     }
     cap = want;
+    if (acct) acct->used += want;
     return hipSuccess;
 }
 
 void DevBuf::release() {
     if (p) (void)hipFree(p);
+    if (acct) acct->used -= cap;
     p = nullptr;
     cap = 0;
 }
@@ -197,6 +200,7 @@ int32_t Engine::create(Engine** out) {
         delete e;
         return kGpuError;
     }
+    e->account_buffers();
     e->acc_cap_ = accumulate_capacity(e->dev_);
     e->acc_cap_g2_ = accumulate_capacity(e->dev_, CurveId::Bn254G2);
     memcpy(e->ctab_, kTunedPlain, sizeof(e->ctab_));
@@ -209,6 +213,13 @@ int32_t Engine::create(Engine** out) {
     }
     *out = e;
     return kOk;
+}
+
+void Engine::account_buffers() {
+    for (DevBuf* b : {&desc_, &digits_, &keysA_, &valsA_, &keysB_, &valsB_, &hist_, &part_, &segvalid_,
+                      &buckets_, &recAk_, &recAp_, &recBk_, &rc_, &winsum_, &err_, &pts_in_, &ptable_,
+                      &scalars_, &srs_, &gtab_, &gtab_gk_, &gtab_g2_, &tmp_})
+        b->acct = &ws_;
 }
 
 Engine::~Engine() {
@@ -411,6 +422,9 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     hipStream_t st = stream_;
 
     const uint64_t bits_bytes = (uint64_t)nseg * 16 * hpb;
+    // merge lists: a listed run has more than kRunLimit records, so each list
+    // holds at most nrec / (kRunLimit + 1) runs
+    const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
     uint8_t* hw = (uint8_t*)hwin_.p;
     // the device part of the call: one launch sequence on the engine stream.
     // Stage events are recorded only outside graph capture.
@@ -436,9 +450,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                           buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
         ZK_TRY(mark(3));
         // bucket partials that crossed lane boundaries: short runs by their first
-        // lane, longer runs (heavy buckets) by lane groups.  A listed run has more
-        // than kRunLimit records, so each list holds at most nrec / (kRunLimit+1).
-        const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
+        // lane, longer runs (heavy buckets) by lane groups.
         launch_merge_runs(pl.curve, (const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
                           d_err + 1, (uint32_t*)recBk_.p, list_cap, st);
         ZK_TRY(mark(4));
@@ -446,7 +458,9 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                              winsum_.p, st);
         ZK_TRY(mark(5));
         ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
-        ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 4, hipMemcpyDeviceToHost, st));
+        // err_[0]: decomposition error flag; err_[1..2]: long / mid run counts of
+        // the merge (a count above list_cap means listed runs were dropped)
+        ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 12, hipMemcpyDeviceToHost, st));
         ZK_TRY(hipMemcpyAsync(hw + bits_bytes + 16, segvalid_.p, (uint64_t)nseg * 4,
                               hipMemcpyDeviceToHost, st));
         return kOk;
@@ -506,9 +520,9 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     }
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
-    uint32_t err;
-    memcpy(&err, hw + bits_bytes, 4);
-    if (err) return kGpuError;
+    uint32_t err[3];
+    memcpy(err, hw + bits_bytes, 12);
+    if (err[0] || err[1] > list_cap || err[2] > list_cap) return kGpuError;
 
     for (int s = 0; s < kNumStages; s++) {
         float ms = 0;
@@ -690,6 +704,10 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
 
 int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     if (!points || n == 0) return kInvalidInput;
+    if (servers_) return kInvalidInput;  // a server reads the current table
+    // no SRS until the new table is complete: a failure below leaves none
+    srs_n_ = 0;
+    srs_host_.clear();
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
     ZK_ALLOC(srs_, table_bytes(n));
     ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
@@ -710,6 +728,7 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
 int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_points,
                               uint32_t* n_loaded) {
     if (!data || len < 12) return kInvalidInput;
+    if (servers_) return kInvalidInput;  // a server reads the current table
     auto rd32 = [&](uint64_t off) {
         return (uint32_t)data[off] | ((uint32_t)data[off + 1] << 8) | ((uint32_t)data[off + 2] << 16) |
                ((uint32_t)data[off + 3] << 24);
@@ -781,11 +800,12 @@ static int32_t build_gtab(const zkh::Fp& gx, const zkh::Fp& gy, DevBuf& dst) {
         memcpy(&blob[i * 64], xs[i].v, 32);
         memcpy(&blob[i * 64 + 32], ys[i].v, 32);
     }
-    DevBuf tmp;
-    if (tmp.ensure(blob.size()) != hipSuccess) return kAllocFailed;
-    ZK_TRY(hipMemcpy(tmp.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    std::swap(dst.p, tmp.p);
-    std::swap(dst.cap, tmp.cap);
+    // dst stays empty (not built) unless the upload completes
+    if (dst.ensure(blob.size()) != hipSuccess) return kAllocFailed;
+    if (hipMemcpy(dst.p, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        dst.release();
+        return kGpuError;
+    }
     return kOk;
 }
 
@@ -847,11 +867,11 @@ int32_t Engine::ensure_gtab_g2() {
         const zkh::Fp* c[4] = {&x.c0, &x.c1, &y.c0, &y.c1};
         for (int k = 0; k < 4; k++) memcpy(&blob[i * 128 + 32 * k], c[k]->v, 32);
     }
-    DevBuf tmp;
-    if (tmp.ensure(blob.size()) != hipSuccess) return kAllocFailed;
-    ZK_TRY(hipMemcpy(tmp.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    std::swap(gtab_g2_.p, tmp.p);
-    std::swap(gtab_g2_.cap, tmp.cap);
+    if (gtab_g2_.ensure(blob.size()) != hipSuccess) return kAllocFailed;
+    if (hipMemcpy(gtab_g2_.p, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        gtab_g2_.release();
+        return kGpuError;
+    }
     return kOk;
 }
 
@@ -885,6 +905,9 @@ int32_t Engine::grumpkin_points_generate(uint64_t seed, uint32_t n, void* d_out)
 
 int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
     if (n == 0) return kInvalidInput;
+    if (servers_) return kInvalidInput;  // a server reads the current table
+    srs_n_ = 0;  // no SRS until the new table is complete
+    srs_host_.clear();
     int rc = ensure_gtab();
     if (rc != kOk) return rc;
     std::vector<uint8_t> ks((uint64_t)n * 32);

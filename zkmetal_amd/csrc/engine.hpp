@@ -7,6 +7,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -24,10 +25,18 @@ enum Status : int32_t {
     kAllocFailed = -4,
 };
 
-// grow-only device buffer
+// device bytes held by one engine's buffers, and an optional cap on them
+struct WsAccount {
+    uint64_t limit = 0;  // 0: no cap
+    uint64_t used = 0;
+};
+
+// grow-only device buffer; with an account, a growth that would take the
+// account past its limit fails (hipErrorOutOfMemory) and keeps the old buffer
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    WsAccount* acct = nullptr;
     hipError_t ensure(size_t bytes);
     void release();
     ~DevBuf() { release(); }
@@ -92,6 +101,18 @@ class Engine {
   public:
     static int32_t create(Engine** out);
     ~Engine();
+    // Shared ownership: a request server attached to this engine's SRS holds
+    // a reference, so destroying the caller's handle first leaves the table
+    // alive until the last server is gone.  release() deletes at zero.
+    void retain() { refs_.fetch_add(1, std::memory_order_relaxed); }
+    static void release(Engine* e) {
+        if (e && e->refs_.fetch_sub(1, std::memory_order_acq_rel) == 1) delete e;
+    }
+    // servers reading the resident table (engine mutex held by the caller):
+    // while any is attached, the srs_* calls that replace the table refuse
+    void attach_server() { servers_++; }
+    void detach_server() { servers_--; }
+    uint32_t attached_servers() const { return servers_; }
 
     // host buffers in, host results out (96 B Jacobian X|Y|Z per MSM)
     int32_t msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t n, uint8_t* out96);
@@ -162,6 +183,11 @@ class Engine {
                                           CurveId cv = CurveId::Bn254) const;
 
     void set_window_bits(uint32_t c) { c_override_ = c; }
+    // cap on the device memory this engine's buffers may grow to (resident
+    // SRS included; 0 = none): a call that needs more returns kAllocFailed
+    // and leaves the engine usable
+    void set_workspace_limit(uint64_t bytes) { ws_.limit = bytes; }
+    uint64_t workspace_bytes() const { return ws_.used; }
     // graph mode: the device launch sequence of a call is captured into a
     // hipGraph and replayed while calls keep the same shape and buffers
     void set_graph_mode(bool on) { graph_mode_ = on; }
@@ -196,6 +222,10 @@ class Engine {
     int dev_ = 0;
     hipStream_t stream_ = nullptr;
     std::mutex mu_;
+    std::atomic<int32_t> refs_{1};
+    uint32_t servers_ = 0;
+    WsAccount ws_;
+    void account_buffers();  // ties every member DevBuf to ws_
     uint32_t c_override_ = 0;
     uint32_t ctab_[33] = {};
     uint32_t ctab_batch_[33] = {};

@@ -18,10 +18,12 @@ uint64_t Server::footprint(uint32_t n) {
     return (uint64_t)n * 32 + 2ull * n * w * 16 + w * (1ull << (c - 1)) * kBucketBytes + (1u << 20);
 }
 
-int32_t Server::create(const void* table, uint32_t table_n, const ServerConfig& cfg,
-                       Server** out, const Engine* tuning) {
-    if (!out || !table || table_n == 0 || cfg.slots == 0 || cfg.max_batch == 0) return kInvalidInput;
+int32_t Server::create(Engine* owner, const ServerConfig& cfg, Server** out) {
+    if (!out || !owner || cfg.slots == 0 || cfg.max_batch == 0) return kInvalidInput;
     *out = nullptr;
+    const void* table = owner->srs_table();
+    const uint32_t table_n = owner->srs_size();
+    if (!table || table_n == 0) return kInvalidInput;
     Server* s = new Server();
     s->table_ = table;
     s->table_n_ = table_n;
@@ -33,9 +35,12 @@ int32_t Server::create(const void* table, uint32_t table_n, const ServerConfig& 
             delete s;
             return rc;
         }
-        if (tuning) e->copy_tuning_from(*tuning);
+        e->copy_tuning_from(*owner);
         s->engines_.push_back(e);
     }
+    owner->attach_server();
+    owner->retain();
+    s->owner_ = owner;
     for (uint32_t i = 0; i < cfg.slots; i++) s->threads_.emplace_back(&Server::worker, s, i);
     *out = s;
     return kOk;
@@ -51,6 +56,13 @@ Server::~Server() {
     for (auto& t : threads_) t.join();
     for (Engine* e : engines_) delete e;
     for (auto& kv : reqs_) delete kv.second;
+    if (owner_) {
+        {
+            std::lock_guard<std::mutex> lk(owner_->mutex());
+            owner_->detach_server();
+        }
+        Engine::release(owner_);
+    }
 }
 
 int32_t Server::submit(const uint8_t* scalars, uint32_t n, uint64_t* ticket, bool copy) {

@@ -39,9 +39,12 @@ struct ServerStats {
 
 class Server {
   public:
-    // slot engines take their window tables from `tuning` when given
-    static int32_t create(const void* table, uint32_t table_n, const ServerConfig& cfg,
-                          Server** out, const Engine* tuning = nullptr);
+    // Serves the resident SRS of `owner` (whose mutex the caller holds):
+    // the server attaches to it (the owner's srs_* calls that would replace
+    // the table refuse until the server is destroyed) and holds a reference
+    // (the owner lives until the server is gone).  Slot engines take the
+    // owner's window tables.
+    static int32_t create(Engine* owner, const ServerConfig& cfg, Server** out);
     ~Server();
 
     // copies the scalars (or, with copy == false, reads them in place: the
@@ -69,6 +72,7 @@ class Server {
     Server() = default;
     void worker(size_t slot);
 
+    Engine* owner_ = nullptr;
     const void* table_ = nullptr;
     uint32_t table_n_ = 0;
     ServerConfig cfg_;
