@@ -405,7 +405,8 @@ def batched(args, eng, peak, kind=None):
     eager_res = eng.msm_batch_resident(dsc, counts)
     tb = eng.last_timing()
     stage = {k: round(tb[k], 4) for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms",
-                                           "reduce_ms", "copyout_ms", "device_ms", "host_tail_ms")}
+                                           "reduce_ms", "device_tail_ms", "copyout_ms", "device_ms",
+                                           "host_tail_ms")}
     graph = not args.no_graph
     eng.set_graph_mode(graph)
     caps0 = eng.graph_captures()

@@ -177,7 +177,9 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
  * out[9]: window bits of the first MSM; out[10]: entries per accumulation
  * lane; out[11]: buckets per reduction group; out[12]: sorted entries;
  * out[13]: accumulation lanes; out[14]: entries with a nonzero digit (the
- * mixed additions performed).  Returns the number of values written. */
+ * mixed additions performed); out[15]: device time of the window combine and
+ * normalisation when the device ran them (0 when the host did, see
+ * zkmsm_set_tail_mode).  Returns the number of values written. */
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
 /* Per-size-class window widths: c_by_log2n[ceil(log2 n)] for 0 <= index < len
@@ -200,6 +202,15 @@ ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32
  * in zkmsm_last_timing are then 0 except the device total.  A call of a new
  * shape captures again.  zkmsm_graph_captures counts the captures. */
 ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
+
+/* Where a call's window combine (Horner over the windows) and affine
+ * normalisation run: 0 auto (the device in graph mode and for calls of 16 or
+ * more MSMs, the host otherwise), 1 the host, 2 the device.  Results are the
+ * same bytes either way.  BN254 G2 calls always use the host.  The device
+ * form lets a graph replay end with final results; the host is faster for a
+ * single MSM (the combine is a serial chain of ~127 doublings, which one
+ * host core runs faster than one GPU lane). */
+ZkMetalStatus zkmsm_set_tail_mode(ZkMetalMSMEngine engine, int32_t mode);
 
 /* Cap on the device memory an engine's buffers may grow to (its resident SRS
  * included; 0 = no cap, the default), e.g. to share one GPU between

@@ -78,7 +78,7 @@ static const uint64_t RORD[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL,
 /* ---------------- Fp ---------------- */
 
 /* t (5 words, value < 2p) -> t mod p in out; keeps t when t < p */
-static inline __attribute__((always_inline)) void f_final_sub(const uint64_t t[5], uint64_t out[4]) {
+static void f_final_sub(const uint64_t t[5], uint64_t out[4]) {
     uint64_t d[4];
     uint64_t borrow = 0;
 #pragma GCC unroll 4
@@ -97,7 +97,7 @@ static inline __attribute__((always_inline)) void f_final_sub(const uint64_t t[5
  * a*b_i to the running value and then m*p with m = t_0 * (-p^-1) mod 2^64,
  * shifting one word down; the running value stays below 2p (p < 2^254), so
  * one spare word t[4] holds every carry. */
-static inline __attribute__((always_inline)) void f_mul(const uint64_t a[4], const uint64_t b[4],
+static void f_mul(const uint64_t a[4], const uint64_t b[4],
                                                         uint64_t out[4]) {
     uint64_t t[5] = {0, 0, 0, 0, 0};
 #pragma GCC unroll 4
@@ -131,7 +131,7 @@ static inline __attribute__((always_inline)) void f_mul(const uint64_t a[4], con
  * shift, the four squares a_i^2 added on the diagonal (10 word products
  * instead of 16), then four word-by-word Montgomery reductions of the
  * 8-word value. */
-static inline __attribute__((always_inline)) void f_sqr(const uint64_t a[4], uint64_t out[4]) {
+static void f_sqr(const uint64_t a[4], uint64_t out[4]) {
     uint64_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     /* cross products, one row per i: w[i+j] += a_i a_j for j > i */
 #pragma GCC unroll 3
@@ -161,8 +161,11 @@ static inline __attribute__((always_inline)) void f_sqr(const uint64_t a[4], uin
         w[2 * i + 1] = (uint64_t)x;
         c = (uint64_t)(x >> 64);
     }
-    /* reduce: w[i..] += m_i p, then the low word is zero */
-    uint64_t top = c;  /* 0: a^2 < p^2 < 2^508 */
+    (void)c;  /* 0: a^2 < p^2 < 2^508 */
+    /* reduce: w[i..i+3] += m_i p (the low word becomes zero); the round's
+     * carry goes into w[i+4], whose own overflow bit enters the next round's
+     * last word */
+    uint64_t extra = 0;
 #pragma GCC unroll 4
     for (int i = 0; i < 4; i++) {
         const uint64_t m = w[i] * QINV;
@@ -172,16 +175,11 @@ static inline __attribute__((always_inline)) void f_sqr(const uint64_t a[4], uin
             x = (u128)m * Q[j] + w[i + j] + (uint64_t)(x >> 64);
             w[i + j] = (uint64_t)x;
         }
-        /* ripple the carry into the upper words */
-        uint64_t cy = (uint64_t)(x >> 64);
-        for (int k = i + 4; k < 8 && cy; k++) {
-            const u128 y = (u128)w[k] + cy;
-            w[k] = (uint64_t)y;
-            cy = (uint64_t)(y >> 64);
-        }
-        top += cy;
+        x = (u128)w[i + 4] + (uint64_t)(x >> 64) + extra;
+        w[i + 4] = (uint64_t)x;
+        extra = (uint64_t)(x >> 64);
     }
-    uint64_t t[5] = {w[4], w[5], w[6], w[7], top};
+    uint64_t t[5] = {w[4], w[5], w[6], w[7], extra};
     f_final_sub(t, out);
 }
 

@@ -93,6 +93,7 @@ def lib():
         "zkmsm_test_xyzz": (i32, [vp, i32, vp, vp, vp, u32]),
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
         "zkmsm_set_workspace_limit": (i32, [vp, u64]),
+        "zkmsm_set_tail_mode": (i32, [vp, i32]),
         "zkmsm_workspace_bytes": (u64, [vp]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
@@ -252,7 +253,7 @@ class HostBuffer:
 
 TIMING_KEYS = ["decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "copyout_ms",
                "device_ms", "host_tail_ms", "wall_ms", "window_bits", "entries_per_lane",
-               "buckets_per_group", "entries", "lanes", "valid_entries"]
+               "buckets_per_group", "entries", "lanes", "valid_entries", "device_tail_ms"]
 
 
 class MetalMSM:
@@ -532,6 +533,13 @@ class MetalMSM:
         it for later calls of the same shape."""
         _check(lib().zkmsm_set_graph_mode(self.handle, 1 if on else 0), "set_graph_mode")
 
+    TAIL_AUTO, TAIL_HOST, TAIL_DEVICE = 0, 1, 2
+
+    def set_tail_mode(self, mode):
+        """Where the window combine and normalisation run: TAIL_AUTO (device in
+        graph mode and for calls of >= 16 MSMs), TAIL_HOST or TAIL_DEVICE."""
+        _check(lib().zkmsm_set_tail_mode(self.handle, mode), "set_tail_mode")
+
     def set_workspace_limit(self, nbytes):
         """Cap the device memory this engine's buffers may grow to (0 = none);
         a call needing more raises MSMError(ERR_ALLOC_FAILED)."""
@@ -547,8 +555,8 @@ class MetalMSM:
         _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")
 
     def last_timing(self):
-        v = (ctypes.c_double * 15)()
-        k = lib().zkmsm_last_timing(self.handle, v, 15)
+        v = (ctypes.c_double * 16)()
+        k = lib().zkmsm_last_timing(self.handle, v, 16)
         return {TIMING_KEYS[i]: v[i] for i in range(k)}
 
     def probe_rates(self):

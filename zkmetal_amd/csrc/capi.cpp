@@ -435,7 +435,7 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     if (!e || !out) return 0;
     std::lock_guard<std::mutex> lk(e->mutex());
     const zk::Timing& t = e->last_timing();
-    double v[15];
+    double v[16];
     for (int s = 0; s < zk::kNumStages; s++) v[s] = t.stage_ms[s];
     v[6] = t.device_ms;
     v[7] = t.host_tail_ms;
@@ -446,7 +446,8 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     v[12] = (double)t.entries;
     v[13] = (double)t.lanes;
     v[14] = (double)t.valid_entries;
-    int32_t k = max_values < 15 ? max_values : 15;
+    v[15] = t.device_tail_ms;
+    int32_t k = max_values < 16 ? max_values : 16;
     for (int32_t i = 0; i < k; i++) out[i] = v[i];
     return k;
 }
@@ -508,6 +509,13 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->set_window_table(c_by_log2n, len, false);
+}
+
+ZkMetalStatus zkmsm_set_tail_mode(ZkMetalMSMEngine engine, int32_t mode) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->set_tail_mode(mode);
 }
 
 ZkMetalStatus zkmsm_set_workspace_limit(ZkMetalMSMEngine engine, uint64_t bytes) {

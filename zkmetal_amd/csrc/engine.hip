@@ -96,9 +96,27 @@ uint32_t default_window_bits(uint64_t n, CurveId cv) {
     return best;
 }
 
+// empty plan that keeps its arrays' capacity (the no-allocation path plans
+// into arrays reserved at engine creation)
+static void plan_reset(Plan& pl) {
+    pl.msms.clear();
+    pl.msm_scalar0.clear();
+    pl.msm_dtile0.clear();
+    pl.segs.clear();
+    pl.seg_lane0.clear();
+    pl.seg_grp0.clear();
+    pl.seg_rcw0.clear();
+    pl.tile_seg.clear();
+    pl.chunk_seg.clear();
+    pl.scalars = pl.entries = pl.buckets = pl.lanes = pl.groups = pl.rc_waves = 0;
+    pl.tiles = pl.chunks = pl.T = pl.dtiles = 0;
+    pl.curve = CurveId::Bn254;
+    pl.G = 8;
+}
+
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
               Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv) {
-    pl = Plan();
+    plan_reset(pl);
     pl.curve = cv;
     const uint32_t eps = entries_per_point(cv);
     pl.msms.resize(nmsm);
@@ -211,14 +229,24 @@ int32_t Engine::create(Engine** out) {
             return kGpuError;
         }
     }
+    if (hipEventCreate(&e->ev_tail_) != hipSuccess) {
+        delete e;
+        return kGpuError;
+    }
+    for (auto& ev : e->stage_ev_) {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            delete e;
+            return kGpuError;
+        }
+    }
     *out = e;
     return kOk;
 }
 
 void Engine::account_buffers() {
     for (DevBuf* b : {&desc_, &digits_, &keysA_, &valsA_, &keysB_, &valsB_, &hist_, &part_, &segvalid_,
-                      &buckets_, &recAk_, &recAp_, &recBk_, &rc_, &winsum_, &err_, &pts_in_, &ptable_,
-                      &scalars_, &srs_, &gtab_, &gtab_gk_, &gtab_g2_, &tmp_})
+                      &buckets_, &recAk_, &recAp_, &recBk_, &rc_, &winsum_, &err_, &wins_, &res_,
+                      &pts_in_, &ptable_, &scalars_, &srs_, &gtab_, &gtab_gk_, &gtab_g2_, &tmp_})
         b->acct = &ws_;
 }
 
@@ -226,6 +254,9 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto& ev : ev_)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ev_tail_) (void)hipEventDestroy(ev_tail_);
+    for (auto& ev : stage_ev_)
         if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -370,35 +401,102 @@ static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out) {
     HC::finish(res, out);
 }
 
+WsSizes ws_sizes(const Plan& pl, bool dev_tail) {
+    WsSizes z{};
+    const uint64_t nseg = pl.segs.size(), nmsm = pl.msms.size();
+    const uint64_t ptb = point_bytes(pl.curve), hpb = host_point_bytes(pl.curve);
+    const uint64_t rec_cap = 2 * pl.lanes + 2;
+    z.desc = layout_for(pl).total;
+    z.digits = pl.entries * 4;
+    z.keys = pl.entries * 2;
+    z.vals = pl.entries * 4;
+    z.hist = (uint64_t)pl.tiles * 256 * 4;
+    z.part = (uint64_t)pl.chunks * 4;
+    z.segvalid = nseg * 4;
+    z.buckets = pl.buckets * ptb;
+    z.rec_keys = rec_cap * 4;
+    z.rec_pts = rec_cap * ptb;
+    z.rc = pl.groups * ptb;
+    // bit-plane points: host form for the host tail, stored form for the device's
+    z.bits = nseg * 16 * (dev_tail ? ptb : hpb);
+    z.err = 16;
+    z.wins = dev_tail ? nseg * ptb : 0;
+    z.res = dev_tail ? nmsm * result_bytes(pl.curve) : 0;
+    // copied back: the results (device tail) or the bit planes, then err, then
+    // the per-segment valid-entry counts
+    z.back = dev_tail ? z.res : z.bits;
+    z.host = z.back + 16 + nseg * 4;
+    const uint64_t parts[] = {z.desc, z.digits, z.keys, z.vals, z.keys, z.vals, z.hist, z.part,
+                              z.segvalid, z.buckets, z.rec_keys, z.rec_pts, z.rec_keys, z.rc, z.bits,
+                              z.err, z.wins, z.res};
+    for (uint64_t b : parts) z.device += (b + 255) & ~uint64_t(255);
+    return z;
+}
+
 int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
                     uint8_t* out96) {
+    const uint32_t nmsm = (uint32_t)pl.msms.size();
+    // window combine on the device (graph mode, large batches) or the host
+    const bool dev_tail = pl.curve != CurveId::Bn254G2 &&
+                          (tail_mode_ == kTailDevice ||
+                           (tail_mode_ == kTailAuto && (graph_mode_ || nmsm >= kDeviceTailMinMsms)));
+    const WsSizes z = ws_sizes(pl, dev_tail);
+    ZK_ALLOC(desc_, z.desc);
+    ZK_ALLOC(hdesc_, z.desc);
+    ZK_ALLOC(digits_, z.digits);
+    ZK_ALLOC(keysA_, z.keys);
+    ZK_ALLOC(valsA_, z.vals);
+    ZK_ALLOC(keysB_, z.keys);
+    ZK_ALLOC(valsB_, z.vals);
+    ZK_ALLOC(hist_, z.hist);
+    ZK_ALLOC(part_, z.part);
+    ZK_ALLOC(segvalid_, z.segvalid);
+    ZK_ALLOC(buckets_, z.buckets);
+    ZK_ALLOC(recAk_, z.rec_keys);
+    ZK_ALLOC(recAp_, z.rec_pts);
+    ZK_ALLOC(recBk_, z.rec_keys);
+    ZK_ALLOC(rc_, z.rc);
+    ZK_ALLOC(winsum_, z.bits);
+    if (dev_tail) {
+        ZK_ALLOC(wins_, z.wins);
+        ZK_ALLOC(res_, z.res);
+    }
+    ZK_ALLOC(err_, z.err);
+    ZK_ALLOC(hwin_, z.host);
+    Workspace ws;
+    ws.desc = (uint8_t*)desc_.p;
+    ws.digits = digits_.p;
+    ws.keysA = keysA_.p;
+    ws.valsA = valsA_.p;
+    ws.keysB = keysB_.p;
+    ws.valsB = valsB_.p;
+    ws.hist = hist_.p;
+    ws.part = part_.p;
+    ws.segvalid = segvalid_.p;
+    ws.buckets = buckets_.p;
+    ws.recAk = recAk_.p;
+    ws.recAp = recAp_.p;
+    ws.recBk = recBk_.p;
+    ws.rc = rc_.p;
+    ws.bits = winsum_.p;
+    ws.err = err_.p;
+    ws.wins = wins_.p;
+    ws.res = res_.p;
+    ws.hdesc = (uint8_t*)hdesc_.p;
+    ws.hw = (uint8_t*)hwin_.p;
+    return run_ws(pl, ws, z, dev_tail, graph_mode_, d_scalars, d_table, table_n, out96);
+}
+
+int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, bool graph,
+                       const void* d_scalars, const void* d_table, uint64_t table_n, uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
     const uint32_t nmsm = (uint32_t)pl.msms.size();
     const uint32_t nseg = (uint32_t)pl.segs.size();
     const DescLayout L = layout_for(pl);
+    const uint64_t ptb = point_bytes(pl.curve);
+    const uint64_t back_bytes = z.back;
 
-    const uint64_t rec_cap = 2 * pl.lanes + 2;
-    ZK_ALLOC(desc_, L.total);
-    ZK_ALLOC(hdesc_, L.total);
-    ZK_ALLOC(digits_, pl.entries * 4);
-    ZK_ALLOC(keysA_, pl.entries * 2);
-    ZK_ALLOC(valsA_, pl.entries * 4);
-    ZK_ALLOC(keysB_, pl.entries * 2);
-    ZK_ALLOC(valsB_, pl.entries * 4);
-    ZK_ALLOC(hist_, (uint64_t)pl.tiles * 256 * 4);
-    ZK_ALLOC(part_, (uint64_t)pl.chunks * 4);
-    ZK_ALLOC(segvalid_, (uint64_t)nseg * 4);
-    const uint64_t ptb = point_bytes(pl.curve), hpb = host_point_bytes(pl.curve);
-    ZK_ALLOC(buckets_, pl.buckets * ptb);
-    ZK_ALLOC(recAk_, rec_cap * 4);
-    ZK_ALLOC(recAp_, rec_cap * ptb);
-    ZK_ALLOC(recBk_, rec_cap * 4);
-    ZK_ALLOC(rc_, pl.groups * ptb);
-    ZK_ALLOC(winsum_, (uint64_t)nseg * 16 * hpb);
-    ZK_ALLOC(err_, 16);
-    ZK_ALLOC(hwin_, (uint64_t)nseg * (16 * hpb + 4) + 16);
-
-    uint8_t* hb = (uint8_t*)hdesc_.p;
+    uint8_t* hb = ws.hdesc;
     memcpy(hb + L.msms, pl.msms.data(), pl.msms.size() * sizeof(MsmDesc));
     memcpy(hb + L.msm_scalar0, pl.msm_scalar0.data(), pl.msm_scalar0.size() * 8);
     memcpy(hb + L.msm_dtile0, pl.msm_dtile0.data(), pl.msm_dtile0.size() * 8);
@@ -408,7 +506,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     memcpy(hb + L.seg_rcw0, pl.seg_rcw0.data(), pl.seg_rcw0.size() * 8);
     memcpy(hb + L.tile_seg, pl.tile_seg.data(), pl.tile_seg.size() * 4);
     memcpy(hb + L.chunk_seg, pl.chunk_seg.data(), pl.chunk_seg.size() * 4);
-    uint8_t* db = (uint8_t*)desc_.p;
+    uint8_t* db = ws.desc;
     const MsmDesc* d_msms = (const MsmDesc*)(db + L.msms);
     const uint64_t* d_msm_scalar0 = (const uint64_t*)(db + L.msm_scalar0);
     const uint64_t* d_msm_dtile0 = (const uint64_t*)(db + L.msm_dtile0);
@@ -418,54 +516,59 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     const uint64_t* d_seg_rcw0 = (const uint64_t*)(db + L.seg_rcw0);
     const uint32_t* d_tile_seg = (const uint32_t*)(db + L.tile_seg);
     const uint32_t* d_chunk_seg = (const uint32_t*)(db + L.chunk_seg);
-    uint32_t* d_err = (uint32_t*)err_.p;
+    uint32_t* d_err = (uint32_t*)ws.err;
     hipStream_t st = stream_;
 
-    const uint64_t bits_bytes = (uint64_t)nseg * 16 * hpb;
     // merge lists: a listed run has more than kRunLimit records, so each list
     // holds at most nrec / (kRunLimit + 1) runs
     const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
-    uint8_t* hw = (uint8_t*)hwin_.p;
+    uint8_t* hw = ws.hw;
     // the device part of the call: one launch sequence on the engine stream.
     // Stage events are recorded only outside graph capture.
     auto enqueue = [&](bool events) -> int32_t {
         auto mark = [&](int i) -> hipError_t { return events ? hipEventRecord(ev_[i], st) : hipSuccess; };
-        ZK_TRY(hipMemcpyAsync(desc_.p, hdesc_.p, L.total, hipMemcpyHostToDevice, st));
-        launch_fill_zero(err_.p, 16, st);
+        ZK_TRY(hipMemcpyAsync(ws.desc, ws.hdesc, L.total, hipMemcpyHostToDevice, st));
+        launch_fill_zero(ws.err, 16, st);
         ZK_TRY(mark(0));
         // the decomposition also counts the first sort pass's tile histograms
         launch_decompose(pl.curve, d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
-                         digits_.p, (uint32_t*)hist_.p, d_err, st);
+                         ws.digits, (uint32_t*)ws.hist, d_err, st);
         ZK_TRY(mark(1));
-        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, digits_.p,
-                         nullptr, 0, (uint32_t*)hist_.p, (uint32_t*)part_.p, (uint32_t*)segvalid_.p,
-                         (uint16_t*)keysA_.p, (uint32_t*)valsA_.p, st, false);
-        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, keysA_.p,
-                         (const uint32_t*)valsA_.p, 1, (uint32_t*)hist_.p, (uint32_t*)part_.p,
-                         (uint32_t*)segvalid_.p, (uint16_t*)keysB_.p, (uint32_t*)valsB_.p, st);
-        launch_fill_zero(buckets_.p, pl.buckets * ptb, st);
+        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, ws.digits,
+                         nullptr, 0, (uint32_t*)ws.hist, (uint32_t*)ws.part, (uint32_t*)ws.segvalid,
+                         (uint16_t*)ws.keysA, (uint32_t*)ws.valsA, st, false);
+        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, ws.keysA,
+                         (const uint32_t*)ws.valsA, 1, (uint32_t*)ws.hist, (uint32_t*)ws.part,
+                         (uint32_t*)ws.segvalid, (uint16_t*)ws.keysB, (uint32_t*)ws.valsB, st);
+        launch_fill_zero(ws.buckets, pl.buckets * ptb, st);
         ZK_TRY(mark(2));
-        launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)segvalid_.p,
-                          (const uint16_t*)keysB_.p, (const uint32_t*)valsB_.p, d_table, pl.T, pl.lanes,
-                          buckets_.p, (uint32_t*)recAk_.p, recAp_.p, st);
+        launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)ws.segvalid,
+                          (const uint16_t*)ws.keysB, (const uint32_t*)ws.valsB, d_table, pl.T, pl.lanes,
+                          ws.buckets, (uint32_t*)ws.recAk, ws.recAp, st);
         ZK_TRY(mark(3));
         // bucket partials that crossed lane boundaries: short runs by their first
         // lane, longer runs (heavy buckets) by lane groups.
-        launch_merge_runs(pl.curve, (const uint32_t*)recAk_.p, recAp_.p, 2 * pl.lanes, kRunLimit, buckets_.p,
-                          d_err + 1, (uint32_t*)recBk_.p, list_cap, st);
+        launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk, ws.recAp, 2 * pl.lanes, kRunLimit, ws.buckets,
+                          d_err + 1, (uint32_t*)ws.recBk, list_cap, st);
         ZK_TRY(mark(4));
-        launch_bucket_reduce(pl.curve, d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, buckets_.p, rc_.p,
-                             winsum_.p, st);
+        launch_bucket_reduce(pl.curve, d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, ws.buckets, ws.rc,
+                             ws.bits, !dev_tail, st);
         ZK_TRY(mark(5));
-        ZK_TRY(hipMemcpyAsync(hw, winsum_.p, bits_bytes, hipMemcpyDeviceToHost, st));
+        if (dev_tail) {
+            launch_tail(pl.curve, d_segs, d_msms, nseg, nmsm, ws.bits, ws.wins, ws.res, st);
+            if (events) ZK_TRY(hipEventRecord(ev_tail_, st));
+            ZK_TRY(hipMemcpyAsync(hw, ws.res, back_bytes, hipMemcpyDeviceToHost, st));
+        } else {
+            ZK_TRY(hipMemcpyAsync(hw, ws.bits, back_bytes, hipMemcpyDeviceToHost, st));
+        }
         // err_[0]: decomposition error flag; err_[1..2]: long / mid run counts of
         // the merge (a count above list_cap means listed runs were dropped)
-        ZK_TRY(hipMemcpyAsync(hw + bits_bytes, err_.p, 12, hipMemcpyDeviceToHost, st));
-        ZK_TRY(hipMemcpyAsync(hw + bits_bytes + 16, segvalid_.p, (uint64_t)nseg * 4,
+        ZK_TRY(hipMemcpyAsync(hw + back_bytes, ws.err, 12, hipMemcpyDeviceToHost, st));
+        ZK_TRY(hipMemcpyAsync(hw + back_bytes + 16, ws.segvalid, (uint64_t)nseg * 4,
                               hipMemcpyDeviceToHost, st));
         return kOk;
     };
-    if (graph_mode_) {
+    if (graph) {
         // replay the captured sequence when the call has the same shape (the
         // curve -- its kernels -- and every launch dimension the capture
         // baked in) and every buffer it touches is where it was at capture
@@ -482,9 +585,11 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                            (uint64_t)pl.G, (uint64_t)nseg})
             key.push_back(v);
         key.push_back(table_n);  // places the table's identity bitmap
-        for (const void* p : std::initializer_list<const void*>{d_scalars, d_table, desc_.p, hdesc_.p, digits_.p, keysA_.p, valsA_.p,
-                              keysB_.p, valsB_.p, hist_.p, part_.p, segvalid_.p, buckets_.p,
-                              recAk_.p, recAp_.p, recBk_.p, rc_.p, winsum_.p, err_.p, hwin_.p})
+        key.push_back(dev_tail ? 1 : 0);
+        for (const void* p : std::initializer_list<const void*>{
+                 d_scalars, d_table, ws.desc, ws.hdesc, ws.digits, ws.keysA, ws.valsA, ws.keysB, ws.valsB,
+                 ws.hist, ws.part, ws.segvalid, ws.buckets, ws.recAk, ws.recAp, ws.recBk, ws.rc, ws.bits,
+                 ws.err, ws.hw, ws.wins, ws.res})
             key.push_back((uint64_t)(uintptr_t)p);
         if (!graph_exec_ || key != graph_key_) {
             if (graph_exec_) {
@@ -521,23 +626,31 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
     uint32_t err[3];
-    memcpy(err, hw + bits_bytes, 12);
+    memcpy(err, hw + back_bytes, 12);
     if (err[0] || err[1] > list_cap || err[2] > list_cap) return kGpuError;
 
     for (int s = 0; s < kNumStages; s++) {
         float ms = 0;
-        if (!graph_mode_) (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
+        if (!graph) (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
         timing_.stage_ms[s] = ms;  // stages are not split inside a graph
+    }
+    timing_.device_tail_ms = 0;
+    if (dev_tail && !graph) {
+        // split stage 5 (ev_[5] .. ev_[6]) into the tail and the copy-out
+        (void)hipEventElapsedTime(&timing_.device_tail_ms, ev_[5], ev_tail_);
+        (void)hipEventElapsedTime(&timing_.stage_ms[kStCopyOut], ev_tail_, ev_[kNumStages]);
     }
     (void)hipEventElapsedTime(&timing_.device_ms, ev_[0], ev_[kNumStages]);
 
-    // host tail: Horner over windows, then affine normalisation
+    // host tail: Horner over windows, then affine normalisation; the device
+    // tail left the normalised results
     auto t_tail0 = std::chrono::steady_clock::now();
-    if (pl.curve == CurveId::Grumpkin) host_tail<HostG1<zkh::HR>>(pl, hw, out96);
+    if (dev_tail) memcpy(out96, hw, back_bytes);
+    else if (pl.curve == CurveId::Grumpkin) host_tail<HostG1<zkh::HR>>(pl, hw, out96);
     else if (pl.curve == CurveId::Bn254G2) host_tail<HostG2View>(pl, hw, out96);
     else host_tail<HostG1<zkh::HP>>(pl, hw, out96);
     auto t_end = std::chrono::steady_clock::now();
-    timing_.host_tail_ms = std::chrono::duration<float, std::milli>(t_end - t_tail0).count();
+    timing_.host_tail_ms = dev_tail ? 0.f : std::chrono::duration<float, std::milli>(t_end - t_tail0).count();
     timing_.wall_ms = std::chrono::duration<float, std::milli>(t_end - t_wall0).count();
     timing_.nmsm = nmsm;
     timing_.window_bits_first = nmsm ? pl.msms[0].c : 0;
@@ -547,7 +660,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     timing_.buckets = pl.buckets;
     timing_.lanes = pl.lanes;
     uint64_t valid = 0;
-    const uint32_t* sv = (const uint32_t*)(hw + bits_bytes + 16);
+    const uint32_t* sv = (const uint32_t*)(hw + back_bytes + 16);
     for (uint32_t s = 0; s < nseg; s++) valid += sv[s];
     timing_.valid_entries = valid;
     return kOk;
@@ -567,33 +680,40 @@ int32_t Engine::msm_host(const uint8_t* points, const uint8_t* scalars, uint32_t
 // SRS base rejects unrelated inputs at once; otherwise every slice is compared
 // with the host image of the SRS (in parallel), which is far cheaper than
 // sending the bases over PCIe and rebuilding their table.
-bool Engine::points_are_srs_prefixes(const uint8_t* points, const uint32_t* counts, uint32_t nmsm) {
+bool Engine::srs_candidate(const uint8_t* points, const uint32_t* counts, uint32_t nmsm) {
     if (!srs_detect_ || srs_n_ == 0) return false;
-    struct Cmp { const uint8_t* a; uint64_t bytes; };
-    std::vector<Cmp> cmp;
     uint64_t off = 0, total = 0;
     for (uint32_t j = 0; j < nmsm; j++) {
         const uint64_t n = counts[j];
         if (n > srs_n_) return false;
         if (n == 0) continue;
         if (memcmp(points + off, srs_first_, 64) != 0) return false;
-        cmp.push_back({points + off, n * 64});
         off += n * 64;
-        total += n * 64;
+        total += n;
     }
-    if (cmp.empty()) return false;
+    if (total == 0) return false;
     if (srs_host_.size() != (uint64_t)srs_n_ * 64) {
         std::vector<uint8_t> img((uint64_t)srs_n_ * 64);
         if (srs_download(img.data(), srs_n_) != kOk) return false;
         srs_host_.swap(img);
     }
+    return true;
+}
+
+bool Engine::srs_confirm(const uint8_t* points, const uint32_t* counts, uint32_t nmsm) {
     // split the bytes to compare into pieces for up to 16 host threads
     struct Piece { const uint8_t* a; const uint8_t* b; uint64_t bytes; };
     std::vector<Piece> pieces;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) total += (uint64_t)counts[j] * 64;
     const uint64_t step = std::max<uint64_t>(1ull << 22, total / 16 + 1);
-    for (const Cmp& c : cmp)
-        for (uint64_t o = 0; o < c.bytes; o += step)
-            pieces.push_back({c.a + o, srs_host_.data() + o, std::min(step, c.bytes - o)});
+    uint64_t off = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        const uint64_t bytes = (uint64_t)counts[j] * 64;
+        for (uint64_t o = 0; o < bytes; o += step)
+            pieces.push_back({points + off + o, srs_host_.data() + o, std::min(step, bytes - o)});
+        off += bytes;
+    }
     std::atomic<bool> same{true};
     std::atomic<size_t> next{0};
     auto work = [&]() {
@@ -608,6 +728,38 @@ bool Engine::points_are_srs_prefixes(const uint8_t* points, const uint32_t* coun
     return same.load();
 }
 
+int32_t Engine::stage_upload(void* dst, const uint8_t* src, uint64_t bytes) {
+    if (bytes == 0) return kOk;
+    const uint64_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
+    const uint32_t T = (uint32_t)std::min<uint64_t>(kStageSlots, nchunks);
+    for (uint32_t t = 0; t < T; t++)
+        if (stage_[t].ensure(std::min<uint64_t>(kStageChunk, bytes)) != hipSuccess) return kAllocFailed;
+    // thread t fills slot t with chunks t, t + T, ...: it reuses its slot once
+    // the copy that last read it -- in this upload or an earlier one -- has
+    // completed (the slot's event; one never recorded is complete)
+    std::atomic<int32_t> err{kOk};
+    auto worker = [&](uint32_t t) {
+        for (uint64_t c = t; c < nchunks && err.load(std::memory_order_relaxed) == kOk; c += T) {
+            if (hipEventSynchronize(stage_ev_[t]) != hipSuccess) {
+                err = kGpuError;
+                return;
+            }
+            const uint64_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+            memcpy(stage_[t].p, src + off, len);
+            if (hipMemcpyAsync((uint8_t*)dst + off, stage_[t].p, len, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                hipEventRecord(stage_ev_[t], stream_) != hipSuccess) {
+                err = kGpuError;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t t = 1; t < T; t++) th.emplace_back(worker, t);
+    worker(0);
+    for (auto& x : th) x.join();
+    return err.load();
+}
+
 int32_t Engine::srs_set_first() {
     srs_host_.clear();
     return srs_download(srs_first_, 1);
@@ -616,10 +768,6 @@ int32_t Engine::srs_set_first() {
 int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
                                const uint32_t* counts, uint32_t nmsm, uint8_t* out96, CurveId cv) {
     if (!points || !scalars || !counts || !out96 || nmsm == 0) return kInvalidInput;
-    if (cv == CurveId::Bn254 && points_are_srs_prefixes(points, counts, nmsm)) {
-        srs_hits_++;
-        return msm_resident(scalars, false, counts, nmsm, out96);
-    }
     std::vector<uint32_t> ns;
     std::vector<uint64_t> t0;
     std::vector<uint32_t> idx;
@@ -636,15 +784,35 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
+    // the scalars go up first, in pinned chunks; whichever base table the
+    // call ends up using reads them from scalars_
+    ZK_ALLOC(scalars_, total * 32);
+    int32_t rc = stage_upload(scalars_.p, scalars, total * 32);
+    if (rc != kOk) return rc;
+    if (cv == CurveId::Bn254 && srs_candidate(points, counts, nmsm)) {
+        // slices that start with the SRS's first base: run on the resident
+        // table while host threads confirm the slices byte for byte; the
+        // result stands only if they do
+        bool same = false;
+        std::thread cmp([&] { same = srs_confirm(points, counts, nmsm); });
+        std::vector<uint8_t> res(nmsm * 96);
+        rc = msm_table(srs_.p, srs_n_, scalars_.p, true, counts, nmsm, res.data());
+        cmp.join();
+        if (rc != kOk) return rc;
+        if (same) {
+            srs_hits_++;
+            memcpy(out96, res.data(), res.size());
+            return kOk;
+        }
+    }
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                       lane_cap(cv), cv);
+    rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
+                   lane_cap(cv), cv);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * input_point_bytes(cv));
     ZK_ALLOC(ptable_, table_bytes(total, cv));
-    ZK_ALLOC(scalars_, total * 32);
-    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, total * input_point_bytes(cv), hipMemcpyHostToDevice, stream_));
-    ZK_TRY(hipMemcpyAsync(scalars_.p, scalars, total * 32, hipMemcpyHostToDevice, stream_));
+    rc = stage_upload(pts_in_.p, points, total * input_point_bytes(cv));
+    if (rc != kOk) return rc;
     launch_build_table(cv, pts_in_.p, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * result_bytes(cv));
     rc = run(pl, scalars_.p, ptable_.p, total, res.data());
@@ -664,7 +832,8 @@ int32_t Engine::multi_msm_host(const uint8_t* points, uint32_t n, const uint8_t*
     if ((uint64_t)n * k >= (1ull << 30)) return kInvalidInput;
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
     ZK_ALLOC(ptable_, table_bytes(n));
-    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
+    const int32_t rc = stage_upload(pts_in_.p, points, (uint64_t)n * 64);
+    if (rc != kOk) return rc;
     launch_build_table(CurveId::Bn254, pts_in_.p, ptable_.p, n, stream_);
     std::vector<uint32_t> counts(k, n);
     return msm_table(ptable_.p, n, scalars, false, counts.data(), k, out96);
@@ -710,7 +879,8 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     srs_host_.clear();
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
     ZK_ALLOC(srs_, table_bytes(n));
-    ZK_TRY(hipMemcpyAsync(pts_in_.p, points, (uint64_t)n * 64, hipMemcpyHostToDevice, stream_));
+    const int32_t rc = stage_upload(pts_in_.p, points, (uint64_t)n * 64);
+    if (rc != kOk) return rc;
     launch_build_table(CurveId::Bn254, pts_in_.p, srs_.p, n, stream_);
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipGetLastError());
@@ -762,7 +932,10 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
     ZK_ALLOC(pts_in_, n * 64);
     ZK_ALLOC(srs_, table_bytes(n));
     ZK_ALLOC(err_, 16);
-    ZK_TRY(hipMemcpyAsync(pts_in_.p, g1, n * 64, hipMemcpyHostToDevice, stream_));
+    {
+        const int32_t rc = stage_upload(pts_in_.p, g1, n * 64);
+        if (rc != kOk) return rc;
+    }
     launch_fill_zero(err_.p, 16, stream_);
     launch_import_be(pts_in_.p, n, srs_.p, (uint32_t*)err_.p, stream_);
     uint32_t bad = 0;
@@ -998,8 +1171,8 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
         uint64_t off = 0;
         for (uint32_t k = 0; k < idx.size(); k++) {
             const uint64_t bytes = (uint64_t)ns[k] * 32;
-            ZK_TRY(hipMemcpyAsync((uint8_t*)scalars_.p + off, host_parts[idx[k]], bytes,
-                                  hipMemcpyHostToDevice, stream_));
+            rc = stage_upload((uint8_t*)scalars_.p + off, host_parts[idx[k]], bytes);
+            if (rc != kOk) return rc;
             off += bytes;
         }
         d_sc = scalars_.p;
@@ -1011,19 +1184,21 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     return kOk;
 }
 
+uint32_t Engine::window_bit(uint32_t n, bool batch, CurveId cv) const {
+    uint32_t c = c_override_;
+    if (!c && cv == CurveId::Bn254) {
+        uint32_t cls = 0;
+        while ((1ull << cls) < n) cls++;  // ceil(log2 n)
+        cls = std::min<uint32_t>(cls, 32);
+        if (batch) c = ctab_batch_[cls];
+        if (!c) c = ctab_[cls];
+    }
+    return c;  // 0 = cost-model default
+}
+
 std::vector<uint32_t> Engine::window_bits_for(const std::vector<uint32_t>& ns, CurveId cv) const {
     std::vector<uint32_t> cs(ns.size());
-    for (size_t j = 0; j < ns.size(); j++) {
-        uint32_t c = c_override_;
-        if (!c && cv == CurveId::Bn254) {
-            uint32_t cls = 0;
-            while ((1ull << cls) < ns[j]) cls++;  // ceil(log2 n)
-            cls = std::min<uint32_t>(cls, 32);
-            if (ns.size() > 1) c = ctab_batch_[cls];
-            if (!c) c = ctab_[cls];
-        }
-        cs[j] = c;  // 0 = cost-model default
-    }
+    for (size_t j = 0; j < ns.size(); j++) cs[j] = window_bit(ns[j], ns.size() > 1, cv);
     return cs;
 }
 

@@ -65,6 +65,7 @@ struct Timing {
     float stage_ms[kNumStages];  // device time per stage (HIP events on the engine stream)
     float device_ms;             // first to last event
     float host_tail_ms;          // window combine + normalisation on the host
+    float device_tail_ms;        // the same on the device (0 when the host ran it)
     float wall_ms;               // whole call, host clock
     uint32_t nmsm, window_bits_first, T, G;
     uint64_t entries, buckets, lanes;
@@ -88,6 +89,25 @@ struct Plan {
     CurveId curve = CurveId::Bn254;
     uint32_t G = 8;  // buckets per bucket-reduction lane (a power of 2, 8..32)
 };
+
+// Device buffers of one call (and the host staging it copies through).
+struct Workspace {
+    uint8_t* desc = nullptr;  // packed descriptor tables
+    void *digits = nullptr, *keysA = nullptr, *valsA = nullptr, *keysB = nullptr, *valsB = nullptr;
+    void *hist = nullptr, *part = nullptr, *segvalid = nullptr, *buckets = nullptr;
+    void *recAk = nullptr, *recAp = nullptr, *recBk = nullptr, *rc = nullptr, *bits = nullptr;
+    void *err = nullptr, *wins = nullptr, *res = nullptr;
+    uint8_t* hdesc = nullptr;  // host: descriptors before upload
+    uint8_t* hw = nullptr;     // host: copy-back area
+};
+
+// bytes of each buffer of a planned call; device = all device buffers, each
+// rounded up to 256 bytes; host = the copy-back area
+struct WsSizes {
+    uint64_t desc, digits, keys, vals, hist, part, segvalid, buckets, rec_keys, rec_pts, rc, bits, err,
+        wins, res, back, host, device;
+};
+WsSizes ws_sizes(const Plan& pl, bool dev_tail);
 
 // window width for an MSM of n points (cost model; overridable)
 uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
@@ -181,6 +201,7 @@ class Engine {
     // only), else 0 = the cost model
     std::vector<uint32_t> window_bits_for(const std::vector<uint32_t>& ns,
                                           CurveId cv = CurveId::Bn254) const;
+    uint32_t window_bit(uint32_t n, bool batch, CurveId cv) const;
 
     void set_window_bits(uint32_t c) { c_override_ = c; }
     // cap on the device memory this engine's buffers may grow to (resident
@@ -191,6 +212,16 @@ class Engine {
     // graph mode: the device launch sequence of a call is captured into a
     // hipGraph and replayed while calls keep the same shape and buffers
     void set_graph_mode(bool on) { graph_mode_ = on; }
+    // where a call's window combine and normalisation run: kTailAuto (the
+    // device in graph mode and for calls of at least kDeviceTailMinMsms MSMs,
+    // else the host), kTailHost, kTailDevice.  BN254 G2 always uses the host.
+    enum TailMode : int32_t { kTailAuto = 0, kTailHost = 1, kTailDevice = 2 };
+    static constexpr uint32_t kDeviceTailMinMsms = 16;
+    int32_t set_tail_mode(int32_t m) {
+        if (m < kTailAuto || m > kTailDevice) return kInvalidInput;
+        tail_mode_ = m;
+        return kOk;
+    }
     uint64_t graph_captures() const { return graph_captures_; }
     const Timing& last_timing() const { return timing_; }
     int device() const { return dev_; }
@@ -215,6 +246,8 @@ class Engine {
 
   private:
     Engine() = default;
+    int32_t run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, bool graph,
+                   const void* d_scalars, const void* d_table, uint64_t table_n, uint8_t* out96);
     int32_t ensure_gtab();
     int32_t ensure_gtab_grumpkin();
     int32_t ensure_gtab_g2();
@@ -230,6 +263,7 @@ class Engine {
     uint32_t ctab_[33] = {};
     uint32_t ctab_batch_[33] = {};
     bool graph_mode_ = false;
+    int32_t tail_mode_ = kTailAuto;
     uint64_t acc_cap_ = 1u << 17;  // resident k_accumulate lanes on this device
     uint64_t acc_cap_g2_ = 1u << 16;  // the same for the G2 accumulation
     uint64_t lane_cap(CurveId cv) const { return cv == CurveId::Bn254G2 ? acc_cap_g2_ : acc_cap_; }
@@ -240,6 +274,7 @@ class Engine {
 
     DevBuf desc_, digits_, keysA_, valsA_, keysB_, valsB_, hist_, part_, segvalid_, buckets_;
     DevBuf recAk_, recAp_, recBk_, rc_, winsum_, err_;  // recBk_: long-run list
+    DevBuf wins_, res_;  // device tail: window sums, results
     DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, gtab_gk_, gtab_g2_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
@@ -251,8 +286,21 @@ class Engine {
     bool srs_detect_ = true;
     uint64_t srs_hits_ = 0;
     int32_t srs_set_first();
-    bool points_are_srs_prefixes(const uint8_t* points, const uint32_t* counts, uint32_t nmsm);
+    // cheap test: every point slice fits the SRS and starts with its first
+    // base (and the host image for the full check is present)
+    bool srs_candidate(const uint8_t* points, const uint32_t* counts, uint32_t nmsm);
+    // exact test of a candidate: every slice equals the SRS prefix, byte for byte
+    bool srs_confirm(const uint8_t* points, const uint32_t* counts, uint32_t nmsm);
+    // host -> device through a ring of pinned chunks filled by several host
+    // threads while earlier chunks are in flight (pageable memory moves at a
+    // fraction of the link rate through the runtime's own staging)
+    static constexpr uint32_t kStageSlots = 8;
+    static constexpr uint64_t kStageChunk = 8ull << 20;
+    int32_t stage_upload(void* dst, const uint8_t* src, uint64_t bytes);
+    HostBuf stage_[kStageSlots];
+    hipEvent_t stage_ev_[kStageSlots] = {};
     hipEvent_t ev_[kNumStages + 1] = {};
+    hipEvent_t ev_tail_ = nullptr;  // after the device tail, before the copy-out
 };
 
 }  // namespace zk

@@ -27,6 +27,10 @@
 #include "kernels.hpp"
 #include "plan.hpp"
 
+#ifndef ZK_ACC_PIPE3
+#define ZK_ACC_PIPE3 0
+#endif
+
 namespace zk {
 
 // BN254 G1 over Fp: the GLV curve of the MSM, the SRS import / generation and
@@ -115,6 +119,7 @@ struct G1Curve {
     static constexpr uint32_t kSlots = 1;
     static constexpr uint64_t kPtBytes = kBucketBytes;
     static __device__ __forceinline__ Pt identity() { return qxyzz_identity(); }
+    static __device__ __forceinline__ Pt select(bool c, const Pt& a, const Pt& b) { return qxyzz_select(c, a, b); }
     static __device__ __forceinline__ Pt run_start() { return Arith<F>::qxyzz_run_start(); }
     static __device__ __forceinline__ void madd_run(Pt& a, bool& id, const Af& q) {
         Arith<F>::qxyzz_madd_run(a, id, q.x, q.y);
@@ -127,6 +132,28 @@ struct G1Curve {
     static __device__ __forceinline__ void store_host(uint4* b, uint64_t i, const Pt& p) {
         store_xyzz(b, i, Arith<F>::qxyzz_to_xyzz(p));
     }
+    static __device__ __forceinline__ Pt dbl(const Pt& p) { return Arith<F>::qxyzz_dbl(p); }
+    // result slot i (96 B): the normalised Jacobian X | Y | Z in the 2^256
+    // Montgomery form, canonical: (x, y, 1), the identity (1, 1, 0)
+    static __device__ __forceinline__ void store_result(uint4* b, uint64_t i, const Pt& p) {
+        using A = Arith<F>;
+        const Fp one = A::fq_to_fp(A::fq_one());
+        Fp x = one, y = one, z = fp_zero();
+        if (!A::qxyzz_is_id(p)) {
+            // 1/ZZ = ZZZ / (ZZ ZZZ), 1/ZZZ = ZZ / (ZZ ZZZ): one inversion
+            const Fq inv = A::fq_inv(A::fq_mul(p.ZZ, p.ZZZ));
+            x = A::fq_to_fp(A::fq_mul(p.X, A::fq_mul(p.ZZZ, inv)));
+            y = A::fq_to_fp(A::fq_mul(p.Y, A::fq_mul(p.ZZ, inv)));
+            z = one;
+        }
+        uint4* o = b + i * 6;
+        const Fp* f[3] = {&x, &y, &z};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            o[2 * k] = make_uint4(f[k]->v[0], f[k]->v[1], f[k]->v[2], f[k]->v[3]);
+            o[2 * k + 1] = make_uint4(f[k]->v[4], f[k]->v[5], f[k]->v[6], f[k]->v[7]);
+        }
+    }
 };
 
 struct G2Curve {
@@ -135,6 +162,7 @@ struct G2Curve {
     static constexpr uint32_t kSlots = 2;
     static constexpr uint64_t kPtBytes = 2 * kBucketBytes;
     static __device__ __forceinline__ Pt identity() { return G2Arith::identity(); }
+    static __device__ __forceinline__ Pt select(bool c, const Pt& a, const Pt& b) { return G2Arith::select(c, a, b); }
     static __device__ __forceinline__ Pt run_start() { return G2Arith::run_start(); }
     static __device__ __forceinline__ void madd_run(Pt& a, bool& id, const Af& q) {
         G2Arith::madd_run(a, id, q.x, q.y);
@@ -162,6 +190,15 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+// point i of b where `valid`, else the identity: limb-wise selects (a
+// conditional load of a whole point as one value made the compiler stage it
+// through scratch memory); an invalid lane reads point 0, which every such
+// buffer holds
+template <class C>
+__device__ __forceinline__ typename C::Pt load_or_id(const uint4* b, uint64_t i, bool valid) {
+    return C::select(valid, C::load(b, valid ? i : 0), C::identity());
+}
+
 template <class C, class Load>
 __device__ __forceinline__ typename C::Pt sum_then_tree(uint32_t nload, uint32_t P, Load load) {
     const uint32_t lane = __lane_id();
@@ -175,8 +212,7 @@ __device__ __forceinline__ typename C::Pt sum_then_tree(uint32_t nload, uint32_t
             q = load(st);
         } else {
             const uint32_t s = P >> (st - nload + 1);
-            q = C::shfl_down(acc, s);
-            if ((lane & (P - 1)) >= s) q = C::identity();
+            q = C::select((lane & (P - 1)) >= s, C::identity(), C::shfl_down(acc, s));
         }
         C::add(acc, q);
     }
@@ -777,6 +813,53 @@ __global__ __launch_bounds__(256) void k_accumulate(
     bool first = true;
     typename C::Pt acc = C::run_start();
     bool acc_id = true;
+    if constexpr (ZK_ACC_PIPE3 && C::kSlots == 1) {
+    // software pipeline, three entries per iteration: before entry p's mixed
+    // addition the base of entry p+2 is gathered (its value was loaded one
+    // step earlier) into the register set that entry p-1 used, and the key and
+    // value of entry p+3 are loaded; each gather has two additions' time to
+    // arrive, and the three sets rotate by name, not by moves.  Indices clamp
+    // to the lane's last entry.
+    uint32_t k = cur;
+    typename C::Af q0, q1, q2;
+    {
+        const uint32_t v0 = vals[b], v1 = vals[min(b + 1, e - 1)];
+        q0 = C::load_tab(tb, v0 & 0x7fffffffu, v0 >> 31);
+        q1 = C::load_tab(tb, v1 & 0x7fffffffu, v1 >> 31);
+    }
+    uint32_t k1 = keys[min(b + 1, e - 1)];
+    const uint64_t p2 = min(b + 2, e - 1);
+    uint32_t kn = keys[p2], vn = vals[p2];
+    auto step = [&](uint64_t p, const typename C::Af& qa, typename C::Af& qc) {
+        const uint64_t pnn = min(p + 3, e - 1);
+        const uint32_t knn = keys[pnn], vnn = vals[pnn];
+        qc = C::load_tab(tb, vn & 0x7fffffffu, vn >> 31);
+        if (k != cur) {
+            // run `cur` ended at p-1 and is not the lane's last run
+            const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
+            if (first && head_open) {
+                put_record<C>(rec_keys, rec_pts, 2 * L, gk, true, acc);
+            } else {
+                C::store(buckets, gk, acc);
+                if (first) put_record<C>(rec_keys, rec_pts, 2 * L, gk, false, acc);
+            }
+            first = false;
+            acc = C::run_start();
+            acc_id = true;
+            cur = k;
+        }
+        C::madd_run(acc, acc_id, qa);
+        k = k1;
+        k1 = kn;
+        kn = knn;
+        vn = vnn;
+    };
+    for (uint64_t p = b; p < e; p += 3) {
+        step(p, q0, q2);
+        if (p + 1 < e) step(p + 1, q1, q0);
+        if (p + 2 < e) step(p + 2, q2, q1);
+    }
+    } else {
     // software pipeline: before entry p's mixed addition the base of entry
     // p+1 is gathered (its value was loaded one iteration earlier, so the
     // gather's address waits on nothing issued in this iteration) and the key
@@ -811,6 +894,7 @@ __global__ __launch_bounds__(256) void k_accumulate(
         kn = knn;
         vn = vnn;
         q = qn;
+    }
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
     const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
@@ -910,7 +994,7 @@ __global__ __launch_bounds__(256) void k_merge_mid(const uint32_t* __restrict__ 
         }
         const typename C::Pt acc = sum_then_tree<C>(wave_max(cnt), 8, [&](uint32_t e) {
             const uint64_t p = start + sub + 8ull * e;
-            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? C::load(pts, p) : C::identity();
+            return load_or_id<C>(pts, p, e < cnt && !(keys[p] & ZK_REC_INVALID));
         });
         // the group's first lane stores if any lane of the group had a payload
         const uint64_t hb = __ballot(has);
@@ -941,7 +1025,7 @@ __global__ __launch_bounds__(256) void k_merge_long(const uint32_t* __restrict__
         }
         typename C::Pt acc = sum_then_tree<C>(wave_max(cnt), 64, [&](uint32_t e) {
             const uint64_t p = start + t + 256ull * e;
-            return (e < cnt && !(keys[p] & ZK_REC_INVALID)) ? C::load(pts, p) : C::identity();
+            return load_or_id<C>(pts, p, e < cnt && !(keys[p] & ZK_REC_INVALID));
         });
         const bool wave_has = __ballot(has) != 0;
         if (lane == 0) {
@@ -1012,7 +1096,7 @@ __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ s
     const typename C::Pt acc = sum_then_tree<C>(cnt, P, [&](uint32_t e) {
         const uint32_t x = part * cnt + e;
         const uint64_t kk = rows ? ((uint64_t)item << g) + x : ((uint64_t)x << g) + item;
-        return item < nitem ? C::load(buckets, sg.bkt0 + kk) : C::identity();
+        return load_or_id<C>(buckets, sg.bkt0 + kk, item < nitem);
     });
     if (item < nitem && part == 0) C::store(rc, seg_rc0[s] + (rows ? item : nrow + item), acc);
 }
@@ -1026,7 +1110,7 @@ template <class C>
 __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
                                                  const uint64_t* __restrict__ seg_rc0,
                                                  uint32_t nseg, const uint4* __restrict__ rc,
-                                                 uint4* __restrict__ bits) {
+                                                 uint4* __restrict__ bits, int host_form) {
     const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t s = (uint32_t)(item >> 4), j = (uint32_t)(item & 15u);
     if (s >= nseg) return;
@@ -1051,9 +1135,59 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
     const uint32_t P = count >= 64 ? 64u : count;
     const typename C::Pt acc = sum_then_tree<C>((count + 63) / 64, P, [&](uint32_t e) {
         const uint32_t i = lane + 64 * e;
-        return i < count ? C::load(rc, base + (masked ? with_bit(i, b) : i)) : C::identity();
+        return load_or_id<C>(rc, base + (masked ? with_bit(i, b) : i), i < count);
     });
-    if (lane == 0) C::store_host(bits, item, acc);  // host form
+    if (lane == 0) {
+        if (host_form) C::store_host(bits, item, acc);  // for the host tail
+        else C::store(bits, item, acc);                 // for k_tail_windows
+    }
+}
+
+// ------------------------------------------------ window combine on the device
+// The device form of the call's tail (the host form is Engine::run's
+// host_tail): result_j = sum_w 2^(c w) S_w with S_w = U_w + sum_{t < c-1}
+// 2^t Q_{w,t}, the planes k_br_bits stored (bits[s*16] = U, bits[s*16 + 1 +
+// t] = Q_t), then the affine normalisation.  The reference does its window
+// combine on the host (Sources/zkMetal/MSM/MSMEngine.swift:899-904); on the
+// device it lets a captured graph end with final results.  The doubling chain
+// is inherently serial (2^127 S_top needs 127 doublings), so one lane per
+// window computes S_w (Horner over its planes), then one lane per MSM runs
+// the Horner over windows and the inversion.
+
+template <class C>
+__global__ __launch_bounds__(64) void k_tail_windows(const SegDesc* __restrict__ segs,
+                                                     const MsmDesc* __restrict__ msms, uint32_t nseg,
+                                                     const uint4* __restrict__ bits,
+                                                     uint4* __restrict__ wins) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nseg) return;
+    const uint32_t c = msms[segs[s].msm].c;
+    const uint64_t b0 = (uint64_t)s * 16;
+    typename C::Pt acc = C::load(bits, b0 + 1 + (c - 2));  // Q_{c-2}
+#pragma unroll 1
+    for (int t = (int)c - 3; t >= 0; t--) {
+        acc = C::dbl(acc);
+        C::add(acc, C::load(bits, b0 + 1 + t));
+    }
+    C::add(acc, C::load(bits, b0));  // U
+    C::store(wins, s, acc);
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void k_tail_combine(const MsmDesc* __restrict__ msms, uint32_t nmsm,
+                                                     const uint4* __restrict__ wins,
+                                                     uint4* __restrict__ out) {
+    const uint32_t j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= nmsm) return;
+    const MsmDesc m = msms[j];
+    typename C::Pt acc = C::load(wins, m.seg0 + m.nwin - 1);
+#pragma unroll 1
+    for (int w = (int)m.nwin - 2; w >= 0; w--) {
+#pragma unroll 1
+        for (uint32_t i = 0; i < m.c; i++) acc = C::dbl(acc);
+        C::add(acc, C::load(wins, m.seg0 + w));
+    }
+    C::store_result(out, j, acc);
 }
 
 // ------------------------------------------------ synthetic base table
@@ -1543,24 +1677,42 @@ void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64
 template <class C>
 static void bucket_reduce_t(const SegDesc* segs, const uint64_t* seg_wave0,
                             const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                            const void* buckets, void* rc, void* bits, hipStream_t st) {
+                            const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st) {
     hipLaunchKernelGGL(k_br_rowcol<C>, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
                        seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
     const uint64_t bit_items = (uint64_t)nseg * 16;
     hipLaunchKernelGGL(k_br_bits<C>, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
-                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits);
+                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits, host_form ? 1 : 0);
 }
 
 void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
                           const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                          const void* buckets, void* rc, void* bits, hipStream_t st) {
+                          const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st) {
     if (!rc_waves) return;
     if (cv == CurveId::Grumpkin)
-        bucket_reduce_t<G1Curve<Bn254Fr>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+        bucket_reduce_t<G1Curve<Bn254Fr>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits,
+                                          host_form, st);
     else if (cv == CurveId::Bn254G2)
-        bucket_reduce_t<G2Curve>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+        bucket_reduce_t<G2Curve>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, true, st);
     else
-        bucket_reduce_t<G1Curve<Bn254Fp>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, st);
+        bucket_reduce_t<G1Curve<Bn254Fp>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits,
+                                          host_form, st);
+}
+
+void launch_tail(CurveId cv, const SegDesc* segs, const MsmDesc* msms, uint32_t nseg, uint32_t nmsm,
+                 const void* bits, void* wins, void* out, hipStream_t st) {
+    if (!nseg || !nmsm) return;
+    if (cv == CurveId::Grumpkin) {
+        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fr>>, dim3((nseg + 63) / 64), dim3(64), 0, st, segs,
+                           msms, nseg, (const uint4*)bits, (uint4*)wins);
+        hipLaunchKernelGGL(k_tail_combine<G1Curve<Bn254Fr>>, dim3((nmsm + 63) / 64), dim3(64), 0, st, msms,
+                           nmsm, (const uint4*)wins, (uint4*)out);
+    } else if (cv == CurveId::Bn254) {
+        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fp>>, dim3((nseg + 63) / 64), dim3(64), 0, st, segs,
+                           msms, nseg, (const uint4*)bits, (uint4*)wins);
+        hipLaunchKernelGGL(k_tail_combine<G1Curve<Bn254Fp>>, dim3((nmsm + 63) / 64), dim3(64), 0, st, msms,
+                           nmsm, (const uint4*)wins, (uint4*)out);
+    }
 }
 
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,

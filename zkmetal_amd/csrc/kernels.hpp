@@ -43,9 +43,17 @@ void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane
 void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
                        void* dest, uint32_t* counts, uint32_t* lists, uint32_t list_cap,
                        hipStream_t st);
+// bits: 16 points per segment (U and the bit-plane sums), in the host form
+// (Xyzz, 2^256 Montgomery, 128 B) for the host tail or in the stored form
+// (144 B) for launch_tail
 void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
                           const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                          const void* buckets, void* rc, void* bits, hipStream_t st);
+                          const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st);
+// window combine and normalisation on the device (BN254 G1 and Grumpkin):
+// bits in the stored form, wins: nseg stored points (scratch), out: nmsm
+// results of 96 B (normalised Jacobian X | Y | Z, 2^256 Montgomery form)
+void launch_tail(CurveId cv, const SegDesc* segs, const MsmDesc* msms, uint32_t nseg, uint32_t nmsm,
+                 const void* bits, void* wins, void* out, hipStream_t st);
 void launch_gen_bases(const void* ks, const void* gtab, uint64_t n, void* table,
                       hipStream_t st);
 // out[i] = k_i * G on Grumpkin, 64-byte affine points (x, y in Fr Montgomery form)
