@@ -55,6 +55,24 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
                                               const uint32_t* counts, uint32_t n_msms,
                                               uint8_t* results);
 
+/* The same as zkmsm_msm_batch_resident_device, allocating nothing: no
+ * hipMalloc / hipFree / pinned-host allocation and no allocating library
+ * call (no counterpart in the reference; for callers that cannot allocate:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  The call's device buffers come from a fixed array of 64 MiB per
+ * device, in the library's device data; the host side uses capacity reserved
+ * when the engine was created.  Limits: at most 65536 points over all MSMs,
+ * at most 64 MSMs, and the call's buffers must fit the array (window widths
+ * from the engine's tables; very small widths need more); a call beyond them
+ * returns ZKMETAL_ERR_INVALID_INPUT without running.  Every engine on a
+ * device shares that device's array: the library serialises such calls per
+ * device (they do not overlap), and the array keeps the last call's
+ * contents.  Results equal zkmsm_msm_batch_resident_device's bytes. */
+ZkMetalStatus zkmsm_msm_batch_resident_device_noalloc(ZkMetalMSMEngine engine,
+                                                      const void* device_scalars,
+                                                      const uint32_t* counts, uint32_t n_msms,
+                                                      uint8_t* results);
+
 /* KZG commitments over the resident table (KZGEngine.commit / batchCommit,
  * Sources/zkMetal/KZG/KZGEngine.swift:131-179): polynomial p has counts[p]
  * coefficients, each an Fr element in Montgomery form (32 B, 8 x u32 LE --

@@ -78,6 +78,7 @@ def lib():
         "zkmsm_srs_size": (u32, [vp]),
         "zkmsm_msm_batch_resident": (i32, [vp, vp, vp, u32, vp]),
         "zkmsm_msm_batch_resident_device": (i32, [vp, vp, vp, u32, vp]),
+        "zkmsm_msm_batch_resident_device_noalloc": (i32, [vp, vp, vp, u32, vp]),
         "zkmsm_msm_batch_device": (i32, [vp, vp, vp, vp, u32, vp]),
         "zkmsm_device_alloc": (i32, [u64, ctypes.POINTER(vp)]),
         "zkmsm_device_free": (i32, [vp]),

@@ -321,6 +321,16 @@ ZkMetalStatus zkmsm_msm_batch_resident_device(ZkMetalMSMEngine engine,
     return e->msm_resident(device_scalars, true, counts, n_msms, results);
 }
 
+ZkMetalStatus zkmsm_msm_batch_resident_device_noalloc(ZkMetalMSMEngine engine,
+                                                      const void* device_scalars,
+                                                      const uint32_t* counts, uint32_t n_msms,
+                                                      uint8_t* results) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->msm_resident_noalloc(device_scalars, counts, n_msms, results);
+}
+
 ZkMetalStatus zkmsm_srs_load_ptau(ZkMetalMSMEngine engine, const uint8_t* data, uint64_t len,
                                   uint32_t max_points, uint32_t* n_loaded) {
     Engine* e = as_engine(engine);

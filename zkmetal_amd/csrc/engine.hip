@@ -219,6 +219,7 @@ int32_t Engine::create(Engine** out) {
         return kGpuError;
     }
     e->account_buffers();
+    (void)e->noalloc_setup();  // on failure the no-allocation entry point reports kGpuError
     e->acc_cap_ = accumulate_capacity(e->dev_);
     e->acc_cap_g2_ = accumulate_capacity(e->dev_, CurveId::Bn254G2);
     memcpy(e->ctab_, kTunedPlain, sizeof(e->ctab_));
@@ -1210,6 +1211,140 @@ int32_t Engine::set_window_table(const uint32_t* c_by_log2n, uint32_t len, bool 
         t[i] = c;
     }
     memcpy(batch ? ctab_batch_ : ctab_, t, sizeof(t));
+    return kOk;
+}
+
+// ------------------------------------------------------ no-allocation path
+// The device buffers of zkmsm_msm_batch_resident_device_noalloc: a fixed
+// array in the library's device data, one copy per device (the runtime puts
+// it on each device that loads the library; it stays until the process
+// ends).  In host code `scratch` names a host-side stand-in, not device
+// memory: host code uses only the address hipGetSymbolAddress returned for
+// the current device (kept per device in g_scratch, copied into each engine).
+static __device__ __align__(256) uint8_t scratch[Engine::kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+// one device's array serves one call at a time
+std::mutex g_noalloc_mu[kMaxDevices];
+constexpr size_t kNoAllocHostBytes = 512u << 10;
+}  // namespace
+
+// at engine creation, with the engine's device current: the device address
+// of this device's array (looked up once per device), and host capacity for
+// the plan and the staging, so that the calls themselves allocate nothing
+int32_t Engine::noalloc_setup() {
+    if (dev_ < 0 || dev_ >= kMaxDevices) return kGpuError;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[dev_]) {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                (void)hipGetLastError();
+                return kGpuError;
+            }
+            g_scratch[dev_] = (uint8_t*)ptr;
+        }
+        scratch_ = g_scratch[dev_];
+    }
+    na_plan_.msms.reserve(kNoAllocMaxMsms);
+    na_plan_.msm_scalar0.reserve(kNoAllocMaxMsms + 1);
+    na_plan_.msm_dtile0.reserve(kNoAllocMaxMsms + 1);
+    na_plan_.segs.reserve(kNoAllocMaxSegs);
+    na_plan_.seg_lane0.reserve(kNoAllocMaxSegs + 1);
+    na_plan_.seg_grp0.reserve(kNoAllocMaxSegs + 1);
+    na_plan_.seg_rcw0.reserve(kNoAllocMaxSegs + 1);
+    na_plan_.tile_seg.reserve(4 * kNoAllocMaxSegs);
+    na_plan_.chunk_seg.reserve(2 * kNoAllocMaxSegs);
+    na_host_.reserve(kNoAllocHostBytes);
+    return kOk;
+}
+
+int32_t Engine::msm_resident_noalloc(const void* d_scalars, const uint32_t* counts, uint32_t nmsm,
+                                     uint8_t* out96) {
+    if (!scratch_) return kGpuError;  // the array's address was not found at creation
+    if (!d_scalars || !counts || !out96 || nmsm == 0 || srs_n_ == 0) return kInvalidInput;
+    if (nmsm > kNoAllocMaxMsms) return kInvalidInput;
+    uint32_t ns[kNoAllocMaxMsms], cs[kNoAllocMaxMsms], idx[kNoAllocMaxMsms];
+    uint64_t t0[kNoAllocMaxMsms];
+    uint32_t k = 0;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nmsm; j++) {
+        if (counts[j] > srs_n_) return kInvalidInput;
+        total += counts[j];
+        if (counts[j] == 0) continue;
+        ns[k] = counts[j];
+        t0[k] = 0;
+        idx[k++] = j;
+    }
+    if (total > kNoAllocMaxPoints) return kInvalidInput;
+    for (uint32_t j = 0; j < nmsm; j++)
+        if (counts[j] == 0) memset(out96 + (uint64_t)j * 96, 0, 96);
+    if (k == 0) return kOk;
+    // the plan's array sizes, before planning: within the reserved capacity
+    uint64_t nseg = 0, tiles = 0, chunks = 0;
+    for (uint32_t i = 0; i < k; i++) {
+        cs[i] = window_bit(ns[i], k > 1, CurveId::Bn254);
+        const uint32_t c = cs[i] ? cs[i] : default_window_bits(ns[i]);
+        if (c < 2 || c > 16) return kInvalidInput;
+        const uint64_t w = (128 + c - 1) / c, nt = (2ull * ns[i] + kSortTile - 1) / kSortTile;
+        nseg += w;
+        tiles += w * nt;
+        chunks += w * ((256 * nt + kScanChunk - 1) / kScanChunk);
+    }
+    if (nseg > na_plan_.segs.capacity() || tiles > na_plan_.tile_seg.capacity() ||
+        chunks > na_plan_.chunk_seg.capacity())
+        return kInvalidInput;
+    // at most 2^15 accumulation lanes: a quarter of the device's resident
+    // lanes, ample for <= 2^16 points, and it bounds the merge records (two
+    // per lane), the largest buffer of a small call
+    int rc = make_plan(ns, t0, k, cs, na_plan_, 0, std::min<uint64_t>(acc_cap_, 1u << 15));
+    if (rc != kOk) return rc;
+    const Plan& pl = na_plan_;
+    const WsSizes z = ws_sizes(pl, true);
+    const uint64_t desc_host = (z.desc + 255) & ~uint64_t(255);
+    const uint64_t res_host = (uint64_t)k * 96;
+    if (z.device > kNoAllocScratchBytes || desc_host + z.host + res_host > na_host_.capacity())
+        return kInvalidInput;  // the call needs more than the array holds
+    // carve the call's buffers out of the array, 256-byte aligned, in the
+    // order ws_sizes counts them
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        void* p = scratch_ + off;
+        off += (bytes + 255) & ~uint64_t(255);
+        return p;
+    };
+    Workspace ws;
+    ws.desc = (uint8_t*)take(z.desc);
+    ws.digits = take(z.digits);
+    ws.keysA = take(z.keys);
+    ws.valsA = take(z.vals);
+    ws.keysB = take(z.keys);
+    ws.valsB = take(z.vals);
+    ws.hist = take(z.hist);
+    ws.part = take(z.part);
+    ws.segvalid = take(z.segvalid);
+    ws.buckets = take(z.buckets);
+    ws.recAk = take(z.rec_keys);
+    ws.recAp = take(z.rec_pts);
+    ws.recBk = take(z.rec_keys);
+    ws.rc = take(z.rc);
+    ws.bits = take(z.bits);
+    ws.err = take(z.err);
+    ws.wins = take(z.wins);
+    ws.res = take(z.res);
+    na_host_.resize(desc_host + z.host + res_host);  // within the reserved capacity
+    ws.hdesc = na_host_.data();
+    ws.hw = na_host_.data() + desc_host;
+    uint8_t* res = ws.hw + z.host;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu[dev_]);
+        rc = run_ws(pl, ws, z, true, false, d_scalars, srs_.p, srs_n_, res);
+    }
+    if (rc != kOk) return rc;
+    for (uint32_t i = 0; i < k; i++) memcpy(out96 + (uint64_t)idx[i] * 96, res + (uint64_t)i * 96, 96);
     return kOk;
 }
 

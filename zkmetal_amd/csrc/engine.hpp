@@ -244,10 +244,25 @@ class Engine {
     int32_t run(const Plan& pl, const void* d_scalars, const void* d_table, uint64_t table_n,
                 uint8_t* out96);
 
+    // zkmsm_msm_batch_resident_device without any allocation: the call's
+    // buffers come from a fixed device array of kNoAllocScratchBytes (one
+    // per device, shared by every engine on it; calls on one device are
+    // serialised), the host side from capacity reserved when the engine was
+    // created.  At most kNoAllocMaxPoints points in all and kNoAllocMaxMsms
+    // MSMs, and the call's buffers must fit the array: otherwise
+    // kInvalidInput.  The window combine runs on the device.
+    static constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+    static constexpr uint32_t kNoAllocMaxPoints = 1u << 16;
+    static constexpr uint32_t kNoAllocMaxMsms = 64;
+    static constexpr uint32_t kNoAllocMaxSegs = 1024;
+    int32_t msm_resident_noalloc(const void* d_scalars, const uint32_t* counts, uint32_t nmsm,
+                                 uint8_t* out96);
+
   private:
     Engine() = default;
     int32_t run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, bool graph,
                    const void* d_scalars, const void* d_table, uint64_t table_n, uint8_t* out96);
+    int32_t noalloc_setup();  // the device array's address, host capacity
     int32_t ensure_gtab();
     int32_t ensure_gtab_grumpkin();
     int32_t ensure_gtab_g2();
@@ -301,6 +316,11 @@ class Engine {
     hipEvent_t stage_ev_[kStageSlots] = {};
     hipEvent_t ev_[kNumStages + 1] = {};
     hipEvent_t ev_tail_ = nullptr;  // after the device tail, before the copy-out
+    // no-allocation path: this device's scratch array, a plan and host
+    // staging whose capacity was reserved at creation
+    uint8_t* scratch_ = nullptr;
+    Plan na_plan_;
+    std::vector<uint8_t> na_host_;
 };
 
 }  // namespace zk
