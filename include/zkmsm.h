@@ -221,6 +221,13 @@ ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32
  * shape captures again.  zkmsm_graph_captures counts the captures. */
 ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
 
+/* A large call runs as a pipeline over parts of its (MSM, window) segments:
+ * the bucket sort of part p + 1 on one stream while part p accumulates on
+ * another, and part p's bucket reduction on a third.  parts = 0 (default)
+ * picks 1, 2 or 4 parts by size; 1..8 forces that many (at most one per
+ * segment).  Results are the same bytes for any value. */
+ZkMetalStatus zkmsm_set_pipeline_parts(ZkMetalMSMEngine engine, uint32_t parts);
+
 /* Where a call's window combine (Horner over the windows) and affine
  * normalisation run: 0 auto (the device in graph mode and for calls of 16 or
  * more MSMs, the host otherwise), 1 the host, 2 the device.  Results are the

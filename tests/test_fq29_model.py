@@ -545,3 +545,82 @@ def test_bounds_at_extremes():
         out5, _ = madd_run(out4, False, to_int(q[0]), to_int(q[1]))
         assert affine(out5) == B.add(B.add(pt, q), q)
         assert affine(add(out4, out4)) == B.add(affine(out4), affine(out4))
+
+
+def jac_dbl(X, Y, Z):
+    """Arith::qjac_dbl (dbl-2009-l, a = 0): X, Y, Z < 2p in and out."""
+    lt(X, 2), lt(Y, 2), lt(Z, 2)
+    A, B_ = fq_sqr(X), fq_sqr(Y)
+    XB = fq_add(X, B_)
+    C, T = fq_sqr(B_), fq_sqr(XB)
+    D0 = fq_sub(4, T, fq_add(A, C))
+    D = fq_add(D0, D0)
+    lt(D, 11)
+    D = fq_csub(2, fq_csub(4, fq_csub(8, D)))
+    lt(D, 2)
+    E = fq_add(fq_add(A, A), A)
+    lt(E, 4.5)
+    Y2 = fq_add(Y, Y)
+    F, Z3 = fq_mul(E, E), fq_mul(Y2, Z)
+    X3 = fq_csub(2, fq_csub(4, fq_sub(4, F, fq_add(D, D))))
+    lt(X3, 2)
+    C8 = fq_add(C, C)
+    C8 = fq_add(C8, C8)
+    C8 = fq_add(C8, C8)
+    lt(C8, 12)
+    Y3 = fq_mul_sub(13, E, fq_sub(2, D, X3), C8)
+    lt(Y3, 14.5)
+    Y3 = fq_csub(2, fq_csub(4, fq_csub(8, Y3)))
+    lt(Y3, 2), lt(Z3, 1.5)
+    return X3, Y3, Z3
+
+
+def dbl_k(pt, k):
+    """Arith::qxyzz_dbl_k: XYZZ -> Jacobian (X ZZ^2, Y ZZZ^2, ZZZ), k
+    doublings, back to XYZZ (Z^2, Z^3)."""
+    X, Y, ZZ, ZZZ = pt
+    lt(X, 6.5), lt(Y, 4), lt(ZZ, 2), lt(ZZZ, 2)
+    ZZ2, ZZZ2 = fq_sqr(ZZ), fq_sqr(ZZZ)
+    X, Y, Z = fq_mul(X, ZZ2), fq_mul(Y, ZZZ2), ZZZ
+    for _ in range(k):
+        X, Y, Z = jac_dbl(X, Y, Z)
+    ZZ = fq_sqr(Z)
+    return X, Y, ZZ, fq_mul(ZZ, Z)
+
+
+def test_repeated_doubling_chain():
+    """The device tail's 2^k P (Jacobian chain between XYZZ forms) equals
+    2^k P of the group law, from accumulators at the top of their bounds and
+    from the identity, with every bound checked on the way."""
+    rnd = random.Random(13)
+
+    def lift(v, bound):
+        r = v * R % P
+        while r + P < bound * P:
+            r += P
+        return limbs(r)
+
+    for k in (1, 2, 13, 16):
+        for _ in range(4):
+            pt = B.mul(B.G, rnd.randrange(1, B.R_ORDER))
+            z = rnd.randrange(1, P)
+            zz, zzz = z * z % P, z * z * z % P
+            acc = (lift(pt[0] * zz % P, 6.5), lift(pt[1] * zzz % P, 4), lift(zz, 2), lift(zzz, 2))
+            assert affine(dbl_k(acc, k)) == B.mul(pt, 1 << k)
+        idp = (lift(0, 1), limbs(5), lift(0, 2), lift(0, 2))  # ZZ = ZZZ = p: the identity
+        assert affine(dbl_k(idp, k)) is None
+    # a wrong constant in the chain is caught: 8C replaced by 4C
+    pt = B.mul(B.G, 12345)
+    X, Y, Z = to_int(pt[0]), to_int(pt[1]), to_int(1)
+    A, B_ = fq_sqr(X), fq_sqr(Y)
+    C = fq_sqr(B_)
+    D0 = fq_sub(4, fq_sqr(fq_add(X, B_)), fq_add(A, C))
+    D = fq_csub(2, fq_csub(4, fq_csub(8, fq_add(D0, D0))))
+    E = fq_add(fq_add(A, A), A)
+    X3 = fq_csub(2, fq_csub(4, fq_sub(4, fq_mul(E, E), fq_add(D, D))))
+    C4 = fq_add(fq_add(C, C), fq_add(C, C))
+    Y3 = fq_mul_sub(13, E, fq_sub(2, D, X3), C4)
+    Z3 = fq_mul(fq_add(Y, Y), Z)
+    z3 = from_int(Z3)
+    bad = (from_int(X3) * pow(z3 * z3, -1, P) % P, from_int(Y3) * pow(z3 * z3 * z3, -1, P) % P)
+    assert bad != B.add(pt, pt)

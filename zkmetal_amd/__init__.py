@@ -95,6 +95,7 @@ def lib():
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
         "zkmsm_set_workspace_limit": (i32, [vp, u64]),
         "zkmsm_set_tail_mode": (i32, [vp, i32]),
+        "zkmsm_set_pipeline_parts": (i32, [vp, u32]),
         "zkmsm_workspace_bytes": (u64, [vp]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
@@ -540,6 +541,11 @@ class MetalMSM:
         """Where the window combine and normalisation run: TAIL_AUTO (device in
         graph mode and for calls of >= 16 MSMs), TAIL_HOST or TAIL_DEVICE."""
         _check(lib().zkmsm_set_tail_mode(self.handle, mode), "set_tail_mode")
+
+    def set_pipeline_parts(self, parts):
+        """Parts a large call is split into for the sort / accumulation
+        overlap (0 = by size, 1..8 = that many); results do not change."""
+        _check(lib().zkmsm_set_pipeline_parts(self.handle, parts), "set_pipeline_parts")
 
     def set_workspace_limit(self, nbytes):
         """Cap the device memory this engine's buffers may grow to (0 = none);

@@ -521,6 +521,14 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
     return e->set_window_table(c_by_log2n, len, false);
 }
 
+ZkMetalStatus zkmsm_set_pipeline_parts(ZkMetalMSMEngine engine, uint32_t parts) {
+    Engine* e = as_engine(engine);
+    if (!e || parts > zk::kMaxParts) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    e->set_parts(parts);
+    return ZKMETAL_SUCCESS;
+}
+
 ZkMetalStatus zkmsm_set_tail_mode(ZkMetalMSMEngine engine, int32_t mode) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;

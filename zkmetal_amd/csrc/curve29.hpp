@@ -622,15 +622,18 @@ struct Arith {
     static __device__ __forceinline__ bool qxyzz_is_id(const QXyzz& p) { return fq_is_zero2(p.ZZ); }
 
     // doubling core: X, Y < 4p.  Returns X3 < 2.5p, Y3 < 3.5p; V, W < 1.5p.
+    // (independent products in pairs: the doubling also runs alone in one
+    // lane, in the device tail, where a second chain hides the latency)
     static __device__ __forceinline__ void qdbl_core(const Fq& X, const Fq& Y, Fq& X3, Fq& Y3, Fq& V, Fq& W) {
         const Fq U = fq_add(Y, Y);  // < 8p
-        V = fq_sqr(U);
-        W = fq_mul(U, V);
-        const Fq S = fq_mul(X, V);
-        const Fq xx = fq_sqr(X);
+        Fq xx, S;
+        fq_sqr_x2(U, V, X, xx);                                       // U^2, X^2
+        fq_mul_sub_x2<0, 1, 0, 1>(U, V, U, W, X, V, X, S);            // U V, X V
         const Fq M = fq_add(fq_add(xx, xx), xx);                    // < 4.5p
         X3 = fq_csub<2>(fq_sub<3>(fq_sqr(M), fq_add(S, S)));        // < 4.5p -> < 2.5p
-        Y3 = fq_sub<2>(fq_mul(M, fq_sub<3>(S, X3)), fq_mul(W, Y));  // < 3.5p
+        Fq MS, WY;
+        fq_mul_sub_x2<0, 1, 0, 1>(M, fq_sub<3>(S, X3), M, MS, W, Y, W, WY);
+        Y3 = fq_sub<2>(MS, WY);                                     // < 3.5p
     }
 
     // 2*(x, y) for an affine, non-identity point
@@ -647,6 +650,54 @@ struct Arith {
         qdbl_core(p.X, p.Y, r.X, r.Y, V, W);
         r.ZZ = fq_mul(V, p.ZZ);
         r.ZZZ = fq_mul(W, p.ZZZ);
+        return r;
+    }
+
+    // ---- repeated doubling (the device tail's window combine).  A chain of
+    // doublings runs in Jacobian coordinates (x = X/Z^2, y = Y/Z^3;
+    // dbl-2009-l for a = 0: 2M + 5S against XYZZ's 6M + 3S), with the
+    // independent products in pairs (one lane runs the chain alone, so a
+    // second multiply-add chain fills the dependent-issue latency).
+    // Bounds: X, Y, Z < 2p in and out; Z = 0 mod p is the identity.
+    static __device__ __forceinline__ void qjac_dbl(Fq& X, Fq& Y, Fq& Z) {
+        Fq A, B, C, T;
+        fq_sqr_x2(X, A, Y, B);                                        // < 1.5p each
+        const Fq XB = fq_add(X, B);                                   // < 3.5p
+        fq_sqr_x2(B, C, XB, T);                                       // < 1.5p each
+        const Fq D0 = fq_sub<4>(T, fq_add(A, C));                     // (X+B)^2 - A - C, < 5.5p
+        Fq D = fq_add(D0, D0);                                        // < 11p
+        D = fq_csub<2>(fq_csub<4>(fq_csub<8>(D)));                    // < 2p
+        const Fq E = fq_add(fq_add(A, A), A);                         // 3 X^2, < 4.5p
+        const Fq Y2 = fq_add(Y, Y);                                   // < 4p
+        Fq EE, Z3;
+        fq_mul_sub_x2<0, 1, 0, 1>(E, E, E, EE, Y2, Z, Y2, Z3);         // E^2, 2 Y Z: < 1.5p each
+        Fq X3 = fq_csub<2>(fq_csub<4>(fq_sub<4>(EE, fq_add(D, D))));  // E^2 - 2D: < 5.5p -> < 2p
+        Fq C8 = fq_add(C, C);
+        C8 = fq_add(C8, C8);
+        C8 = fq_add(C8, C8);                                          // 8C < 12p
+        // E (D - X3) - 8C + 13p: < 14.5p -> < 2p
+        Fq Y3 = fq_mul_sub<13, 1>(E, fq_sub<2>(D, X3), C8);
+        Y3 = fq_csub<2>(fq_csub<4>(fq_csub<8>(Y3)));
+        X = X3;
+        Y = Y3;
+        Z = Z3;
+    }
+
+    // 2^k P for an XYZZ point (X < 6.5p, Y < 4p, ZZ, ZZZ < 2p): to Jacobian
+    // (X ZZ^2, Y ZZZ^2, ZZZ), k doublings, back (ZZ = Z^2, ZZZ = Z^3).  The
+    // identity (ZZ = ZZZ = 0) stays the identity.  Out: X, Y < 2p, ZZ, ZZZ < 1.5p.
+    static __device__ __forceinline__ QXyzz qxyzz_dbl_k(const QXyzz& p, uint32_t k) {
+        Fq ZZ2, ZZZ2, X, Y;
+        fq_sqr_x2(p.ZZ, ZZ2, p.ZZZ, ZZZ2);
+        fq_mul_sub_x2<0, 1, 0, 1>(p.X, ZZ2, p.X, X, p.Y, ZZZ2, p.Y, Y);  // < 1.5p each
+        Fq Z = p.ZZZ;                                                     // < 2p
+#pragma unroll 1
+        for (uint32_t i = 0; i < k; i++) qjac_dbl(X, Y, Z);
+        QXyzz r;
+        r.X = X;
+        r.Y = Y;
+        r.ZZ = fq_sqr(Z);
+        r.ZZZ = fq_mul(r.ZZ, Z);
         return r;
     }
 

@@ -110,12 +110,22 @@ static void plan_reset(Plan& pl) {
     pl.chunk_seg.clear();
     pl.scalars = pl.entries = pl.buckets = pl.lanes = pl.groups = pl.rc_waves = 0;
     pl.tiles = pl.chunks = pl.T = pl.dtiles = 0;
+    pl.nparts = 1;
     pl.curve = CurveId::Bn254;
     pl.G = 8;
 }
 
+// parts for a call of `entries` sorted entries: the exposed sort is the first
+// part's, and every part adds a round of launches and merge records
+uint32_t default_parts(uint64_t entries) {
+    // measured: a 2^22-point MSM (2^26 entries) runs best in 2 parts, the
+    // 64-MSM batch (2^27.7 entries) in 3; more parts add merge records and
+    // launches faster than they hide sort time
+    return entries >= (1ull << 27) ? 3u : entries >= (1ull << 24) ? 2u : 1u;
+}
+
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
-              Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv) {
+              Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv, uint32_t nparts) {
     plan_reset(pl);
     pl.curve = cv;
     const uint32_t eps = entries_per_point(cv);
@@ -163,14 +173,53 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     // 32-bit positions / bucket ids on the device
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
-    // accumulation: at most lane_cap lanes (one resident round of waves --
-    // every segment rounds its lane count up, hence the - nseg), >= 4 entries
-    // each, and (up to 32) at least half a bucket's mean entry count, so that
-    // a bucket's partials span few lanes and merge without a long-run pass
-    const uint64_t half_bucket = pl.entries / (2 * std::max<uint64_t>(1, pl.buckets));
-    const uint64_t cap = lane_cap > 2ull * nseg + 64 ? lane_cap - nseg : std::max<uint64_t>(lane_cap / 2, 64);
-    pl.T = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(4, std::min<uint64_t>(32, half_bucket)),
-                                        (pl.entries + cap - 1) / cap);
+    // parts: consecutive segment ranges of about entries / P entries each
+    uint32_t P = nparts ? nparts : default_parts(pl.entries);
+    P = std::max<uint32_t>(1, std::min<uint32_t>({P, kMaxParts, nseg}));
+    {
+        uint32_t s = 0;
+        uint64_t done = 0;
+        pl.nparts = 0;
+        for (uint32_t k = 0; k < P && s < nseg; k++) {
+            PlanPart& pt = pl.parts[pl.nparts++];
+            pt = PlanPart{};
+            pt.seg_lo = s;
+            const uint64_t target = pl.entries * (k + 1) / P;
+            // at least one segment, and leave one for each later part
+            do {
+                done += pl.segs[s].len;
+                s++;
+            } while (s < nseg - (P - 1 - k) && done < target);
+            if (k == P - 1) {
+                for (; s < nseg; s++) done += pl.segs[s].len;
+            }
+            pt.seg_hi = s;
+        }
+    }
+    // accumulation, per part: at most lane_cap lanes (one resident round of
+    // waves -- every segment rounds its lane count up, hence the - nseg), >= 4
+    // entries each, and (up to 32) at least half a bucket's mean entry count,
+    // so that a bucket's partials span few lanes and merge without a long-run
+    // pass
+    for (uint32_t k = 0; k < pl.nparts; k++) {
+        PlanPart& pt = pl.parts[k];
+        uint64_t ent = 0, bk = 0;
+        for (uint32_t s = pt.seg_lo; s < pt.seg_hi; s++) {
+            ent += pl.segs[s].len;
+            bk += pl.segs[s].nbk;
+        }
+        const uint64_t ns_p = pt.seg_hi - pt.seg_lo;
+        const uint64_t half_bucket = ent / (2 * std::max<uint64_t>(1, bk));
+        const uint64_t cap = lane_cap > 2ull * ns_p + 64 ? lane_cap - ns_p : std::max<uint64_t>(lane_cap / 2, 64);
+        pt.T = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(4, std::min<uint64_t>(32, half_bucket)),
+                                            (ent + cap - 1) / cap);
+        pt.entries = ent;
+        pt.tile_lo = pl.segs[pt.seg_lo].tile0;
+        pt.tile_hi = pl.segs[pt.seg_hi - 1].tile0 + pl.segs[pt.seg_hi - 1].ntiles;
+        pt.chunk_lo = pl.segs[pt.seg_lo].chunk0;
+        pt.chunk_hi = pl.segs[pt.seg_hi - 1].chunk0 + pl.segs[pt.seg_hi - 1].nchunks;
+    }
+    pl.T = pl.parts[0].T;
     // bucket reduction: per buckets per lane before the shuffle tree; 8 keeps
     // the tree shallow for small calls, larger batches (throughput-bound) use
     // up to 32 so that fewer additions go to tree levels
@@ -179,14 +228,21 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
     pl.seg_rcw0.resize(nseg + 1);
+    uint32_t part = 0;
     for (uint32_t s = 0; s < nseg; s++) {
         SegDesc& sg = pl.segs[s];
+        if (s == pl.parts[part].seg_hi) part++;
+        PlanPart& pt = pl.parts[part];
+        if (s == pt.seg_lo) {
+            pt.lane_lo = pl.lanes;
+            pt.rcw_lo = pl.rc_waves;
+        }
         sg.lane0 = pl.lanes;
         sg.grp0 = pl.groups;
         pl.seg_lane0[s] = pl.lanes;
         pl.seg_grp0[s] = pl.groups;
         pl.seg_rcw0[s] = pl.rc_waves;
-        pl.lanes += (sg.len + pl.T - 1) / pl.T;
+        pl.lanes += (sg.len + pt.T - 1) / pt.T;
         // bucket reduction: 2^h rows of 2^g buckets and 2^g columns of 2^h,
         // max(1, len/8) lanes per row / column, rows and columns wave-aligned
         const uint32_t cb = 31 - __builtin_clz(sg.nbk), g = cb / 2, h = cb - g;
@@ -196,6 +252,10 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         pl.groups += (1u << h) + (1u << g);
         for (uint32_t t = 0; t < sg.ntiles; t++) pl.tile_seg.push_back(s);
         for (uint32_t t = 0; t < sg.nchunks; t++) pl.chunk_seg.push_back(s);
+        if (s + 1 == pt.seg_hi) {
+            pt.lane_hi = pl.lanes;
+            pt.rcw_hi = pl.rc_waves;
+        }
     }
     pl.seg_lane0[nseg] = pl.lanes;
     pl.seg_grp0[nseg] = pl.groups;
@@ -214,9 +274,24 @@ int32_t Engine::create(Engine** out) {
         return kNoGpu;
     }
     Engine* e = new Engine();
-    if (hipGetDevice(&e->dev_) != hipSuccess || hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&e->dev_) != hipSuccess || hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->stream_sort_, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return kGpuError;
+    }
+    {
+        bool ok = hipEventCreate(&e->ev_dec_) == hipSuccess;
+        for (uint32_t p = 0; p < kMaxParts; p++)
+            for (hipEvent_t* ev : {&e->ev_s0_[p], &e->ev_s1_[p], &e->ev_a0_[p], &e->ev_a1_[p], &e->ev_m1_[p],
+                                   &e->ev_r0_[p], &e->ev_r1_[p]})
+                ok = ok && hipEventCreate(ev) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&e->dep_fork_, hipEventDisableTiming) == hipSuccess;
+        for (uint32_t p = 0; p < kMaxParts; p++)
+            ok = ok && hipEventCreateWithFlags(&e->dep_sort_[p], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            delete e;
+            return kGpuError;
+        }
     }
     e->account_buffers();
     (void)e->noalloc_setup();  // on failure the no-allocation entry point reports kGpuError
@@ -253,6 +328,14 @@ void Engine::account_buffers() {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (stream_sort_) (void)hipStreamSynchronize(stream_sort_);
+    for (uint32_t p = 0; p < kMaxParts; p++)
+        for (hipEvent_t ev : {ev_s0_[p], ev_s1_[p], ev_a0_[p], ev_a1_[p], ev_m1_[p], ev_r0_[p], ev_r1_[p],
+                              dep_sort_[p]})
+            if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {ev_dec_, dep_fork_})
+        if (ev) (void)hipEventDestroy(ev);
+    if (stream_sort_) (void)hipStreamDestroy(stream_sort_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto& ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
@@ -420,13 +503,13 @@ WsSizes ws_sizes(const Plan& pl, bool dev_tail) {
     z.rc = pl.groups * ptb;
     // bit-plane points: host form for the host tail, stored form for the device's
     z.bits = nseg * 16 * (dev_tail ? ptb : hpb);
-    z.err = 16;
+    z.err = 4 * (1 + 2 * kMaxParts) + 12;  // the decomposition flag, two merge counters per part
     z.wins = dev_tail ? nseg * ptb : 0;
     z.res = dev_tail ? nmsm * result_bytes(pl.curve) : 0;
     // copied back: the results (device tail) or the bit planes, then err, then
     // the per-segment valid-entry counts
     z.back = dev_tail ? z.res : z.bits;
-    z.host = z.back + 16 + nseg * 4;
+    z.host = z.back + z.err + nseg * 4;
     const uint64_t parts[] = {z.desc, z.digits, z.keys, z.vals, z.keys, z.vals, z.hist, z.part,
                               z.segvalid, z.buckets, z.rec_keys, z.rec_pts, z.rec_keys, z.rc, z.bits,
                               z.err, z.wins, z.res};
@@ -519,53 +602,85 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     const uint32_t* d_chunk_seg = (const uint32_t*)(db + L.chunk_seg);
     uint32_t* d_err = (uint32_t*)ws.err;
     hipStream_t st = stream_;
-
-    // merge lists: a listed run has more than kRunLimit records, so each list
-    // holds at most nrec / (kRunLimit + 1) runs
-    const uint32_t list_cap = (uint32_t)std::min<uint64_t>(2 * pl.lanes / (kRunLimit + 1) + 1, 1u << 29);
+    const uint32_t P = pl.nparts;
+    // merge lists of part p: a listed run has more than kRunLimit records, so
+    // each list holds at most nrec / (kRunLimit + 1) runs
+    uint32_t list_cap[kMaxParts];
+    for (uint32_t p = 0; p < P; p++)
+        list_cap[p] = (uint32_t)std::min<uint64_t>(
+            2 * (pl.parts[p].lane_hi - pl.parts[p].lane_lo) / (kRunLimit + 1) + 1, 1u << 29);
     uint8_t* hw = ws.hw;
-    // the device part of the call: one launch sequence on the engine stream.
-    // Stage events are recorded only outside graph capture.
+    // The device part of the call.  Stream A (the engine stream): descriptors,
+    // decomposition, then per part the accumulation and the merge, then the
+    // bucket reduction and the device tail; stream S: the two sort passes of
+    // each part, so that part p + 1 sorts while part p accumulates (the
+    // accumulation leaves registers and LDS for the sort's workgroups).
+    // Timing events are recorded only outside graph capture; the dependency
+    // events are part of a captured graph.
     auto enqueue = [&](bool events) -> int32_t {
-        auto mark = [&](int i) -> hipError_t { return events ? hipEventRecord(ev_[i], st) : hipSuccess; };
+        auto rec = [&](hipEvent_t e, hipStream_t s) -> hipError_t { return events ? hipEventRecord(e, s) : hipSuccess; };
+        // one part: everything on stream A (no cross-stream waits)
+        hipStream_t S = P > 1 ? stream_sort_ : st;
         ZK_TRY(hipMemcpyAsync(ws.desc, ws.hdesc, L.total, hipMemcpyHostToDevice, st));
-        launch_fill_zero(ws.err, 16, st);
-        ZK_TRY(mark(0));
+        launch_fill_zero(ws.err, z.err, st);
+        ZK_TRY(rec(ev_[0], st));
         // the decomposition also counts the first sort pass's tile histograms
         launch_decompose(pl.curve, d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
                          ws.digits, (uint32_t*)ws.hist, d_err, st);
-        ZK_TRY(mark(1));
-        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, ws.digits,
-                         nullptr, 0, (uint32_t*)ws.hist, (uint32_t*)ws.part, (uint32_t*)ws.segvalid,
-                         (uint16_t*)ws.keysA, (uint32_t*)ws.valsA, st, false);
-        launch_sort_pass(d_segs, nseg, d_tile_seg, pl.tiles, d_chunk_seg, pl.chunks, ws.keysA,
-                         (const uint32_t*)ws.valsA, 1, (uint32_t*)ws.hist, (uint32_t*)ws.part,
-                         (uint32_t*)ws.segvalid, (uint16_t*)ws.keysB, (uint32_t*)ws.valsB, st);
+        ZK_TRY(rec(ev_dec_, st));
         launch_fill_zero(ws.buckets, pl.buckets * ptb, st);
-        ZK_TRY(mark(2));
-        launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)ws.segvalid,
-                          (const uint16_t*)ws.keysB, (const uint32_t*)ws.valsB, d_table, pl.T, pl.lanes,
-                          ws.buckets, (uint32_t*)ws.recAk, ws.recAp, st);
-        ZK_TRY(mark(3));
-        // bucket partials that crossed lane boundaries: short runs by their first
-        // lane, longer runs (heavy buckets) by lane groups.
-        launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk, ws.recAp, 2 * pl.lanes, kRunLimit, ws.buckets,
-                          d_err + 1, (uint32_t*)ws.recBk, list_cap, st);
-        ZK_TRY(mark(4));
-        launch_bucket_reduce(pl.curve, d_segs, d_seg_rcw0, d_seg_grp0, nseg, pl.rc_waves, pl.G, ws.buckets, ws.rc,
-                             ws.bits, !dev_tail, st);
-        ZK_TRY(mark(5));
+        if (P > 1) {
+            ZK_TRY(hipEventRecord(dep_fork_, st));
+            ZK_TRY(hipStreamWaitEvent(S, dep_fork_, 0));
+        }
+        for (uint32_t p = 0; p < P; p++) {
+            const PlanPart& pt = pl.parts[p];
+            const uint32_t ns_p = pt.seg_hi - pt.seg_lo;
+            ZK_TRY(rec(ev_s0_[p], S));
+            launch_sort_pass(d_segs, pt.seg_lo, ns_p, d_tile_seg, pt.tile_lo, pt.tile_hi - pt.tile_lo, d_chunk_seg,
+                             pt.chunk_lo, pt.chunk_hi - pt.chunk_lo, ws.digits, nullptr, 0, (uint32_t*)ws.hist,
+                             (uint32_t*)ws.part, (uint32_t*)ws.segvalid, (uint16_t*)ws.keysA,
+                             (uint32_t*)ws.valsA, S, false);
+            launch_sort_pass(d_segs, pt.seg_lo, ns_p, d_tile_seg, pt.tile_lo, pt.tile_hi - pt.tile_lo, d_chunk_seg,
+                             pt.chunk_lo, pt.chunk_hi - pt.chunk_lo, ws.keysA, (const uint32_t*)ws.valsA, 1,
+                             (uint32_t*)ws.hist, (uint32_t*)ws.part, (uint32_t*)ws.segvalid,
+                             (uint16_t*)ws.keysB, (uint32_t*)ws.valsB, S);
+            ZK_TRY(rec(ev_s1_[p], S));
+            if (P > 1) ZK_TRY(hipEventRecord(dep_sort_[p], S));
+        }
+        for (uint32_t p = 0; p < P; p++) {
+            const PlanPart& pt = pl.parts[p];
+            if (P > 1) ZK_TRY(hipStreamWaitEvent(st, dep_sort_[p], 0));
+            ZK_TRY(rec(ev_a0_[p], st));
+            launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)ws.segvalid,
+                              (const uint16_t*)ws.keysB, (const uint32_t*)ws.valsB, d_table, pt.T, pt.lane_lo,
+                              pt.lane_hi, ws.buckets, (uint32_t*)ws.recAk, ws.recAp, st);
+            ZK_TRY(rec(ev_a1_[p], st));
+            // bucket partials that crossed lane boundaries: short runs by their
+            // first lane, longer runs (heavy buckets) by lane groups
+            launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk + 2 * pt.lane_lo,
+                              (const uint8_t*)ws.recAp + 2 * pt.lane_lo * ptb, 2 * (pt.lane_hi - pt.lane_lo),
+                              kRunLimit, ws.buckets, d_err + 1 + 2 * p, (uint32_t*)ws.recBk, list_cap[p], st);
+            ZK_TRY(rec(ev_m1_[p], st));
+        }
+        // the bucket reduction of all parts at once, after the last merge: as
+        // a kernel beside the next part's accumulation it finds no free slots
+        // (its waves need as many registers as the accumulation's) and
+        // measured slower
+        ZK_TRY(rec(ev_r0_[0], st));
+        launch_bucket_reduce(pl.curve, d_segs, d_seg_rcw0, d_seg_grp0, nseg, 0, nseg, 0, pl.rc_waves, pl.G,
+                             ws.buckets, ws.rc, ws.bits, !dev_tail, st);
+        ZK_TRY(rec(ev_r1_[0], st));
         if (dev_tail) {
             launch_tail(pl.curve, d_segs, d_msms, nseg, nmsm, ws.bits, ws.wins, ws.res, st);
-            if (events) ZK_TRY(hipEventRecord(ev_tail_, st));
-            ZK_TRY(hipMemcpyAsync(hw, ws.res, back_bytes, hipMemcpyDeviceToHost, st));
-        } else {
-            ZK_TRY(hipMemcpyAsync(hw, ws.bits, back_bytes, hipMemcpyDeviceToHost, st));
+            ZK_TRY(rec(ev_tail_, st));
         }
-        // err_[0]: decomposition error flag; err_[1..2]: long / mid run counts of
-        // the merge (a count above list_cap means listed runs were dropped)
-        ZK_TRY(hipMemcpyAsync(hw + back_bytes, ws.err, 12, hipMemcpyDeviceToHost, st));
-        ZK_TRY(hipMemcpyAsync(hw + back_bytes + 16, ws.segvalid, (uint64_t)nseg * 4,
+        ZK_TRY(rec(ev_[kStCopyOut], st));
+        ZK_TRY(hipMemcpyAsync(hw, dev_tail ? ws.res : ws.bits, back_bytes, hipMemcpyDeviceToHost, st));
+        // err[0]: decomposition error flag; err[1 + 2p], err[2 + 2p]: long / mid
+        // run counts of part p's merge (above list_cap: listed runs were dropped)
+        ZK_TRY(hipMemcpyAsync(hw + back_bytes, ws.err, 4 * (1 + 2 * P), hipMemcpyDeviceToHost, st));
+        ZK_TRY(hipMemcpyAsync(hw + back_bytes + z.err, ws.segvalid, (uint64_t)nseg * 4,
                               hipMemcpyDeviceToHost, st));
         return kOk;
     };
@@ -586,6 +701,10 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                            (uint64_t)pl.G, (uint64_t)nseg})
             key.push_back(v);
         key.push_back(table_n);  // places the table's identity bitmap
+        for (uint32_t p = 0; p < P; p++) {
+            const PlanPart& pt = pl.parts[p];
+            for (uint64_t v : {(uint64_t)pt.seg_hi, (uint64_t)pt.T, pt.lane_hi, pt.rcw_hi}) key.push_back(v);
+        }
         key.push_back(dev_tail ? 1 : 0);
         for (const void* p : std::initializer_list<const void*>{
                  d_scalars, d_table, ws.desc, ws.hdesc, ws.digits, ws.keysA, ws.valsA, ws.keysB, ws.valsB,
@@ -626,20 +745,31 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     }
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
-    uint32_t err[3];
-    memcpy(err, hw + back_bytes, 12);
-    if (err[0] || err[1] > list_cap || err[2] > list_cap) return kGpuError;
+    uint32_t err[1 + 2 * kMaxParts];
+    memcpy(err, hw + back_bytes, 4 * (1 + 2 * P));
+    if (err[0]) return kGpuError;
+    for (uint32_t p = 0; p < P; p++)
+        if (err[1 + 2 * p] > list_cap[p] || err[2 + 2 * p] > list_cap[p]) return kGpuError;
 
-    for (int s = 0; s < kNumStages; s++) {
-        float ms = 0;
-        if (!graph) (void)hipEventElapsedTime(&ms, ev_[s], ev_[s + 1]);
-        timing_.stage_ms[s] = ms;  // stages are not split inside a graph
-    }
+    // stage times: each stage's kernels summed over the parts (they overlap
+    // across streams; a graph replay has only the device total)
+    for (int s = 0; s < kNumStages; s++) timing_.stage_ms[s] = 0;
     timing_.device_tail_ms = 0;
-    if (dev_tail && !graph) {
-        // split stage 5 (ev_[5] .. ev_[6]) into the tail and the copy-out
-        (void)hipEventElapsedTime(&timing_.device_tail_ms, ev_[5], ev_tail_);
-        (void)hipEventElapsedTime(&timing_.stage_ms[kStCopyOut], ev_tail_, ev_[kNumStages]);
+    if (!graph) {
+        auto span = [](hipEvent_t a, hipEvent_t b) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            return ms;
+        };
+        timing_.stage_ms[kStDecompose] = span(ev_[0], ev_dec_);
+        for (uint32_t p = 0; p < P; p++) {
+            timing_.stage_ms[kStSort] += span(ev_s0_[p], ev_s1_[p]);
+            timing_.stage_ms[kStAccumulate] += span(ev_a0_[p], ev_a1_[p]);
+            timing_.stage_ms[kStMerge] += span(ev_a1_[p], ev_m1_[p]);
+        }
+        timing_.stage_ms[kStReduce] = span(ev_r0_[0], ev_r1_[0]);
+        if (dev_tail) timing_.device_tail_ms = span(ev_r1_[0], ev_tail_);
+        timing_.stage_ms[kStCopyOut] = span(ev_[kStCopyOut], ev_[kNumStages]);
     }
     (void)hipEventElapsedTime(&timing_.device_ms, ev_[0], ev_[kNumStages]);
 
@@ -661,7 +791,7 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     timing_.buckets = pl.buckets;
     timing_.lanes = pl.lanes;
     uint64_t valid = 0;
-    const uint32_t* sv = (const uint32_t*)(hw + back_bytes + 16);
+    const uint32_t* sv = (const uint32_t*)(hw + back_bytes + z.err);
     for (uint32_t s = 0; s < nseg; s++) valid += sv[s];
     timing_.valid_entries = valid;
     return kOk;
@@ -808,7 +938,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     }
     Plan pl;
     rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                   lane_cap(cv), cv);
+                   lane_cap(cv), cv, parts_req_);
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * input_point_bytes(cv));
     ZK_ALLOC(ptable_, table_bytes(total, cv));
@@ -860,7 +990,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                       lane_cap(cv), cv);
+                       lane_cap(cv), cv, parts_req_);
     if (rc != kOk) return rc;
     ZK_ALLOC(ptable_, table_bytes(total, cv));
     launch_build_table(cv, d_points, ptable_.p, total, stream_);
@@ -1164,7 +1294,8 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
-    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_);
+    int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_,
+                       CurveId::Bn254, parts_req_);
     if (rc != kOk) return rc;
     const void* d_sc = d_scalars;
     if (host_parts) {

@@ -7,6 +7,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -72,6 +73,15 @@ struct Timing {
     uint64_t valid_entries;  // sorted entries with a nonzero digit (mixed additions done)
 };
 
+// A call's segments split into consecutive parts: the sort of part p + 1
+// (its own stream) runs while part p accumulates.  Ranges are [lo, hi).
+constexpr uint32_t kMaxParts = 8;
+struct PlanPart {
+    uint32_t seg_lo, seg_hi, tile_lo, tile_hi, chunk_lo, chunk_hi;
+    uint32_t T;  // entries per accumulation lane in this part
+    uint64_t lane_lo, lane_hi, rcw_lo, rcw_hi, entries;
+};
+
 struct Plan {
     std::vector<MsmDesc> msms;
     std::vector<uint64_t> msm_scalar0;  // nmsm + 1
@@ -85,7 +95,9 @@ struct Plan {
     std::vector<uint32_t> tile_seg;
     std::vector<uint32_t> chunk_seg;
     uint64_t scalars = 0, entries = 0, buckets = 0, lanes = 0, groups = 0, rc_waves = 0;
-    uint32_t tiles = 0, chunks = 0, T = 0, dtiles = 0;
+    uint32_t tiles = 0, chunks = 0, T = 0, dtiles = 0;  // T: the first part's
+    uint32_t nparts = 1;
+    PlanPart parts[kMaxParts];
     CurveId curve = CurveId::Bn254;
     uint32_t G = 8;  // buckets per bucket-reduction lane (a power of 2, 8..32)
 };
@@ -112,10 +124,13 @@ WsSizes ws_sizes(const Plan& pl, bool dev_tail);
 // window width for an MSM of n points (cost model; overridable)
 uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
 
-// build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j
+// build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j;
+// nparts: how many parts to split the segments into (0: by size, see
+// default_parts; at most kMaxParts and the segment count)
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
               const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0,
-              uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254);
+              uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254, uint32_t nparts = 0);
+uint32_t default_parts(uint64_t entries);
 
 class Engine {
   public:
@@ -212,6 +227,9 @@ class Engine {
     // graph mode: the device launch sequence of a call is captured into a
     // hipGraph and replayed while calls keep the same shape and buffers
     void set_graph_mode(bool on) { graph_mode_ = on; }
+    // parts a call's segments are split into for the sort / accumulation
+    // overlap (0: by size); a measurement knob, results are the same
+    void set_parts(uint32_t p) { parts_req_ = std::min<uint32_t>(p, kMaxParts); }
     // where a call's window combine and normalisation run: kTailAuto (the
     // device in graph mode and for calls of at least kDeviceTailMinMsms MSMs,
     // else the host), kTailHost, kTailDevice.  BN254 G2 always uses the host.
@@ -316,6 +334,16 @@ class Engine {
     hipEvent_t stage_ev_[kStageSlots] = {};
     hipEvent_t ev_[kNumStages + 1] = {};
     hipEvent_t ev_tail_ = nullptr;  // after the device tail, before the copy-out
+    // the part pipeline: the sort stream; timing events per part (sort,
+    // accumulation and merge starts and ends; the reduction uses slot 0) and
+    // the dependency events between the two streams
+    hipStream_t stream_sort_ = nullptr;
+    hipEvent_t ev_dec_ = nullptr;
+    hipEvent_t ev_s0_[kMaxParts] = {}, ev_s1_[kMaxParts] = {}, ev_a0_[kMaxParts] = {},
+               ev_a1_[kMaxParts] = {}, ev_m1_[kMaxParts] = {}, ev_r0_[kMaxParts] = {}, ev_r1_[kMaxParts] = {};
+    hipEvent_t dep_fork_ = nullptr;
+    hipEvent_t dep_sort_[kMaxParts] = {};
+    uint32_t parts_req_ = 0;  // 0: default_parts
     // no-allocation path: this device's scratch array, a plan and host
     // staging whose capacity was reserved at creation
     uint8_t* scratch_ = nullptr;

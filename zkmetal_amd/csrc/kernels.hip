@@ -27,10 +27,6 @@
 #include "kernels.hpp"
 #include "plan.hpp"
 
-#ifndef ZK_ACC_PIPE3
-#define ZK_ACC_PIPE3 0
-#endif
-
 namespace zk {
 
 // BN254 G1 over Fp: the GLV curve of the MSM, the SRS import / generation and
@@ -133,6 +129,7 @@ struct G1Curve {
         store_xyzz(b, i, Arith<F>::qxyzz_to_xyzz(p));
     }
     static __device__ __forceinline__ Pt dbl(const Pt& p) { return Arith<F>::qxyzz_dbl(p); }
+    static __device__ __forceinline__ Pt dbl_k(const Pt& p, uint32_t k) { return Arith<F>::qxyzz_dbl_k(p, k); }
     // result slot i (96 B): the normalised Jacobian X | Y | Z in the 2^256
     // Montgomery form, canonical: (x, y, 1), the identity (1, 1, 0)
     static __device__ __forceinline__ void store_result(uint4* b, uint64_t i, const Pt& p) {
@@ -531,11 +528,12 @@ __device__ __forceinline__ uint32_t tile_len(const SegDesc& sg, uint32_t start, 
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const SegDesc* __restrict__ segs,
                                                             const uint32_t* __restrict__ tile_seg,
+                                                            uint32_t tile_lo,
                                                             const void* __restrict__ in, int pass,
                                                             const uint32_t* __restrict__ seg_valid,
                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[kSortWaves][256];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = tile_lo + blockIdx.x;
     const uint32_t s = tile_seg[t];
     const SegDesc sg = segs[s];
     const uint32_t tin = t - sg.tile0;
@@ -580,11 +578,11 @@ __device__ __forceinline__ void load16(const uint32_t* p, uint32_t v[16]) {
 }
 
 __global__ __launch_bounds__(256) void k_scan_up(const SegDesc* __restrict__ segs,
-                                                 const uint32_t* __restrict__ chunk_seg,
+                                                 const uint32_t* __restrict__ chunk_seg, uint32_t chunk_lo,
                                                  const uint32_t* __restrict__ hist,
                                                  uint32_t* __restrict__ part) {
     __shared__ uint32_t lds4[4];
-    const uint32_t c = blockIdx.x;
+    const uint32_t c = chunk_lo + blockIdx.x;
     const SegDesc sg = segs[chunk_seg[c]];
     const uint32_t M = 256u * sg.ntiles;
     const uint32_t i0 = (c - sg.chunk0) * kScanChunk + 16u * threadIdx.x;
@@ -600,10 +598,10 @@ __global__ __launch_bounds__(256) void k_scan_up(const SegDesc* __restrict__ seg
     if (threadIdx.x == 0) part[c] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_scan_spine(const SegDesc* __restrict__ segs,
+__global__ __launch_bounds__(256) void k_scan_spine(const SegDesc* __restrict__ segs, uint32_t seg_lo,
                                                     uint32_t* __restrict__ part) {
     __shared__ uint32_t lds4[4];
-    const SegDesc sg = segs[blockIdx.x];
+    const SegDesc sg = segs[seg_lo + blockIdx.x];
     uint32_t carry = 0;
     for (uint32_t b = 0; b < sg.nchunks; b += 256) {
         const uint32_t i = b + threadIdx.x;
@@ -616,12 +614,12 @@ __global__ __launch_bounds__(256) void k_scan_spine(const SegDesc* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(const SegDesc* __restrict__ segs,
-                                                   const uint32_t* __restrict__ chunk_seg,
+                                                   const uint32_t* __restrict__ chunk_seg, uint32_t chunk_lo,
                                                    uint32_t* __restrict__ hist,
                                                    const uint32_t* __restrict__ part, int pass,
                                                    uint32_t* __restrict__ seg_valid) {
     __shared__ uint32_t lds4[4];
-    const uint32_t c = blockIdx.x;
+    const uint32_t c = chunk_lo + blockIdx.x;
     const uint32_t s = chunk_seg[c];
     const SegDesc sg = segs[s];
     const uint32_t M = 256u * sg.ntiles;
@@ -656,7 +654,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const SegDesc* __restrict__ s
 // each digit's run to its global destination with consecutive threads
 // writing consecutive entries.
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
-    const SegDesc* __restrict__ segs, const uint32_t* __restrict__ tile_seg,
+    const SegDesc* __restrict__ segs, const uint32_t* __restrict__ tile_seg, uint32_t tile_lo,
     const void* __restrict__ in_keys, const uint32_t* __restrict__ in_vals, int pass,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ seg_valid,
     uint16_t* __restrict__ out_keys, uint32_t* __restrict__ out_vals) {
@@ -666,7 +664,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __shared__ uint32_t ldsw[kSortWaves];
     __shared__ uint16_t lkey[kSortTile];
     __shared__ uint32_t lval[kSortTile];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = tile_lo + blockIdx.x;
     const uint32_t s = tile_seg[t];
     const SegDesc sg = segs[s];
     const uint32_t tin = t - sg.tile0;
@@ -792,10 +790,10 @@ __global__ __launch_bounds__(256) void k_accumulate(
     const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_lane0, uint32_t nseg,
     const MsmDesc* __restrict__ msms, const uint32_t* __restrict__ seg_valid,
     const uint16_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const uint4* __restrict__ table, uint32_t T, uint64_t total_lanes, uint4* __restrict__ buckets,
-    uint32_t* __restrict__ rec_keys, uint4* __restrict__ rec_pts) {
-    const uint64_t L = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= total_lanes) return;
+    const uint4* __restrict__ table, uint32_t T, uint64_t lane_lo, uint64_t lane_hi,
+    uint4* __restrict__ buckets, uint32_t* __restrict__ rec_keys, uint4* __restrict__ rec_pts) {
+    const uint64_t L = lane_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= lane_hi) return;
     const uint32_t s = upper_index(seg_lane0, nseg, L);
     const SegDesc sg = segs[s];
     const uint64_t l = L - sg.lane0;
@@ -813,53 +811,6 @@ __global__ __launch_bounds__(256) void k_accumulate(
     bool first = true;
     typename C::Pt acc = C::run_start();
     bool acc_id = true;
-    if constexpr (ZK_ACC_PIPE3 && C::kSlots == 1) {
-    // software pipeline, three entries per iteration: before entry p's mixed
-    // addition the base of entry p+2 is gathered (its value was loaded one
-    // step earlier) into the register set that entry p-1 used, and the key and
-    // value of entry p+3 are loaded; each gather has two additions' time to
-    // arrive, and the three sets rotate by name, not by moves.  Indices clamp
-    // to the lane's last entry.
-    uint32_t k = cur;
-    typename C::Af q0, q1, q2;
-    {
-        const uint32_t v0 = vals[b], v1 = vals[min(b + 1, e - 1)];
-        q0 = C::load_tab(tb, v0 & 0x7fffffffu, v0 >> 31);
-        q1 = C::load_tab(tb, v1 & 0x7fffffffu, v1 >> 31);
-    }
-    uint32_t k1 = keys[min(b + 1, e - 1)];
-    const uint64_t p2 = min(b + 2, e - 1);
-    uint32_t kn = keys[p2], vn = vals[p2];
-    auto step = [&](uint64_t p, const typename C::Af& qa, typename C::Af& qc) {
-        const uint64_t pnn = min(p + 3, e - 1);
-        const uint32_t knn = keys[pnn], vnn = vals[pnn];
-        qc = C::load_tab(tb, vn & 0x7fffffffu, vn >> 31);
-        if (k != cur) {
-            // run `cur` ended at p-1 and is not the lane's last run
-            const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
-            if (first && head_open) {
-                put_record<C>(rec_keys, rec_pts, 2 * L, gk, true, acc);
-            } else {
-                C::store(buckets, gk, acc);
-                if (first) put_record<C>(rec_keys, rec_pts, 2 * L, gk, false, acc);
-            }
-            first = false;
-            acc = C::run_start();
-            acc_id = true;
-            cur = k;
-        }
-        C::madd_run(acc, acc_id, qa);
-        k = k1;
-        k1 = kn;
-        kn = knn;
-        vn = vnn;
-    };
-    for (uint64_t p = b; p < e; p += 3) {
-        step(p, q0, q2);
-        if (p + 1 < e) step(p + 1, q1, q0);
-        if (p + 2 < e) step(p + 2, q2, q1);
-    }
-    } else {
     // software pipeline: before entry p's mixed addition the base of entry
     // p+1 is gathered (its value was loaded one iteration earlier, so the
     // gather's address waits on nothing issued in this iteration) and the key
@@ -894,7 +845,6 @@ __global__ __launch_bounds__(256) void k_accumulate(
         kn = knn;
         vn = vnn;
         q = qn;
-    }
     }
     const bool tail_open = (e < segend) && keys[e] == cur;
     const uint32_t gk = (uint32_t)(sg.bkt0 + key_decode(cur));
@@ -1071,11 +1021,11 @@ template <class C>
 __global__ __launch_bounds__(256) void k_br_rowcol(const SegDesc* __restrict__ segs,
                                                    const uint64_t* __restrict__ seg_wave0,
                                                    const uint64_t* __restrict__ seg_rc0,
-                                                   uint32_t nseg, uint64_t waves, uint32_t per,
-                                                   const uint4* __restrict__ buckets,
+                                                   uint32_t nseg, uint64_t wave_lo, uint64_t wave_hi,
+                                                   uint32_t per, const uint4* __restrict__ buckets,
                                                    uint4* __restrict__ rc) {
-    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= waves) return;
+    const uint64_t wave = wave_lo + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= wave_hi) return;
     const uint32_t lane = __lane_id();
     const uint32_t s = upper_index(seg_wave0, nseg, wave);
     const SegDesc sg = segs[s];
@@ -1109,11 +1059,11 @@ __device__ __forceinline__ uint32_t with_bit(uint32_t i, uint32_t b) {
 template <class C>
 __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ segs,
                                                  const uint64_t* __restrict__ seg_rc0,
-                                                 uint32_t nseg, const uint4* __restrict__ rc,
+                                                 uint32_t seg_lo, uint32_t seg_hi, const uint4* __restrict__ rc,
                                                  uint4* __restrict__ bits, int host_form) {
-    const uint64_t item = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t item = (uint64_t)seg_lo * 16 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t s = (uint32_t)(item >> 4), j = (uint32_t)(item & 15u);
-    if (s >= nseg) return;
+    if (s >= seg_hi) return;
     const uint32_t lane = __lane_id();
     const SegDesc sg = segs[s];
     uint32_t g, h;
@@ -1183,8 +1133,7 @@ __global__ __launch_bounds__(64) void k_tail_combine(const MsmDesc* __restrict__
     typename C::Pt acc = C::load(wins, m.seg0 + m.nwin - 1);
 #pragma unroll 1
     for (int w = (int)m.nwin - 2; w >= 0; w--) {
-#pragma unroll 1
-        for (uint32_t i = 0; i < m.c; i++) acc = C::dbl(acc);
+        acc = C::dbl_k(acc, m.c);  // 2^c acc, in Jacobian coordinates
         C::add(acc, C::load(wins, m.seg0 + w));
     }
     C::store_result(out, j, acc);
@@ -1589,21 +1538,21 @@ void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile
                            nmsm, segs, (const uint4*)scalars, idm, (uint2*)digits, hist, err);
 }
 
-void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
-                      uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
-                      const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
-                      uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
+void launch_sort_pass(const SegDesc* segs, uint32_t seg_lo, uint32_t nseg, const uint32_t* tile_seg,
+                      uint32_t tile_lo, uint32_t ntiles, const uint32_t* chunk_seg, uint32_t chunk_lo,
+                      uint32_t nchunks, const void* in_keys, const uint32_t* in_vals, int pass,
+                      uint32_t* hist, uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
                       uint32_t* out_vals, hipStream_t st, bool count) {
     if (!ntiles) return;
     if (count)  // else the tile histograms are already in hist
-        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg, tile_lo,
                            in_keys, pass, (const uint32_t*)seg_valid, hist);
-    hipLaunchKernelGGL(k_scan_up, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg,
+    hipLaunchKernelGGL(k_scan_up, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg, chunk_lo,
                        (const uint32_t*)hist, part);
-    hipLaunchKernelGGL(k_scan_spine, dim3(nseg), dim3(256), 0, st, segs, part);
-    hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg, hist,
+    hipLaunchKernelGGL(k_scan_spine, dim3(nseg), dim3(256), 0, st, segs, seg_lo, part);
+    hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, segs, chunk_seg, chunk_lo, hist,
                        (const uint32_t*)part, pass, seg_valid);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg,
+    hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, segs, tile_seg, tile_lo,
                        in_keys, in_vals, pass, (const uint32_t*)hist, (const uint32_t*)seg_valid,
                        out_keys, out_vals);
 }
@@ -1631,13 +1580,13 @@ uint64_t accumulate_capacity(int device, CurveId cv) {
 
 void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
-                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
-                       void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st) {
-    if (!lanes) return;
+                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lane_lo,
+                       uint64_t lane_hi, void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st) {
+    if (lane_hi <= lane_lo) return;
 #define ZK_ACC(C)                                                                                  \
-    hipLaunchKernelGGL(k_accumulate<C>, dim3(blocks_for(lanes, 256)), dim3(256), 0, st, segs,      \
-                       seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table, T, lanes, \
-                       (uint4*)buckets, rec_keys, (uint4*)rec_pts)
+    hipLaunchKernelGGL(k_accumulate<C>, dim3(blocks_for(lane_hi - lane_lo, 256)), dim3(256), 0, st, \
+                       segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table, T,   \
+                       lane_lo, lane_hi, (uint4*)buckets, rec_keys, (uint4*)rec_pts)
     if (cv == CurveId::Grumpkin) ZK_ACC(G1Curve<Bn254Fr>);
     else if (cv == CurveId::Bn254G2) ZK_ACC(G2Curve);
     else ZK_ACC(G1Curve<Bn254Fp>);
@@ -1675,28 +1624,31 @@ void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64
 }
 
 template <class C>
-static void bucket_reduce_t(const SegDesc* segs, const uint64_t* seg_wave0,
-                            const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                            const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st) {
-    hipLaunchKernelGGL(k_br_rowcol<C>, dim3(blocks_for(rc_waves, 4)), dim3(256), 0, st, segs,
-                       seg_wave0, seg_rc0, nseg, rc_waves, per, (const uint4*)buckets, (uint4*)rc);
-    const uint64_t bit_items = (uint64_t)nseg * 16;
+static void bucket_reduce_t(const SegDesc* segs, const uint64_t* seg_wave0, const uint64_t* seg_rc0,
+                            uint32_t nseg, uint32_t seg_lo, uint32_t seg_hi, uint64_t wave_lo,
+                            uint64_t wave_hi, uint32_t per, const void* buckets, void* rc, void* bits,
+                            bool host_form, hipStream_t st) {
+    hipLaunchKernelGGL(k_br_rowcol<C>, dim3(blocks_for(wave_hi - wave_lo, 4)), dim3(256), 0, st, segs,
+                       seg_wave0, seg_rc0, nseg, wave_lo, wave_hi, per, (const uint4*)buckets, (uint4*)rc);
+    const uint64_t bit_items = (uint64_t)(seg_hi - seg_lo) * 16;
     hipLaunchKernelGGL(k_br_bits<C>, dim3(blocks_for(bit_items, 4)), dim3(256), 0, st, segs,
-                       seg_rc0, nseg, (const uint4*)rc, (uint4*)bits, host_form ? 1 : 0);
+                       seg_rc0, seg_lo, seg_hi, (const uint4*)rc, (uint4*)bits, host_form ? 1 : 0);
 }
 
 void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
-                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                          const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st) {
-    if (!rc_waves) return;
+                          const uint64_t* seg_rc0, uint32_t nseg, uint32_t seg_lo, uint32_t seg_hi,
+                          uint64_t wave_lo, uint64_t wave_hi, uint32_t per, const void* buckets, void* rc,
+                          void* bits, bool host_form, hipStream_t st) {
+    if (wave_hi <= wave_lo || seg_hi <= seg_lo) return;
     if (cv == CurveId::Grumpkin)
-        bucket_reduce_t<G1Curve<Bn254Fr>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits,
-                                          host_form, st);
+        bucket_reduce_t<G1Curve<Bn254Fr>>(segs, seg_wave0, seg_rc0, nseg, seg_lo, seg_hi, wave_lo, wave_hi, per,
+                                          buckets, rc, bits, host_form, st);
     else if (cv == CurveId::Bn254G2)
-        bucket_reduce_t<G2Curve>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits, true, st);
+        bucket_reduce_t<G2Curve>(segs, seg_wave0, seg_rc0, nseg, seg_lo, seg_hi, wave_lo, wave_hi, per, buckets,
+                                 rc, bits, true, st);
     else
-        bucket_reduce_t<G1Curve<Bn254Fp>>(segs, seg_wave0, seg_rc0, nseg, rc_waves, per, buckets, rc, bits,
-                                          host_form, st);
+        bucket_reduce_t<G1Curve<Bn254Fp>>(segs, seg_wave0, seg_rc0, nseg, seg_lo, seg_hi, wave_lo, wave_hi, per,
+                                          buckets, rc, bits, host_form, st);
 }
 
 void launch_tail(CurveId cv, const SegDesc* segs, const MsmDesc* msms, uint32_t nseg, uint32_t nmsm,

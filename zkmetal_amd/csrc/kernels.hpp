@@ -24,20 +24,23 @@ void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile
                       uint32_t dtiles, const SegDesc* segs, const void* scalars,
                       const void* table, uint64_t table_n, void* digits, uint32_t* hist,
                       uint32_t* err, hipStream_t st);
-void launch_sort_pass(const SegDesc* segs, uint32_t nseg, const uint32_t* tile_seg,
-                      uint32_t ntiles, const uint32_t* chunk_seg, uint32_t nchunks,
-                      const void* in_keys, const uint32_t* in_vals, int pass, uint32_t* hist,
-                      uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
+// one radix pass over segments [seg_lo, seg_lo + nseg): their tiles
+// [tile_lo, tile_lo + ntiles) and scan chunks [chunk_lo, chunk_lo + nchunks)
+void launch_sort_pass(const SegDesc* segs, uint32_t seg_lo, uint32_t nseg, const uint32_t* tile_seg,
+                      uint32_t tile_lo, uint32_t ntiles, const uint32_t* chunk_seg, uint32_t chunk_lo,
+                      uint32_t nchunks, const void* in_keys, const uint32_t* in_vals, int pass,
+                      uint32_t* hist, uint32_t* part, uint32_t* seg_valid, uint16_t* out_keys,
                       uint32_t* out_vals, hipStream_t st, bool count = true);
 void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 // lanes of k_accumulate resident at once on the device (CUs x workgroups per
 // CU at its register use x 256): a lane count just above it leaves a second,
 // nearly empty round of waves
 uint64_t accumulate_capacity(int device, CurveId cv = CurveId::Bn254);
+// lanes [lane_lo, lane_hi) of the call's lane space
 void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
                        const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
-                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lanes,
-                       void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
+                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lane_lo,
+                       uint64_t lane_hi, void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
 // counts[0] / counts[1]: long / mid run counts (zeroed by the caller);
 // lists: 2 * list_cap words (long list, then mid list)
 void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64_t nrec, uint32_t limit,
@@ -46,9 +49,11 @@ void launch_merge_runs(CurveId cv, const uint32_t* keys, const void* pts, uint64
 // bits: 16 points per segment (U and the bit-plane sums), in the host form
 // (Xyzz, 2^256 Montgomery, 128 B) for the host tail or in the stored form
 // (144 B) for launch_tail
+// segments [seg_lo, seg_hi) and their row/column waves [wave_lo, wave_hi)
 void launch_bucket_reduce(CurveId cv, const SegDesc* segs, const uint64_t* seg_wave0,
-                          const uint64_t* seg_rc0, uint32_t nseg, uint64_t rc_waves, uint32_t per,
-                          const void* buckets, void* rc, void* bits, bool host_form, hipStream_t st);
+                          const uint64_t* seg_rc0, uint32_t nseg, uint32_t seg_lo, uint32_t seg_hi,
+                          uint64_t wave_lo, uint64_t wave_hi, uint32_t per, const void* buckets, void* rc,
+                          void* bits, bool host_form, hipStream_t st);
 // window combine and normalisation on the device (BN254 G1 and Grumpkin):
 // bits in the stored form, wins: nseg stored points (scratch), out: nmsm
 // results of 96 B (normalised Jacobian X | Y | Z, 2^256 Montgomery form)
