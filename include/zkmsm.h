@@ -221,12 +221,36 @@ ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32
  * shape captures again.  zkmsm_graph_captures counts the captures. */
 ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
 
-/* A large call runs as a pipeline over parts of its (MSM, window) segments:
- * the bucket sort of part p + 1 on one stream while part p accumulates on
- * another, and part p's bucket reduction on a third.  parts = 0 (default)
- * picks 1, 2 or 4 parts by size; 1..8 forces that many (at most one per
- * segment).  Results are the same bytes for any value. */
+/* A large call runs as a pipeline over parts of its bucket segments: the
+ * bucket sort of part p + 1 on a second stream while part p accumulates,
+ * then one bucket reduction over all parts.  parts = 0 (default) picks 1, 2
+ * or 3 parts by size; 1..8 forces that many (at most one per segment).
+ * Results are the same bytes for any value. */
 ZkMetalStatus zkmsm_set_pipeline_parts(ZkMetalMSMEngine engine, uint32_t parts);
+
+/* Window-shifted resident tables.  mode 1 (auto, the default) or 2 (all):
+ * an SRS set afterwards (zkmsm_srs_upload, zkmsm_srs_generate,
+ * zkmsm_srs_load_ptau) also gets, for every 16-bit window w = 1..7, the
+ * copies 2^(16 w) P_i and their endomorphism images, built on the device
+ * (eight times the table's memory: 16 GiB for 2^23 bases).  An MSM over the
+ * resident table can then accumulate all its windows into one set of 2^15
+ * buckets: the bucket reduction shrinks eightfold and the window combine
+ * needs no doublings, at the price of gathers spread over the larger table.
+ * Auto takes it, in calls of several MSMs, for those whose cost estimate
+ * favours it (the reduction saved against the padding added), for a single
+ * MSM of at most 2^17 points, and for MSMs of >= 4096 points in calls whose
+ * window combine runs on the device; all takes it for every MSM (measured
+ * on MI355X: the 64-MSM batch 20.8 -> 18.0 ms, 32 witness-like MSMs of 2^18
+ * points 13.1 -> 8.0 ms, one 2^16 MSM 0.61 -> 0.42 ms, one 2^22 MSM 6.05 ->
+ * 6.23 ms, which auto runs plain).  mode 0 (off):
+ * plain plans, and the next SRS is plain.  When the copies do not fit
+ * (device memory, zkmsm_set_workspace_limit) the plain table is kept.
+ * Results are the same bytes in every mode.  No counterpart in the
+ * reference (its MSM reduces every window separately,
+ * Sources/zkMetal/MSM/MSMEngine.swift:899-904). */
+ZkMetalStatus zkmsm_set_window_shift(ZkMetalMSMEngine engine, int32_t mode);
+/* 16 when the resident table holds window-shifted copies, else 0 */
+uint32_t zkmsm_srs_window_shift(ZkMetalMSMEngine engine);
 
 /* Where a call's window combine (Horner over the windows) and affine
  * normalisation run: 0 auto (the device in graph mode and for calls of 16 or

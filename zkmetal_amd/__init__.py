@@ -95,6 +95,8 @@ def lib():
         "zkmsm_test_glv": (i32, [vp, vp, vp, u32]),
         "zkmsm_set_workspace_limit": (i32, [vp, u64]),
         "zkmsm_set_tail_mode": (i32, [vp, i32]),
+        "zkmsm_set_window_shift": (i32, [vp, i32]),
+        "zkmsm_srs_window_shift": (u32, [vp]),
         "zkmsm_set_pipeline_parts": (i32, [vp, u32]),
         "zkmsm_workspace_bytes": (u64, [vp]),
         "zkmsm_probe_rates": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
@@ -541,6 +543,21 @@ class MetalMSM:
         """Where the window combine and normalisation run: TAIL_AUTO (device in
         graph mode and for calls of >= 16 MSMs), TAIL_HOST or TAIL_DEVICE."""
         _check(lib().zkmsm_set_tail_mode(self.handle, mode), "set_tail_mode")
+
+    SHIFT_OFF, SHIFT_AUTO, SHIFT_ALL = 0, 1, 2
+
+    def set_window_shift(self, mode):
+        """Window-shifted resident tables: SHIFT_AUTO (default) or SHIFT_ALL
+        make an SRS set afterwards also hold 2^(16 w) P_i for every window w,
+        and MSMs on it share one bucket set across windows (auto: when the
+        cost estimate favours it; all: always); SHIFT_OFF runs plain and
+        makes the next SRS plain.  True / False mean auto / off.  Results do
+        not change."""
+        _check(lib().zkmsm_set_window_shift(self.handle, int(mode)), "set_window_shift")
+
+    def srs_window_shift(self):
+        """16 when the resident table holds window-shifted copies, else 0."""
+        return int(lib().zkmsm_srs_window_shift(self.handle))
 
     def set_pipeline_parts(self, parts):
         """Parts a large call is split into for the sort / accumulation

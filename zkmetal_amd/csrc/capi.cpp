@@ -529,6 +529,20 @@ ZkMetalStatus zkmsm_set_pipeline_parts(ZkMetalMSMEngine engine, uint32_t parts) 
     return ZKMETAL_SUCCESS;
 }
 
+ZkMetalStatus zkmsm_set_window_shift(ZkMetalMSMEngine engine, int32_t mode) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->set_window_shift(mode);
+}
+
+uint32_t zkmsm_srs_window_shift(ZkMetalMSMEngine engine) {
+    Engine* e = as_engine(engine);
+    if (!e) return 0;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->srs_window_shift();
+}
+
 ZkMetalStatus zkmsm_set_tail_mode(ZkMetalMSMEngine engine, int32_t mode) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;

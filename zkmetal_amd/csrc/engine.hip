@@ -124,11 +124,45 @@ uint32_t default_parts(uint64_t entries) {
     return entries >= (1ull << 27) ? 3u : entries >= (1ull << 24) ? 2u : 1u;
 }
 
+// Window-shifted or plain, by an estimate in mixed-addition units: every
+// digit entry costs one, every bucket of the reduction kReduceCost (the
+// row/column and bit-plane kernels spend about 3.3 times a bucket
+// accumulation's time per bucket in large calls).  Shifted: ceil(128 / c_s)
+// windows of 2n entries padded to whole sort tiles into one set of
+// 2^(c_s - 1) buckets; plain: ceil(128 / c) windows of 2n entries, each with
+// 2^(c-1) buckets.
+bool shifted_pays(uint64_t n, uint32_t c, uint32_t c_s) {
+    constexpr double kReduceCost = 3.3;
+    const double padded = (double)((2 * n + kSortTile - 1) / kSortTile * kSortTile);
+    const double plain = (double)((128 + c - 1) / c) * (2.0 * (double)n + kReduceCost * (double)(1u << (c - 1)));
+    const double shifted = (double)((128 + c_s - 1) / c_s) * padded + kReduceCost * (double)(1u << (c_s - 1));
+    return shifted < plain;
+}
+
+// A call of one MSM has a latency-bound reduction (about 0.16 ms from 2^16
+// to 2^22 points whatever its bucket count), and its shifted buckets are
+// deep enough to span several accumulation lanes, which costs merge work:
+// measured on MI355X, one shifted MSM takes 0.42 / 0.74 / 1.81 / 6.23 ms
+// against 0.61 / 0.65 / 1.71 / 6.05 ms plain at 2^16 / 2^18 / 2^20 / 2^22.
+constexpr uint64_t kShiftSingleMax = 1u << 17;
+
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
-              Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv, uint32_t nparts) {
+              Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv, uint32_t nparts,
+              uint64_t shift_stride, uint32_t shift_c, uint32_t shift_floor) {
     plan_reset(pl);
     pl.curve = cv;
     const uint32_t eps = entries_per_point(cv);
+    const bool can_shift = cv == CurveId::Bn254 && shift_stride && shift_c >= 2 && shift_c <= 16 &&
+                           shift_stride % kSortTile == 0;
+    // a call of fewer MSMs than pipeline parts splits each window-shifted MSM's
+    // windows over several segments, so that the parts have segments to take
+    uint32_t spm = 1;
+    if (can_shift && nmsm) {
+        uint64_t est = 0;
+        for (uint32_t j = 0; j < nmsm; j++) est += (uint64_t)eps * ns[j] * ((128 + shift_c - 1) / shift_c);
+        const uint32_t P = nparts ? nparts : default_parts(est);
+        if (nmsm < P) spm = (P + nmsm - 1) / nmsm;
+    }
     pl.msms.resize(nmsm);
     pl.msm_scalar0.resize(nmsm + 1);
     pl.msm_dtile0.resize(nmsm + 1);
@@ -136,6 +170,9 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         if (ns[j] == 0) return kInvalidInput;
         uint32_t c = (cs && cs[j]) ? cs[j] : default_window_bits(ns[j], cv);
         if (c < 2 || c > 16) return kInvalidInput;
+        const bool sh = can_shift && ((shift_floor && ns[j] >= shift_floor) ||
+                                      (shifted_pays(ns[j], c, shift_c) && (nmsm > 1 || ns[j] <= kShiftSingleMax)));
+        if (sh) c = shift_c;
         MsmDesc& m = pl.msms[j];
         m.scalar0 = pl.scalars;
         m.table0 = table0[j];
@@ -145,20 +182,26 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
         if (m.nwin > kMaxWindows) return kInvalidInput;  // Grumpkin: c >= 4
         m.seg0 = (uint32_t)pl.segs.size();
         m.form = form;
+        m.wps = sh ? (m.nwin + spm - 1) / spm : 1;
+        m.nseg = (m.nwin + m.wps - 1) / m.wps;
+        m.cstep = sh ? 0 : c;
+        const uint32_t ntw = (eps * ns[j] + kSortTile - 1) / kSortTile;  // sort tiles per window
         pl.msm_scalar0[j] = pl.scalars;
         pl.msm_dtile0[j] = pl.dtiles;
-        pl.dtiles += (eps * ns[j] + kSortTile - 1) / kSortTile;
+        pl.dtiles += ntw;
         pl.scalars += ns[j];
-        for (uint32_t w = 0; w < m.nwin; w++) {
+        for (uint32_t g = 0; g < m.nseg; g++) {
+            const uint32_t w0 = g * m.wps, nv = std::min(m.wps, m.nwin - w0);
             SegDesc s{};
             s.ent0 = pl.entries;
             s.bkt0 = pl.buckets;
-            s.len = eps * ns[j];
+            s.len = sh ? nv * ntw * (uint32_t)kSortTile : eps * ns[j];
             s.nbk = 1u << (c - 1);
             s.tile0 = pl.tiles;
-            s.ntiles = (s.len + kSortTile - 1) / kSortTile;
+            s.ntiles = sh ? nv * ntw : ntw;
             s.msm = j;
-            s.win = w;
+            s.wadj = sh ? (uint32_t)(shift_stride - (uint64_t)ntw * kSortTile) : 0u;
+            s.wbase = sh ? (uint32_t)(w0 * shift_stride) : 0u;
             s.chunk0 = pl.chunks;
             s.nchunks = (256u * s.ntiles + kScanChunk - 1) / kScanChunk;
             pl.chunks += s.nchunks;
@@ -287,7 +330,9 @@ int32_t Engine::create(Engine** out) {
                 ok = ok && hipEventCreate(ev) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&e->dep_fork_, hipEventDisableTiming) == hipSuccess;
         for (uint32_t p = 0; p < kMaxParts; p++)
-            ok = ok && hipEventCreateWithFlags(&e->dep_sort_[p], hipEventDisableTiming) == hipSuccess;
+            ok = ok && hipEventCreateWithFlags(&e->dep_sort_[p], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&e->dep_acc_[p], hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&e->dep_join_, hipEventDisableTiming) == hipSuccess;
         if (!ok) {
             delete e;
             return kGpuError;
@@ -331,9 +376,9 @@ Engine::~Engine() {
     if (stream_sort_) (void)hipStreamSynchronize(stream_sort_);
     for (uint32_t p = 0; p < kMaxParts; p++)
         for (hipEvent_t ev : {ev_s0_[p], ev_s1_[p], ev_a0_[p], ev_a1_[p], ev_m1_[p], ev_r0_[p], ev_r1_[p],
-                              dep_sort_[p]})
+                              dep_sort_[p], dep_acc_[p]})
             if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {ev_dec_, dep_fork_})
+    for (hipEvent_t ev : {ev_dec_, dep_fork_, dep_join_})
         if (ev) (void)hipEventDestroy(ev);
     if (stream_sort_) (void)hipStreamDestroy(stream_sort_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -457,16 +502,21 @@ static void host_tail(const Plan& pl, const uint8_t* hw, uint8_t* out) {
     auto tail = [&](uint32_t jb, uint32_t je) {
         for (uint32_t j = jb; j < je; j++) {
             const MsmDesc& m = pl.msms[j];
-            const uint32_t c = m.c, cb = c - 1, g = cb / 2;
+            const int cb = (int)m.c - 1;  // planes per segment (bucket index bits)
             typename HC::Pt acc = HC::identity();
-            for (int b = (int)(c * (m.nwin - 1) + cb - 1); b >= 0; b--) {
+            // Horner over the bits of sum_w 2^(cstep w) S_w: at bit b, segment
+            // w adds its plane t = b - cstep w (and U_w at t = 0); plain
+            // windows (cstep = c) meet one segment per bit, window-shifted
+            // segments (cstep = 0) all of them
+            for (int b = (int)(m.cstep * (m.nseg - 1)) + cb - 1; b >= 0; b--) {
                 if (!HC::is_id(acc)) acc = HC::xdbl(acc);
-                const uint32_t w = (uint32_t)b / c, t = (uint32_t)b % c;
-                if (t >= cb) continue;  // bit c-1 of a window carries no plane
-                const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * HC::kIn;
-                const uint32_t slot = t < g ? 1 + t : 1 + g + (t - g);
-                acc = HC::xadd(acc, HC::load(bs + (uint64_t)slot * HC::kIn));
-                if (t == 0) acc = HC::xadd(acc, HC::load(bs));  // U_w
+                for (uint32_t w = 0; w < m.nseg; w++) {
+                    const int t = b - (int)(m.cstep * w);
+                    if (t < 0 || t >= cb) continue;
+                    const uint8_t* bs = hw + (uint64_t)(m.seg0 + w) * 16 * HC::kIn;
+                    acc = HC::xadd(acc, HC::load(bs + (uint64_t)(1 + t) * HC::kIn));  // Q_{w,t}
+                    if (t == 0) acc = HC::xadd(acc, HC::load(bs));                 // U_w
+                }
             }
             res[j] = acc;
         }
@@ -521,9 +571,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
                     uint8_t* out96) {
     const uint32_t nmsm = (uint32_t)pl.msms.size();
     // window combine on the device (graph mode, large batches) or the host
-    const bool dev_tail = pl.curve != CurveId::Bn254G2 &&
-                          (tail_mode_ == kTailDevice ||
-                           (tail_mode_ == kTailAuto && (graph_mode_ || nmsm >= kDeviceTailMinMsms)));
+    const bool dev_tail = device_tail(nmsm, pl.curve);
     const WsSizes z = ws_sizes(pl, dev_tail);
     ZK_ALLOC(desc_, z.desc);
     ZK_ALLOC(hdesc_, z.desc);
@@ -605,16 +653,26 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     const uint32_t P = pl.nparts;
     // merge lists of part p: a listed run has more than kRunLimit records, so
     // each list holds at most nrec / (kRunLimit + 1) runs
+    // each list holds at most nrec / (kRunLimit + 1) runs; the parts' lists
+    // sit side by side in recBk (2 * lanes + 2 words hold them all), as the
+    // merges of different parts may overlap
     uint32_t list_cap[kMaxParts];
-    for (uint32_t p = 0; p < P; p++)
+    uint64_t list_off[kMaxParts];
+    uint64_t off = 0;
+    for (uint32_t p = 0; p < P; p++) {
         list_cap[p] = (uint32_t)std::min<uint64_t>(
             2 * (pl.parts[p].lane_hi - pl.parts[p].lane_lo) / (kRunLimit + 1) + 1, 1u << 29);
+        list_off[p] = off;
+        off += 2 * list_cap[p];
+    }
     uint8_t* hw = ws.hw;
     // The device part of the call.  Stream A (the engine stream): descriptors,
-    // decomposition, then per part the accumulation and the merge, then the
-    // bucket reduction and the device tail; stream S: the two sort passes of
-    // each part, so that part p + 1 sorts while part p accumulates (the
-    // accumulation leaves registers and LDS for the sort's workgroups).
+    // decomposition, then per part the accumulation (and the last part's
+    // merge), then the bucket reduction and the device tail; stream S: the
+    // two sort passes of each part, so that part p + 1 sorts while part p
+    // accumulates (the accumulation leaves registers and LDS for the sort's
+    // workgroups), then the merges of all parts but the last, each beside the
+    // next part's accumulation.
     // Timing events are recorded only outside graph capture; the dependency
     // events are part of a captured graph.
     auto enqueue = [&](bool events) -> int32_t {
@@ -657,11 +715,25 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                               pt.lane_hi, ws.buckets, (uint32_t*)ws.recAk, ws.recAp, st);
             ZK_TRY(rec(ev_a1_[p], st));
             // bucket partials that crossed lane boundaries: short runs by their
-            // first lane, longer runs (heavy buckets) by lane groups
+            // first lane, longer runs (heavy buckets) by lane groups.  Part p's
+            // merge touches only its own segments' buckets and records, so
+            // before the last part it runs on stream S beside the next part's
+            // accumulation (after the sorts queued there)
+            hipStream_t ms = st;
+            if (p + 1 < P) {
+                ZK_TRY(hipEventRecord(dep_acc_[p], st));
+                ZK_TRY(hipStreamWaitEvent(S, dep_acc_[p], 0));
+                ms = S;
+            }
             launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk + 2 * pt.lane_lo,
                               (const uint8_t*)ws.recAp + 2 * pt.lane_lo * ptb, 2 * (pt.lane_hi - pt.lane_lo),
-                              kRunLimit, ws.buckets, d_err + 1 + 2 * p, (uint32_t*)ws.recBk, list_cap[p], st);
-            ZK_TRY(rec(ev_m1_[p], st));
+                              kRunLimit, ws.buckets, d_err + 1 + 2 * p, (uint32_t*)ws.recBk + list_off[p],
+                              list_cap[p], ms);
+            ZK_TRY(rec(ev_m1_[p], ms));
+        }
+        if (P > 1) {  // join: the reduction reads every part's merged buckets
+            ZK_TRY(hipEventRecord(dep_join_, S));
+            ZK_TRY(hipStreamWaitEvent(st, dep_join_, 0));
         }
         // the bucket reduction of all parts at once, after the last merge: as
         // a kernel beside the next part's accumulation it finds no free slots
@@ -1002,6 +1074,43 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     return kOk;
 }
 
+int32_t Engine::srs_alloc(uint64_t n, uint64_t* stride) {
+    *stride = 0;
+    srs_shift_ = 0;
+    const uint64_t base = table_bytes(n);
+    if (shift_mode_ != kShiftOff) {
+        // window copies kSortTile-aligned (a padded window's positions stay
+        // inside its copy), all table entries below 2^31 (32-bit sort values)
+        const uint64_t s = ((base + kTableEntryBytes - 1) / kTableEntryBytes + kSortTile - 1) / kSortTile * kSortTile;
+        const uint64_t nw = (128 + kShiftC - 1) / kShiftC;
+        if (s * nw < (1ull << 31) && srs_.ensure(s * nw * kTableEntryBytes) == hipSuccess) {
+            *stride = s;
+            return kOk;
+        }
+    }
+    ZK_ALLOC(srs_, base);
+    return kOk;
+}
+
+int32_t Engine::srs_finish(uint64_t n, uint64_t stride) {
+    if (stride) launch_shift_windows(srs_.p, n, stride, kShiftC, (128 + kShiftC - 1) / kShiftC, stream_);
+    ZK_TRY(hipStreamSynchronize(stream_));
+    ZK_TRY(hipGetLastError());
+    srs_shift_ = stride;
+    return kOk;
+}
+
+uint32_t Engine::shift_floor(uint32_t nmsm) const {
+    if (shift_mode_ == kShiftAll) return 1;
+    return device_tail(nmsm, CurveId::Bn254) ? kShiftDeviceTailFloor : 0;
+}
+
+uint64_t Engine::plan_shift(const void* table, uint64_t shift_stride) const {
+    if (c_override_ || shift_mode_ == kShiftOff) return 0;  // a forced window width runs plain
+    if (shift_stride == kOwnTable) return table == srs_.p ? srs_shift_ : 0;
+    return shift_stride;
+}
+
 int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     if (!points || n == 0) return kInvalidInput;
     if (servers_) return kInvalidInput;  // a server reads the current table
@@ -1009,12 +1118,14 @@ int32_t Engine::srs_upload(const uint8_t* points, uint32_t n) {
     srs_n_ = 0;
     srs_host_.clear();
     ZK_ALLOC(pts_in_, (uint64_t)n * 64);
-    ZK_ALLOC(srs_, table_bytes(n));
-    const int32_t rc = stage_upload(pts_in_.p, points, (uint64_t)n * 64);
+    uint64_t stride = 0;
+    int32_t rc = srs_alloc(n, &stride);
+    if (rc != kOk) return rc;
+    rc = stage_upload(pts_in_.p, points, (uint64_t)n * 64);
     if (rc != kOk) return rc;
     launch_build_table(CurveId::Bn254, pts_in_.p, srs_.p, n, stream_);
-    ZK_TRY(hipStreamSynchronize(stream_));
-    ZK_TRY(hipGetLastError());
+    rc = srs_finish(n, stride);
+    if (rc != kOk) return rc;
     srs_n_ = n;
     srs_host_.assign(points, points + (uint64_t)n * 64);  // the caller's own bytes
     memcpy(srs_first_, points, 64);
@@ -1061,7 +1172,11 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
     if (n == 0 || n >= (1ull << 31)) return kInvalidInput;
     srs_n_ = 0;
     ZK_ALLOC(pts_in_, n * 64);
-    ZK_ALLOC(srs_, table_bytes(n));
+    uint64_t stride = 0;
+    {
+        const int32_t rc = srs_alloc(n, &stride);
+        if (rc != kOk) return rc;
+    }
     ZK_ALLOC(err_, 16);
     {
         const int32_t rc = stage_upload(pts_in_.p, g1, n * 64);
@@ -1074,6 +1189,10 @@ int32_t Engine::srs_load_ptau(const uint8_t* data, uint64_t len, uint32_t max_po
     ZK_TRY(hipStreamSynchronize(stream_));
     ZK_TRY(hipGetLastError());
     if (bad) return kInvalidInput;  // not canonical or not on the curve
+    {
+        const int32_t rc = srs_finish(n, stride);
+        if (rc != kOk) return rc;
+    }
     srs_n_ = (uint32_t)n;
     const int32_t rc = srs_set_first();
     if (rc != kOk) return rc;
@@ -1217,11 +1336,13 @@ int32_t Engine::srs_generate(uint64_t seed, uint32_t n) {
     std::vector<uint8_t> ks((uint64_t)n * 32);
     zkh::gen_scalars(seed, n, 0, ks.data());
     ZK_ALLOC(tmp_, (uint64_t)n * 32);
-    ZK_ALLOC(srs_, table_bytes(n));
+    uint64_t stride = 0;
+    rc = srs_alloc(n, &stride);
+    if (rc != kOk) return rc;
     ZK_TRY(hipMemcpyAsync(tmp_.p, ks.data(), ks.size(), hipMemcpyHostToDevice, stream_));
     launch_gen_bases(tmp_.p, gtab_.p, n, srs_.p, stream_);
-    ZK_TRY(hipStreamSynchronize(stream_));
-    ZK_TRY(hipGetLastError());
+    rc = srs_finish(n, stride);
+    if (rc != kOk) return rc;
     srs_n_ = n;
     return srs_set_first();
 }
@@ -1273,7 +1394,7 @@ int32_t Engine::msm_table(const void* table, uint32_t table_n, const void* scala
 int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
                                 const uint8_t* const* host_parts, const void* d_scalars,
                                 const uint32_t* counts, uint32_t nmsm, uint8_t* out96,
-                                uint32_t form) {
+                                uint32_t form, uint64_t shift_stride) {
     if (!table || !counts || !out96 || nmsm == 0) return kInvalidInput;
     if (!host_parts && !d_scalars) return kInvalidInput;
     std::vector<uint32_t> ns, idx;
@@ -1295,7 +1416,8 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     if (total >= (1ull << 30)) return kInvalidInput;
     Plan pl;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_,
-                       CurveId::Bn254, parts_req_);
+                       CurveId::Bn254, parts_req_, plan_shift(table, shift_stride), kShiftC,
+                       shift_floor((uint32_t)ns.size()));
     if (rc != kOk) return rc;
     const void* d_sc = d_scalars;
     if (host_parts) {
@@ -1431,7 +1553,8 @@ int32_t Engine::msm_resident_noalloc(const void* d_scalars, const uint32_t* coun
     // at most 2^15 accumulation lanes: a quarter of the device's resident
     // lanes, ample for <= 2^16 points, and it bounds the merge records (two
     // per lane), the largest buffer of a small call
-    int rc = make_plan(ns, t0, k, cs, na_plan_, 0, std::min<uint64_t>(acc_cap_, 1u << 15));
+    int rc = make_plan(ns, t0, k, cs, na_plan_, 0, std::min<uint64_t>(acc_cap_, 1u << 15), CurveId::Bn254, 0,
+                       plan_shift(srs_.p, kOwnTable), kShiftC, shift_mode_ == kShiftAll ? 1u : 0u);
     if (rc != kOk) return rc;
     const Plan& pl = na_plan_;
     const WsSizes z = ws_sizes(pl, true);

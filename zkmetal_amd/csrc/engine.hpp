@@ -126,11 +126,17 @@ uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
 
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j;
 // nparts: how many parts to split the segments into (0: by size, see
-// default_parts; at most kMaxParts and the segment count)
+// default_parts; at most kMaxParts and the segment count); shift_stride > 0:
+// the table also holds window-shifted copies for window width shift_c, copy w
+// at entry w * shift_stride (a multiple of kSortTile), and MSMs run
+// window-shifted at width shift_c when shifted_pays or (shift_floor > 0) they
+// have at least shift_floor points
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
               const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0,
-              uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254, uint32_t nparts = 0);
+              uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254, uint32_t nparts = 0,
+              uint64_t shift_stride = 0, uint32_t shift_c = 0, uint32_t shift_floor = 0);
 uint32_t default_parts(uint64_t entries);
+bool shifted_pays(uint64_t n, uint32_t c, uint32_t c_s);
 
 class Engine {
   public:
@@ -184,6 +190,24 @@ class Engine {
     void set_srs_detect(bool on) { srs_detect_ = on; }
     uint64_t srs_detect_hits() const { return srs_hits_; }
 
+    // window-shifted resident tables: with mode kShiftAuto (default) or
+    // kShiftAll an SRS set afterwards also gets the copies 2^(kShiftC w) P_i
+    // of every window w (memory: the table times ceil(128 / kShiftC)); MSMs
+    // on it share one bucket set across windows when shifted_pays (auto) or
+    // always (all).  kShiftOff: plain plans, and the next SRS is plain.
+    // Falls back to the plain table when the copies do not fit.  Results are
+    // the same bytes in every mode.
+    static constexpr uint32_t kShiftC = 16;
+    enum ShiftMode { kShiftOff = 0, kShiftAuto = 1, kShiftAll = 2 };
+    int32_t set_window_shift(int32_t mode) {
+        if (mode < kShiftOff || mode > kShiftAll) return kInvalidInput;
+        shift_mode_ = mode;
+        return kOk;
+    }
+    // kShiftC when the resident table holds the shifted copies, else 0
+    uint32_t srs_window_shift() const { return srs_shift_ ? kShiftC : 0; }
+    uint64_t srs_shift_stride() const { return srs_shift_; }
+
     // MSMs over prefixes of the resident table; scalars on host or device
     int32_t msm_resident(const void* scalars, bool scalars_on_device, const uint32_t* counts,
                          uint32_t nmsm, uint8_t* out96);
@@ -193,10 +217,14 @@ class Engine {
                       uint8_t* out96);
     // scalars given per MSM: host_parts[j] (n_j x 32 B, host) or, when
     // host_parts is null, one device buffer holding all MSMs' scalars
+    // shift_stride: the table's window-shifted copies (srs_shift_stride() of
+    // the engine that owns it; 0 = none; kOwnTable = this engine's resident
+    // table's when table is it)
+    static constexpr uint64_t kOwnTable = ~0ull;
     int32_t msm_table_parts(const void* table, uint32_t table_n,
                             const uint8_t* const* host_parts, const void* d_scalars,
                             const uint32_t* counts, uint32_t nmsm, uint8_t* out96,
-                            uint32_t form = 0);
+                            uint32_t form = 0, uint64_t shift_stride = kOwnTable);
     // KZG commitments over the resident table: coefficients are Fr elements in
     // Montgomery form (converted to standard form on the device)
     int32_t kzg_commit(const void* coeffs, bool on_device, const uint32_t* counts, uint32_t npoly,
@@ -211,6 +239,7 @@ class Engine {
         memcpy(ctab_, o.ctab_, sizeof(ctab_));
         memcpy(ctab_batch_, o.ctab_batch_, sizeof(ctab_batch_));
         c_override_ = o.c_override_;
+        shift_mode_ = o.shift_mode_;
     }
     // per-MSM window widths: the override, else the tuned tables (BN254 G1
     // only), else 0 = the cost model
@@ -235,6 +264,10 @@ class Engine {
     // else the host), kTailHost, kTailDevice.  BN254 G2 always uses the host.
     enum TailMode : int32_t { kTailAuto = 0, kTailHost = 1, kTailDevice = 2 };
     static constexpr uint32_t kDeviceTailMinMsms = 16;
+    bool device_tail(uint32_t nmsm, CurveId cv) const {
+        return cv != CurveId::Bn254G2 &&
+               (tail_mode_ == kTailDevice || (tail_mode_ == kTailAuto && (graph_mode_ || nmsm >= kDeviceTailMinMsms)));
+    }
     int32_t set_tail_mode(int32_t m) {
         if (m < kTailAuto || m > kTailDevice) return kInvalidInput;
         tail_mode_ = m;
@@ -311,6 +344,20 @@ class Engine {
     DevBuf pts_in_, ptable_, scalars_, srs_, gtab_, gtab_gk_, gtab_g2_, tmp_;
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
+    int32_t shift_mode_ = kShiftAuto;
+    uint64_t srs_shift_ = 0;  // entries between the resident table's window copies (0: none)
+    // allocate the resident table for n bases; *stride = the window copies'
+    // stride when they fit (built by srs_finish), else 0
+    int32_t srs_alloc(uint64_t n, uint64_t* stride);
+    // after the table's bases are written: the shifted copies, then wait
+    int32_t srs_finish(uint64_t n, uint64_t stride);
+    uint64_t plan_shift(const void* table, uint64_t shift_stride) const;
+    // make_plan's shift_floor for a call of nmsm MSMs: every MSM in kShiftAll
+    // mode; in a call whose window combine runs on the device, every MSM of
+    // at least kShiftDeviceTailFloor points (a plain MSM's serial doubling
+    // chain would set the combine's latency)
+    static constexpr uint32_t kShiftDeviceTailFloor = 4096;
+    uint32_t shift_floor(uint32_t nmsm) const;
     // host image of the resident bases (64 B each) for the prefix check:
     // srs_first_ is filled whenever an SRS is set, srs_host_ on the first
     // call whose first base matches it
@@ -343,6 +390,8 @@ class Engine {
                ev_a1_[kMaxParts] = {}, ev_m1_[kMaxParts] = {}, ev_r0_[kMaxParts] = {}, ev_r1_[kMaxParts] = {};
     hipEvent_t dep_fork_ = nullptr;
     hipEvent_t dep_sort_[kMaxParts] = {};
+    hipEvent_t dep_acc_[kMaxParts] = {};  // part p accumulated: its merge may start on the sort stream
+    hipEvent_t dep_join_ = nullptr;       // the sort stream's last merge, before the reduction
     uint32_t parts_req_ = 0;  // 0: default_parts
     // no-allocation path: this device's scratch array, a plan and host
     // staging whose capacity was reserved at creation

@@ -364,6 +364,42 @@ __global__ __launch_bounds__(256) void k_build_table(const uint4* __restrict__ p
     for (uint32_t w = threadIdx.x; w < words; w += 256) dst[w] = stage[(w / kPair) * kRow + w % kPair];
 }
 
+// Window-shifted copies of a BN254 G1 base table (the resident SRS): for
+// windows w = 1 .. nwin-1, entries w * stride + 2i and w * stride + 2i + 1
+// hold 2^(c w) P_i and phi(2^(c w) P_i) = (beta x, y), affine and canonical
+// like the table's own entries (window 0).  A digit of window w then adds
+// its base's shifted copy, so every window of an MSM can share one set of
+// buckets and the window sums need no doublings to combine.  One lane per
+// base: c Jacobian doublings per window (dbl-2009-l, exact for points of odd
+// order), one inversion back to affine; identity bases are skipped (they
+// carry no digits).
+__global__ __launch_bounds__(256) void k_shift_windows(uint4* __restrict__ table, uint64_t n,
+                                                       uint64_t stride, uint32_t c, uint32_t nwin) {
+    using A = Arith<Bn254Fp>;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t* idmask = reinterpret_cast<const uint64_t*>(table + n * 2 * (kTableEntryBytes / 16));
+    if ((idmask[i >> 6] >> (i & 63)) & 1) return;
+    QAff q = load_qtab(table, 2 * i, false);
+    Fq X = q.x, Y = q.y, Z = A::fq_one();  // < p each
+#pragma unroll 1
+    for (uint32_t w = 1; w < nwin; w++) {
+#pragma unroll 1
+        for (uint32_t k = 0; k < c; k++) A::qjac_dbl(X, Y, Z);  // X, Y, Z < 2p
+        const Fq zi = A::fq_inv(Z);                              // < 1.5p
+        const Fq zi2 = A::fq_sqr(zi);
+        q.x = A::template fq_csub<1>(A::fq_mul(X, zi2));             // X / Z^2, < p
+        q.y = A::template fq_csub<1>(A::fq_mul(Y, A::fq_mul(zi2, zi)));  // Y / Z^3, < p
+        A::store_qtab(table, w * stride + 2 * i, q);
+        QAff ph = q;
+        ph.x = A::template fq_csub<1>(A::fq_mul(A::fq_beta(), q.x));
+        A::store_qtab(table, w * stride + 2 * i + 1, ph);
+        X = q.x;
+        Y = q.y;
+        Z = A::fq_one();
+    }
+}
+
 // --------------------------------------------------- GLV + signed digits
 
 // Grumpkin's group order is the BN254 base-field prime p: scalars are reduced
@@ -413,13 +449,16 @@ __device__ __forceinline__ uint32_t window_bits256(const uint32_t k[8], uint32_t
 
 // Block b decomposes the kSortTile / EPS scalars of first-pass sort tile tin
 // of MSM j (msm_dtile0[j] <= b < msm_dtile0[j+1], tin = b - msm_dtile0[j]):
-// their entries are that tile of every window's segment, so the block also
+// their entries are tile tin of every window's entries, so the block also
 // counts the tile's 256-bin histogram of key byte 0 for each window (LDS
 // atomics) and writes it where the scan expects it; the first pass then
 // needs no histogram kernel.  EPS = 2: BN254 G1, GLV halves k1 (entry 2i,
 // base P_i) and k2 (entry 2i + 1, phi(P_i)); EPS = 1: one entry per scalar
 // over 255 bits, the scalar reduced mod r (kModR, BN254 G2) or mod p
-// (Grumpkin).
+// (Grumpkin).  Window w's entries: segment seg0 + w / wps, at local window
+// v = w % wps, i.e. tiles v * ntw + tin and positions v * ntw * kSortTile +
+// (entry) of the segment, ntw = the window's tiles; a window-shifted MSM
+// (wps > 1) pads each window to whole tiles with "no bucket" digits.
 constexpr int kDecThreads = 256;
 static_assert(kSortTile <= (1 << 15) && kSortTile / 2 / kDecThreads >= 1, "decomposition tiling");
 
@@ -445,17 +484,26 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
     const uint32_t c = m.c, half = 1u << (c - 1), full = 1u << c;
     bool carry_left = false;
     constexpr int kPer = kSortTile / EPS / kDecThreads;  // scalars per thread
+    const uint32_t ntw = (EPS * m.n + kSortTile - 1) / kSortTile;  // tiles per window
+    const uint64_t vstride = (uint64_t)ntw * kSortTile;             // positions per local window
     for (int r = 0; r < kPer; r++) {
         const uint64_t i = (uint64_t)tin * (kSortTile / EPS) + (uint64_t)r * kDecThreads + threadIdx.x;
-        if (i >= m.n) break;
-        const uint64_t g = m.scalar0 + i;
-        uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
-        uint32_t k[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-        if (m.form == 1) fr_from_mont(k);
-        const uint64_t pt = m.table0 / EPS + i;  // the base's point index in the table
-        if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
+        const bool live = i < m.n;
+        // past the MSM's scalars: done, unless its windows are padded to whole
+        // tiles (window-shifted), whose tails take "no bucket" digits
+        if (!live && m.wps == 1) break;
+        uint32_t k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (live) {
+            const uint64_t g = m.scalar0 + i;
+            uint4 s0 = scalars[2 * g], s1 = scalars[2 * g + 1];
+            k[0] = s0.x; k[1] = s0.y; k[2] = s0.z; k[3] = s0.w;
+            k[4] = s1.x; k[5] = s1.y; k[6] = s1.z; k[7] = s1.w;
+            if (m.form == 1) fr_from_mont(k);
+            const uint64_t pt = m.table0 / EPS + i;  // the base's point index in the table
+            if ((idmask[pt >> 6] >> (pt & 63)) & 1) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) k[q] = 0;  // identity base: no digits
+                for (int q = 0; q < 8; q++) k[q] = 0;  // identity base: no digits
+            }
         }
         if constexpr (EPS == 1) {
             if constexpr (kModR) reduce_mod_r(k);
@@ -474,6 +522,7 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
         }
         GlvSplit sp = glv_split(k);
         uint32_t carry1 = 0, carry2 = 0;
+        uint32_t v = 0, sg = m.seg0;  // window w: local window v of segment sg
         for (uint32_t w = 0; w < m.nwin; w++) {
             uint32_t d1 = window_bits128(sp.k1, w, c) + carry1;
             uint32_t d2 = window_bits128(sp.k2, w, c) + carry2;
@@ -482,12 +531,17 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
             carry2 = d2 > half;
             if (carry1) { d1 = full - d1; neg1 ^= 1u; }
             if (carry2) { d2 = full - d2; neg2 ^= 1u; }
-            uint32_t w1 = d1 ? (key_encode(d1 - 1u) | (neg1 << 31)) : ZK_KEY_NONE;
-            uint32_t w2 = d2 ? (key_encode(d2 - 1u) | (neg2 << 31)) : ZK_KEY_NONE;
-            const uint64_t e = segs[m.seg0 + w].ent0 + 2 * i;
+            const uint32_t vt = v << 16;
+            uint32_t w1 = d1 ? (key_encode(d1 - 1u) | vt | (neg1 << 31)) : ZK_KEY_NONE;
+            uint32_t w2 = d2 ? (key_encode(d2 - 1u) | vt | (neg2 << 31)) : ZK_KEY_NONE;
+            const uint64_t e = segs[sg].ent0 + v * vstride + 2 * i;
             digits[e >> 1] = make_uint2(w1, w2);
             atomicAdd(&cnt[w * 128 + ((w1 & 0xffu) >> 1)], 1u << ((w1 & 1u) * 16));
             atomicAdd(&cnt[w * 128 + ((w2 & 0xffu) >> 1)], 1u << ((w2 & 1u) * 16));
+            if (++v == m.wps) {
+                v = 0;
+                sg++;
+            }
         }
         carry_left |= (carry1 | carry2) != 0;
     }
@@ -495,8 +549,8 @@ __global__ __launch_bounds__(kDecThreads) void k_decompose(const MsmDesc* __rest
     __syncthreads();
     for (uint32_t x = threadIdx.x; x < m.nwin * 256u; x += kDecThreads) {
         const uint32_t w = x >> 8, d = x & 255u;
-        const SegDesc sg = segs[m.seg0 + w];
-        hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + tin] =
+        const SegDesc sg = segs[m.seg0 + w / m.wps];
+        hist[(uint64_t)sg.tile0 * 256 + (uint64_t)d * sg.ntiles + (w % m.wps) * ntw + tin] =
             (cnt[w * 128 + (d >> 1)] >> ((d & 1u) * 16)) & 0xffffu;
     }
 }
@@ -686,7 +740,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
             if (pass == 0) {
                 const uint32_t wd = reinterpret_cast<const uint32_t*>(in_keys)[gp];
                 key[r] = wd & 0xffffu;
-                val[r] = (start + pos) | (wd & 0x80000000u);
+                // the table entry: the position, shifted to its window's copy
+                val[r] = (start + pos + ((wd >> 16) & 0xffu) * sg.wadj + sg.wbase) | (wd & 0x80000000u);
             } else {
                 key[r] = reinterpret_cast<const uint16_t*>(in_keys)[gp];
                 val[r] = in_vals[gp];
@@ -1095,14 +1150,16 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
 
 // ------------------------------------------------ window combine on the device
 // The device form of the call's tail (the host form is Engine::run's
-// host_tail): result_j = sum_w 2^(c w) S_w with S_w = U_w + sum_{t < c-1}
-// 2^t Q_{w,t}, the planes k_br_bits stored (bits[s*16] = U, bits[s*16 + 1 +
-// t] = Q_t), then the affine normalisation.  The reference does its window
-// combine on the host (Sources/zkMetal/MSM/MSMEngine.swift:899-904); on the
-// device it lets a captured graph end with final results.  The doubling chain
-// is inherently serial (2^127 S_top needs 127 doublings), so one lane per
-// window computes S_w (Horner over its planes), then one lane per MSM runs
-// the Horner over windows and the inversion.
+// host_tail): result_j = sum_w 2^(cstep w) S_w over the MSM's segments w,
+// S_w = U_w + sum_{t < c-1} 2^t Q_{w,t}, the planes k_br_bits stored
+// (bits[s*16] = U, bits[s*16 + 1 + t] = Q_t), then the affine normalisation;
+// cstep = c for plain windows, 0 for window-shifted segments (a plain sum).
+// The reference does its window combine on the host
+// (Sources/zkMetal/MSM/MSMEngine.swift:899-904); on the device it lets a
+// captured graph end with final results.  The doubling chain is inherently
+// serial (2^127 S_top needs 127 doublings), so one lane per segment computes
+// S_w (Horner over its planes), then one lane per MSM runs the Horner over
+// segments and the inversion.
 
 template <class C>
 __global__ __launch_bounds__(64) void k_tail_windows(const SegDesc* __restrict__ segs,
@@ -1130,10 +1187,12 @@ __global__ __launch_bounds__(64) void k_tail_combine(const MsmDesc* __restrict__
     const uint32_t j = blockIdx.x * 64 + threadIdx.x;
     if (j >= nmsm) return;
     const MsmDesc m = msms[j];
-    typename C::Pt acc = C::load(wins, m.seg0 + m.nwin - 1);
+    typename C::Pt acc = C::load(wins, m.seg0 + m.nseg - 1);
 #pragma unroll 1
-    for (int w = (int)m.nwin - 2; w >= 0; w--) {
-        acc = C::dbl_k(acc, m.c);  // 2^c acc, in Jacobian coordinates
+    for (int w = (int)m.nseg - 2; w >= 0; w--) {
+        // 2^c acc, in Jacobian coordinates (window-shifted segments: cstep = 0,
+        // their sums already carry the window weights)
+        if (m.cstep) acc = C::dbl_k(acc, m.cstep);
         C::add(acc, C::load(wins, m.seg0 + w));
     }
     C::store_result(out, j, acc);
@@ -1519,6 +1578,12 @@ void launch_build_table(CurveId cv, const void* pts, void* table, uint64_t n, hi
     else
         hipLaunchKernelGGL((k_build_table<Bn254Fp, 2>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
                            (const uint4*)pts, (uint4*)table, idm, n);
+}
+
+void launch_shift_windows(void* table, uint64_t n, uint64_t stride, uint32_t c, uint32_t nwin,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_shift_windows, dim3(blocks_for(n, 256)), dim3(256), 0, st, (uint4*)table, n,
+                       stride, c, nwin);
 }
 
 void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,

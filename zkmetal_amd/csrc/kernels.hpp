@@ -18,6 +18,11 @@ inline uint64_t table_bytes(uint64_t n, CurveId cv = CurveId::Bn254) {
 void launch_import_be(const void* raw, uint64_t n, void* table, uint32_t* bad, hipStream_t st);
 void launch_table_export(CurveId cv, const void* table, uint64_t n, void* pts, hipStream_t st);
 void launch_build_table(CurveId cv, const void* pts, void* table, uint64_t n, hipStream_t st);
+// BN254 G1 table of n bases: window-shifted copies 2^(c w) P_i, phi(2^(c w) P_i)
+// at entries w * stride + 2i, 2i + 1 for w = 1 .. nwin-1 (the table's
+// allocation holds nwin * stride entries; stride * 128 >= table_bytes(n))
+void launch_shift_windows(void* table, uint64_t n, uint64_t stride, uint32_t c, uint32_t nwin,
+                          hipStream_t st);
 // one block per first-pass sort tile of each MSM (msm_dtile0, dtiles blocks):
 // digits, and the first sort pass's tile histograms into hist
 void launch_decompose(CurveId cv, const MsmDesc* msms, const uint64_t* msm_dtile0, uint32_t nmsm,

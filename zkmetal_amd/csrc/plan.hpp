@@ -4,9 +4,15 @@
 // width c_j and W_j = ceil(128 / c_j) signed windows; its GLV half-scalars
 // address a base table laid out as [P_0, phi(P_0), P_1, phi(P_1), ...]
 // (affine, 128 bytes per entry), so "entry" 2i+h of an MSM is base i
-// (h = 0) or its endomorphism image (h = 1).  Every (MSM, window) pair is a
-// segment: 2 n_j digit entries that are sorted by bucket, accumulated into
-// 2^(c_j - 1) buckets and reduced to one window sum.
+// (h = 0) or its endomorphism image (h = 1).  A segment is a set of digit
+// entries that are sorted by bucket, accumulated into 2^(c_j - 1) buckets and
+// reduced to one window sum.  Plain plan: every (MSM, window) pair is a
+// segment of 2 n_j entries, and the window sums combine as sum_w 2^(c w) S_w.
+// Window-shifted plan (a resident table that also holds the copies
+// 2^(c w) P_i and their phi images for every window w, at entry w * stride):
+// a segment holds the digits of wps consecutive windows, each window's 2 n_j
+// entries padded to whole sort tiles, and its digits address the shifted
+// copies, so the segments' sums simply add up.
 #pragma once
 #include <stdint.h>
 
@@ -48,10 +54,12 @@ struct MsmDesc {
     uint64_t table0;   // first table entry (affine point index)
     uint32_t n;        // number of bases / scalars
     uint32_t c;        // window width in bits (<= 16)
-    uint32_t nwin;     // ceil(128 / c)
+    uint32_t nwin;     // ceil(128 / c) digit windows
     uint32_t seg0;     // first segment
     uint32_t form;     // scalar encoding: 0 standard form, 1 Fr Montgomery form
-    uint32_t reserved;
+    uint32_t nseg;     // segments (nwin plain; ceil(nwin / wps) window-shifted)
+    uint32_t wps;      // windows per segment (1 plain)
+    uint32_t cstep;    // doublings between consecutive segments' sums (c plain, 0 shifted)
 };
 
 struct SegDesc {
@@ -59,15 +67,21 @@ struct SegDesc {
     uint64_t bkt0;    // first bucket in the bucket array
     uint64_t lane0;   // first accumulation lane
     uint64_t grp0;    // first bucket-reduction group
-    uint32_t len;     // entries = 2 n
+    uint32_t len;     // entries: 2 n plain, wps windows of padded 2 n shifted
     uint32_t nbk;     // buckets = 2^(c-1)
     uint32_t tile0;   // first sort tile
     uint32_t ntiles;  // sort tiles of this segment
     uint32_t msm;     // owning MSM
-    uint32_t win;     // window index
+    uint32_t wadj;    // window-shifted: table stride - padded window length (0 plain)
     uint32_t chunk0;  // first histogram-scan chunk (kScanChunk counts each)
     uint32_t nchunks;
+    uint32_t wbase;   // window-shifted: table entry of the segment's first window (0 plain)
+    uint32_t pad;
 };
+// A digit at position pos of segment s, written for the segment's local
+// window v (digit word bits 16..23), addresses table entry
+// pos + v * wadj + wbase relative to the MSM's table0: for a plain segment
+// (v = 0, wadj = wbase = 0) the position itself.
 
 // Merge records: key bit 31 set = no payload (padding); bits 0..30 = index of
 // the bucket (or window sum) the payload belongs to.

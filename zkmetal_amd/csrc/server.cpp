@@ -27,6 +27,7 @@ int32_t Server::create(Engine* owner, const ServerConfig& cfg, Server** out) {
     Server* s = new Server();
     s->table_ = table;
     s->table_n_ = table_n;
+    s->table_shift_ = owner->srs_shift_stride();
     s->cfg_ = cfg;
     for (uint32_t i = 0; i < cfg.slots; i++) {
         Engine* e = nullptr;
@@ -137,7 +138,7 @@ void Server::worker(size_t slot) {
         {
             std::lock_guard<std::mutex> elk(eng->mutex());
             rc = eng->msm_table_parts(table_, table_n_, parts.data(), nullptr, counts.data(),
-                                      (uint32_t)counts.size(), res.data());
+                                      (uint32_t)counts.size(), res.data(), 0, table_shift_);
         }
         const auto now = std::chrono::steady_clock::now();
         {

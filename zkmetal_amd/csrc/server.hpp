@@ -75,6 +75,7 @@ class Server {
     Engine* owner_ = nullptr;
     const void* table_ = nullptr;
     uint32_t table_n_ = 0;
+    uint64_t table_shift_ = 0;  // the owner's window-shifted copies (stride, 0: none)
     ServerConfig cfg_;
     std::vector<Engine*> engines_;
     std::vector<std::thread> threads_;
