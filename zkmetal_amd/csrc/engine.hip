@@ -330,9 +330,7 @@ int32_t Engine::create(Engine** out) {
                 ok = ok && hipEventCreate(ev) == hipSuccess;
         ok = ok && hipEventCreateWithFlags(&e->dep_fork_, hipEventDisableTiming) == hipSuccess;
         for (uint32_t p = 0; p < kMaxParts; p++)
-            ok = ok && hipEventCreateWithFlags(&e->dep_sort_[p], hipEventDisableTiming) == hipSuccess &&
-                 hipEventCreateWithFlags(&e->dep_acc_[p], hipEventDisableTiming) == hipSuccess;
-        ok = ok && hipEventCreateWithFlags(&e->dep_join_, hipEventDisableTiming) == hipSuccess;
+            ok = ok && hipEventCreateWithFlags(&e->dep_sort_[p], hipEventDisableTiming) == hipSuccess;
         if (!ok) {
             delete e;
             return kGpuError;
@@ -376,9 +374,9 @@ Engine::~Engine() {
     if (stream_sort_) (void)hipStreamSynchronize(stream_sort_);
     for (uint32_t p = 0; p < kMaxParts; p++)
         for (hipEvent_t ev : {ev_s0_[p], ev_s1_[p], ev_a0_[p], ev_a1_[p], ev_m1_[p], ev_r0_[p], ev_r1_[p],
-                              dep_sort_[p], dep_acc_[p]})
+                              dep_sort_[p]})
             if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {ev_dec_, dep_fork_, dep_join_})
+    for (hipEvent_t ev : {ev_dec_, dep_fork_})
         if (ev) (void)hipEventDestroy(ev);
     if (stream_sort_) (void)hipStreamDestroy(stream_sort_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -392,7 +390,7 @@ Engine::~Engine() {
 
 // pack the plan's descriptor arrays into one blob (16-byte aligned sections)
 struct DescLayout {
-    size_t msms, msm_scalar0, msm_dtile0, segs, seg_lane0, seg_grp0, seg_rcw0, tile_seg, chunk_seg,
+    size_t msms, msm_scalar0, msm_dtile0, segs, seg_lane0, seg_grp0, seg_rcw0, tile_seg, chunk_seg, part_T,
         total;
 };
 
@@ -413,6 +411,7 @@ static DescLayout layout_for(const Plan& pl) {
     L.seg_rcw0 = take(pl.seg_rcw0.size() * 8);
     L.tile_seg = take(pl.tile_seg.size() * 4);
     L.chunk_seg = take(pl.chunk_seg.size() * 4);
+    L.part_T = take(kMaxParts * 4);  // entries per lane of each part (k_lane_plan)
     L.total = off;
     return L;
 }
@@ -638,12 +637,17 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     memcpy(hb + L.seg_rcw0, pl.seg_rcw0.data(), pl.seg_rcw0.size() * 8);
     memcpy(hb + L.tile_seg, pl.tile_seg.data(), pl.tile_seg.size() * 4);
     memcpy(hb + L.chunk_seg, pl.chunk_seg.data(), pl.chunk_seg.size() * 4);
+    for (uint32_t p = 0; p < kMaxParts; p++)
+        reinterpret_cast<uint32_t*>(hb + L.part_T)[p] = p < pl.nparts ? pl.parts[p].T : 0u;
     uint8_t* db = ws.desc;
     const MsmDesc* d_msms = (const MsmDesc*)(db + L.msms);
     const uint64_t* d_msm_scalar0 = (const uint64_t*)(db + L.msm_scalar0);
     const uint64_t* d_msm_dtile0 = (const uint64_t*)(db + L.msm_dtile0);
     const SegDesc* d_segs = (const SegDesc*)(db + L.segs);
-    const uint64_t* d_seg_lane0 = (const uint64_t*)(db + L.seg_lane0);
+    // seg_lane0 and part_T: the host plan's values, rewritten on the device
+    // per part by k_lane_plan from the sorted valid counts
+    uint64_t* d_seg_lane0 = (uint64_t*)(db + L.seg_lane0);
+    uint32_t* d_part_T = (uint32_t*)(db + L.part_T);
     const uint64_t* d_seg_grp0 = (const uint64_t*)(db + L.seg_grp0);
     const uint64_t* d_seg_rcw0 = (const uint64_t*)(db + L.seg_rcw0);
     const uint32_t* d_tile_seg = (const uint32_t*)(db + L.tile_seg);
@@ -654,8 +658,7 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     // merge lists of part p: a listed run has more than kRunLimit records, so
     // each list holds at most nrec / (kRunLimit + 1) runs
     // each list holds at most nrec / (kRunLimit + 1) runs; the parts' lists
-    // sit side by side in recBk (2 * lanes + 2 words hold them all), as the
-    // merges of different parts may overlap
+    // sit side by side in recBk (2 * lanes + 2 words hold them all)
     uint32_t list_cap[kMaxParts];
     uint64_t list_off[kMaxParts];
     uint64_t off = 0;
@@ -667,12 +670,11 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     }
     uint8_t* hw = ws.hw;
     // The device part of the call.  Stream A (the engine stream): descriptors,
-    // decomposition, then per part the accumulation (and the last part's
-    // merge), then the bucket reduction and the device tail; stream S: the
-    // two sort passes of each part, so that part p + 1 sorts while part p
+    // decomposition, then per part the accumulation and the merge, then the
+    // bucket reduction and the device tail; stream S: the two sort passes and
+    // the lane plan of each part, so that part p + 1 sorts while part p
     // accumulates (the accumulation leaves registers and LDS for the sort's
-    // workgroups), then the merges of all parts but the last, each beside the
-    // next part's accumulation.
+    // workgroups).
     // Timing events are recorded only outside graph capture; the dependency
     // events are part of a captured graph.
     auto enqueue = [&](bool events) -> int32_t {
@@ -686,11 +688,11 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
         launch_decompose(pl.curve, d_msms, d_msm_dtile0, nmsm, pl.dtiles, d_segs, d_scalars, d_table, table_n,
                          ws.digits, (uint32_t*)ws.hist, d_err, st);
         ZK_TRY(rec(ev_dec_, st));
-        launch_fill_zero(ws.buckets, pl.buckets * ptb, st);
-        if (P > 1) {
+        if (P > 1) {  // the sorts need the digits, not the cleared buckets
             ZK_TRY(hipEventRecord(dep_fork_, st));
             ZK_TRY(hipStreamWaitEvent(S, dep_fork_, 0));
         }
+        launch_fill_zero(ws.buckets, pl.buckets * ptb, st);
         for (uint32_t p = 0; p < P; p++) {
             const PlanPart& pt = pl.parts[p];
             const uint32_t ns_p = pt.seg_hi - pt.seg_lo;
@@ -703,6 +705,8 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                              pt.chunk_lo, pt.chunk_hi - pt.chunk_lo, ws.keysA, (const uint32_t*)ws.valsA, 1,
                              (uint32_t*)ws.hist, (uint32_t*)ws.part, (uint32_t*)ws.segvalid,
                              (uint16_t*)ws.keysB, (uint32_t*)ws.valsB, S);
+            launch_lane_plan(d_segs, (const uint32_t*)ws.segvalid, pt.seg_lo, ns_p, pt.lane_lo, pt.lane_hi,
+                             d_seg_lane0, d_part_T + p, S);
             ZK_TRY(rec(ev_s1_[p], S));
             if (P > 1) ZK_TRY(hipEventRecord(dep_sort_[p], S));
         }
@@ -710,30 +714,21 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
             const PlanPart& pt = pl.parts[p];
             if (P > 1) ZK_TRY(hipStreamWaitEvent(st, dep_sort_[p], 0));
             ZK_TRY(rec(ev_a0_[p], st));
-            launch_accumulate(pl.curve, d_segs, d_seg_lane0, nseg, d_msms, (const uint32_t*)ws.segvalid,
-                              (const uint16_t*)ws.keysB, (const uint32_t*)ws.valsB, d_table, pt.T, pt.lane_lo,
-                              pt.lane_hi, ws.buckets, (uint32_t*)ws.recAk, ws.recAp, st);
+            launch_accumulate(pl.curve, d_segs, d_seg_lane0, pt.seg_lo, pt.seg_hi - pt.seg_lo, d_msms,
+                              (const uint32_t*)ws.segvalid, (const uint16_t*)ws.keysB, (const uint32_t*)ws.valsB,
+                              d_table, d_part_T + p, pt.lane_lo, pt.lane_hi, ws.buckets, (uint32_t*)ws.recAk,
+                              ws.recAp, st);
             ZK_TRY(rec(ev_a1_[p], st));
             // bucket partials that crossed lane boundaries: short runs by their
-            // first lane, longer runs (heavy buckets) by lane groups.  Part p's
-            // merge touches only its own segments' buckets and records, so
-            // before the last part it runs on stream S beside the next part's
-            // accumulation (after the sorts queued there)
-            hipStream_t ms = st;
-            if (p + 1 < P) {
-                ZK_TRY(hipEventRecord(dep_acc_[p], st));
-                ZK_TRY(hipStreamWaitEvent(S, dep_acc_[p], 0));
-                ms = S;
-            }
+            // first lane, longer runs (heavy buckets) by lane groups.  (On
+            // stream S beside the next part's accumulation the merges measured
+            // 1 ms slower on skewed scalars, whose heavy buckets take the
+            // long-run kernel, and at most 0.1 ms faster elsewhere.)
             launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk + 2 * pt.lane_lo,
                               (const uint8_t*)ws.recAp + 2 * pt.lane_lo * ptb, 2 * (pt.lane_hi - pt.lane_lo),
                               kRunLimit, ws.buckets, d_err + 1 + 2 * p, (uint32_t*)ws.recBk + list_off[p],
-                              list_cap[p], ms);
-            ZK_TRY(rec(ev_m1_[p], ms));
-        }
-        if (P > 1) {  // join: the reduction reads every part's merged buckets
-            ZK_TRY(hipEventRecord(dep_join_, S));
-            ZK_TRY(hipStreamWaitEvent(st, dep_join_, 0));
+                              list_cap[p], st);
+            ZK_TRY(rec(ev_m1_[p], st));
         }
         // the bucket reduction of all parts at once, after the last merge: as
         // a kernel beside the next part's accumulation it finds no free slots

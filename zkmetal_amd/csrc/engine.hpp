@@ -390,8 +390,6 @@ class Engine {
                ev_a1_[kMaxParts] = {}, ev_m1_[kMaxParts] = {}, ev_r0_[kMaxParts] = {}, ev_r1_[kMaxParts] = {};
     hipEvent_t dep_fork_ = nullptr;
     hipEvent_t dep_sort_[kMaxParts] = {};
-    hipEvent_t dep_acc_[kMaxParts] = {};  // part p accumulated: its merge may start on the sort stream
-    hipEvent_t dep_join_ = nullptr;       // the sort stream's last merge, before the reduction
     uint32_t parts_req_ = 0;  // 0: default_parts
     // no-allocation path: this device's scratch array, a plan and host
     // staging whose capacity was reserved at creation

@@ -827,8 +827,9 @@ __global__ __launch_bounds__(256) void k_fill_zero(uint4* __restrict__ p, uint64
 }
 
 // --------------------------------------------------- bucket accumulation
-// Lane L of segment s owns sorted entries [ent0 + l*T, ent0 + (l+1)*T) clipped
-// to the segment's valid entries.  It walks them once with mixed additions;
+// Lane L of segment s (l = L - seg_lane0[s]; T and seg_lane0 as k_lane_plan
+// set them for the part) owns sorted entries [ent0 + l*T, ent0 + (l+1)*T)
+// clipped to the segment's valid entries.  It walks them once with mixed additions;
 // a bucket run that starts and ends inside the range is final and is written
 // to the bucket array, the (at most two) runs that cross the range ends go to
 // merge records 2L (first run) and 2L+1 (last run).
@@ -840,18 +841,56 @@ __device__ __forceinline__ void put_record(uint32_t* keys, uint4* pts, uint64_t 
     if (valid) C::store(pts, slot, v);
 }
 
+// Lanes of one part, [seg_lo, seg_lo + nseg), from the sort's counts of
+// entries that have a bucket (seg_valid): the planner sizes T for all of a
+// segment's entries, but the sort drops the zero digits -- 70% of them for
+// witness-like scalars -- and the lanes past a segment's valid entries have
+// nothing to do.  Here T = max(4, min(32, half a bucket's mean valid load),
+// valid / (lanes - nseg)), so the part's planned lanes (one resident round)
+// are all busy again, and seg_lane0[s] = lane_lo + sum over the earlier
+// segments of ceil(valid / T) (at most lanes in all: each segment rounds up
+// by less than one lane).  One workgroup; T goes to *Tout for k_accumulate.
+__global__ __launch_bounds__(256) void k_lane_plan(const SegDesc* __restrict__ segs,
+                                                   const uint32_t* __restrict__ seg_valid, uint32_t seg_lo,
+                                                   uint32_t nseg, uint64_t lane_lo, uint64_t lanes,
+                                                   uint64_t* __restrict__ seg_lane0, uint32_t* __restrict__ Tout) {
+    __shared__ uint32_t lds4[4];
+    uint32_t v = 0, b = 0;
+    for (uint32_t i = threadIdx.x; i < nseg; i += 256) {
+        v += seg_valid[seg_lo + i];
+        b += segs[seg_lo + i].nbk;
+    }
+    uint32_t V, B;
+    (void)block_exclusive_scan(v, lds4, &V);
+    (void)block_exclusive_scan(b, lds4, &B);
+    const uint64_t cap = lanes > nseg ? lanes - nseg : 1;
+    uint32_t T = max(4u, min(32u, V / (2u * max(B, 1u))));
+    T = max(T, (uint32_t)((V + cap - 1) / cap));
+    if (threadIdx.x == 0) *Tout = T;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < nseg; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t c = i < nseg ? (seg_valid[seg_lo + i] + T - 1) / T : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(c, lds4, &tot);
+        if (i < nseg) seg_lane0[seg_lo + i] = lane_lo + carry + ex;
+        carry += tot;
+    }
+}
+
 template <class C>
 __global__ __launch_bounds__(256) void k_accumulate(
-    const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_lane0, uint32_t nseg,
+    const SegDesc* __restrict__ segs, const uint64_t* __restrict__ seg_lane0, uint32_t seg_lo, uint32_t nseg,
     const MsmDesc* __restrict__ msms, const uint32_t* __restrict__ seg_valid,
     const uint16_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const uint4* __restrict__ table, uint32_t T, uint64_t lane_lo, uint64_t lane_hi,
+    const uint4* __restrict__ table, const uint32_t* __restrict__ Tp, uint64_t lane_lo, uint64_t lane_hi,
     uint4* __restrict__ buckets, uint32_t* __restrict__ rec_keys, uint4* __restrict__ rec_pts) {
     const uint64_t L = lane_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= lane_hi) return;
-    const uint32_t s = upper_index(seg_lane0, nseg, L);
+    const uint32_t T = *Tp;  // k_lane_plan's entries per lane for this part
+    const uint32_t s = seg_lo + upper_index(seg_lane0 + seg_lo, nseg, L);
     const SegDesc sg = segs[s];
-    const uint64_t l = L - sg.lane0;
+    const uint64_t l = L - seg_lane0[s];
     const uint64_t segend = sg.ent0 + seg_valid[s];
     const uint64_t b = sg.ent0 + l * T;
     const uint64_t e = min(b + T, segend);
@@ -1643,15 +1682,21 @@ uint64_t accumulate_capacity(int device, CurveId cv) {
     return (uint64_t)cus * (uint64_t)blocks * 256u;
 }
 
-void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
-                       const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
-                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lane_lo,
+void launch_lane_plan(const SegDesc* segs, const uint32_t* seg_valid, uint32_t seg_lo, uint32_t nseg,
+                      uint64_t lane_lo, uint64_t lane_hi, uint64_t* seg_lane0, uint32_t* T, hipStream_t st) {
+    hipLaunchKernelGGL(k_lane_plan, dim3(1), dim3(256), 0, st, segs, seg_valid, seg_lo, nseg, lane_lo,
+                       lane_hi - lane_lo, seg_lane0, T);
+}
+
+void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t seg_lo,
+                       uint32_t nseg, const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
+                       const uint32_t* vals, const void* table, const uint32_t* T, uint64_t lane_lo,
                        uint64_t lane_hi, void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st) {
     if (lane_hi <= lane_lo) return;
 #define ZK_ACC(C)                                                                                  \
     hipLaunchKernelGGL(k_accumulate<C>, dim3(blocks_for(lane_hi - lane_lo, 256)), dim3(256), 0, st, \
-                       segs, seg_lane0, nseg, msms, seg_valid, keys, vals, (const uint4*)table, T,   \
-                       lane_lo, lane_hi, (uint4*)buckets, rec_keys, (uint4*)rec_pts)
+                       segs, seg_lane0, seg_lo, nseg, msms, seg_valid, keys, vals, (const uint4*)table, \
+                       T, lane_lo, lane_hi, (uint4*)buckets, rec_keys, (uint4*)rec_pts)
     if (cv == CurveId::Grumpkin) ZK_ACC(G1Curve<Bn254Fr>);
     else if (cv == CurveId::Bn254G2) ZK_ACC(G2Curve);
     else ZK_ACC(G1Curve<Bn254Fp>);

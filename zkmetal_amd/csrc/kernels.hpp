@@ -41,10 +41,16 @@ void launch_fill_zero(void* p, uint64_t bytes, hipStream_t st);
 // CU at its register use x 256): a lane count just above it leaves a second,
 // nearly empty round of waves
 uint64_t accumulate_capacity(int device, CurveId cv = CurveId::Bn254);
-// lanes [lane_lo, lane_hi) of the call's lane space
-void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t nseg,
-                       const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
-                       const uint32_t* vals, const void* table, uint32_t T, uint64_t lane_lo,
+// after a part's sort: its entries per lane (*T) and the first lane of each
+// of its segments (seg_lane0[seg_lo .. seg_lo + nseg)) from the segments'
+// valid-entry counts, within the part's lanes [lane_lo, lane_hi)
+void launch_lane_plan(const SegDesc* segs, const uint32_t* seg_valid, uint32_t seg_lo, uint32_t nseg,
+                      uint64_t lane_lo, uint64_t lane_hi, uint64_t* seg_lane0, uint32_t* T, hipStream_t st);
+// lanes [lane_lo, lane_hi) of the call's lane space: the part of segments
+// [seg_lo, seg_lo + nseg), laid out by launch_lane_plan
+void launch_accumulate(CurveId cv, const SegDesc* segs, const uint64_t* seg_lane0, uint32_t seg_lo,
+                       uint32_t nseg, const MsmDesc* msms, const uint32_t* seg_valid, const uint16_t* keys,
+                       const uint32_t* vals, const void* table, const uint32_t* T, uint64_t lane_lo,
                        uint64_t lane_hi, void* buckets, uint32_t* rec_keys, void* rec_pts, hipStream_t st);
 // counts[0] / counts[1]: long / mid run counts (zeroed by the caller);
 // lists: 2 * list_cap words (long list, then mid list)
