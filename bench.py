@@ -401,14 +401,14 @@ def batched(args, eng, peak, kind=None):
     dsc = zk.DeviceBuffer(scal.nbytes)
     dsc.upload(scal)
     # stage split from one eager call (a graph replay has one device span)
-    eng.set_graph_mode(False)
+    eng.set_graph_mode(eng.GRAPH_OFF)
     eager_res = eng.msm_batch_resident(dsc, counts)
     tb = eng.last_timing()
     stage = {k: round(tb[k], 4) for k in ("decompose_ms", "sort_ms", "accumulate_ms", "merge_ms",
                                            "reduce_ms", "device_tail_ms", "copyout_ms", "device_ms",
                                            "host_tail_ms")}
     graph = not args.no_graph
-    eng.set_graph_mode(graph)
+    eng.set_graph_mode(eng.GRAPH_ON if graph else eng.GRAPH_OFF)
     caps0 = eng.graph_captures()
     res = {}
 
@@ -419,7 +419,7 @@ def batched(args, eng, peak, kind=None):
     batched_res = res["r"]
     captures = eng.graph_captures() - caps0
     device_ms = eng.last_timing()["device_ms"]
-    eng.set_graph_mode(False)
+    eng.set_graph_mode(eng.GRAPH_OFF)
     # back-to-back plain launches of the same MSMs on the same device
     offs = np.concatenate([[0], np.cumsum(counts)])
     views = []
@@ -665,6 +665,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     eng = zk.MetalMSM()
+    # every call eagerly, so that HIP events time each kernel stage; the
+    # batched legs turn graph replays on for their timed steps themselves
+    eng.set_graph_mode(eng.GRAPH_OFF)
     if args.profile:
         profile(args, eng)
         eng.close()

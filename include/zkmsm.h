@@ -213,13 +213,18 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
 ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
                                            uint32_t len);
 
-/* Graph mode (off by default): the device launch sequence of a call (all of
+/* Graph mode.  mode 1 (on): the device launch sequence of a call (all of
  * its MSMs: decomposition, sort, accumulation, merge, reduction, copy-out) is
  * captured once into a hipGraph and replayed by later calls of the same
- * shape (MSM sizes, window widths, table and scalar addresses); stage times
- * in zkmsm_last_timing are then 0 except the device total.  A call of a new
- * shape captures again.  zkmsm_graph_captures counts the captures. */
-ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on);
+ * shape (MSM sizes, window widths, table and scalar addresses); a call of a
+ * new shape captures again, and the window combine runs on the device
+ * (zkmsm_set_tail_mode auto).  mode 2 (auto, the default): the same from the
+ * second call in a row of one shape (a one-off call runs eagerly and pays no
+ * capture), with the tail where zkmsm_set_tail_mode puts it for eager calls.
+ * mode 0 (off): every call eagerly.  Stage times in zkmsm_last_timing are 0
+ * for a replay except the device total.  zkmsm_graph_captures counts the
+ * captures.  Results are the same bytes in every mode. */
+ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t mode);
 
 /* A large call runs as a pipeline over parts of its bucket segments: the
  * bucket sort of part p + 1 on a second stream while part p accumulates,

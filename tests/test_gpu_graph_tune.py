@@ -50,6 +50,34 @@ def test_graph_replays_match_eager(srs):
     assert srs.msm_batch_resident(other, [3000]) == eager_other
 
 
+def test_auto_mode_replays_a_repeated_shape(srs):
+    counts = [N_SRS, 20000, 1, 300]
+    scal = zk.gen_scalars(SEED + 5, sum(counts), 1)
+    other = zk.gen_scalars(SEED + 6, 3000, 0)
+    d = zk.DeviceBuffer(scal.nbytes)
+    d.upload(scal)
+    try:
+        srs.set_graph_mode(srs.GRAPH_OFF)
+        eager = srs.msm_batch_resident(d, counts)
+        eager_other = srs.msm_batch_resident(other, [3000])
+        srs.set_graph_mode(srs.GRAPH_AUTO)
+        c0 = srs.graph_captures()
+        # first call eager, the second (same shape and buffers) captures, the
+        # rest replay
+        assert [srs.msm_batch_resident(d, counts) for _ in range(4)] == [eager] * 4
+        assert srs.graph_captures() - c0 == 1
+        assert srs.last_timing()["accumulate_ms"] == 0  # a replay: device total only
+        # a one-off shape runs eagerly and captures nothing; the captured shape
+        # then replays again
+        assert srs.msm_batch_resident(other, [3000]) == eager_other
+        assert srs.last_timing()["accumulate_ms"] > 0
+        assert srs.msm_batch_resident(d, counts) == eager
+        assert srs.graph_captures() - c0 == 1
+    finally:
+        srs.set_graph_mode(srs.GRAPH_AUTO)
+        d.free()
+
+
 def test_tuner_tables_keep_results(srs):
     sc = zk.gen_scalars(SEED + 3, 3000, 2)
     before = srs.msm_batch_resident(sc, [3000])

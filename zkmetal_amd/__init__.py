@@ -532,10 +532,14 @@ class MetalMSM:
         fn = lib().zkmsm_set_batch_window_table if batch else lib().zkmsm_set_window_table
         _check(fn(self.handle, ctypes.c_void_p(arr.ctypes.data), len(arr)), "set_window_table")
 
-    def set_graph_mode(self, on):
-        """Capture each call's device launch sequence in a hipGraph and replay
-        it for later calls of the same shape."""
-        _check(lib().zkmsm_set_graph_mode(self.handle, 1 if on else 0), "set_graph_mode")
+    GRAPH_OFF, GRAPH_ON, GRAPH_AUTO = 0, 1, 2
+
+    def set_graph_mode(self, mode):
+        """GRAPH_ON (or True): capture each call's device launch sequence in a
+        hipGraph and replay it for later calls of the same shape; GRAPH_AUTO
+        (the default): the same from the second call in a row of one shape;
+        GRAPH_OFF (or False): every call eagerly."""
+        _check(lib().zkmsm_set_graph_mode(self.handle, int(mode)), "set_graph_mode")
 
     TAIL_AUTO, TAIL_HOST, TAIL_DEVICE = 0, 1, 2
 

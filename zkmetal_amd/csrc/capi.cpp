@@ -565,12 +565,11 @@ uint64_t zkmsm_workspace_bytes(ZkMetalMSMEngine engine) {
     return e->workspace_bytes();
 }
 
-ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t on) {
+ZkMetalStatus zkmsm_set_graph_mode(ZkMetalMSMEngine engine, int32_t mode) {
     Engine* e = as_engine(engine);
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
-    e->set_graph_mode(on != 0);
-    return ZKMETAL_SUCCESS;
+    return e->set_graph_mode(mode);
 }
 
 uint64_t zkmsm_graph_captures(ZkMetalMSMEngine engine) {

@@ -618,7 +618,7 @@ int32_t Engine::run(const Plan& pl, const void* d_scalars, const void* d_table, 
     return run_ws(pl, ws, z, dev_tail, graph_mode_, d_scalars, d_table, table_n, out96);
 }
 
-int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, bool graph,
+int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, int32_t graph_mode,
                        const void* d_scalars, const void* d_table, uint64_t table_n, uint8_t* out96) {
     auto t_wall0 = std::chrono::steady_clock::now();
     const uint32_t nmsm = (uint32_t)pl.msms.size();
@@ -751,11 +751,15 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                               hipMemcpyDeviceToHost, st));
         return kOk;
     };
-    if (graph) {
-        // replay the captured sequence when the call has the same shape (the
-        // curve -- its kernels -- and every launch dimension the capture
-        // baked in) and every buffer it touches is where it was at capture
-        std::vector<uint64_t> key;
+    // The shape key: a captured sequence is replayed when the call has the
+    // same shape (the curve -- its kernels -- and every launch dimension the
+    // capture baked in) and every buffer it touches is where it was at
+    // capture.  Graph mode on: capture on a new key, else replay; auto: the
+    // same from the second call in a row with one key (a one-off call runs
+    // eagerly and pays no capture).
+    bool graph = false;
+    std::vector<uint64_t> key;
+    if (graph_mode != kGraphOff) {
         key.reserve(pl.msms.size() * 3 + 40);
         key.push_back((uint64_t)pl.curve);
         for (const MsmDesc& m : pl.msms) {
@@ -778,6 +782,10 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                  ws.hist, ws.part, ws.segvalid, ws.buckets, ws.recAk, ws.recAp, ws.recBk, ws.rc, ws.bits,
                  ws.err, ws.hw, ws.wins, ws.res})
             key.push_back((uint64_t)(uintptr_t)p);
+        graph = graph_mode == kGraphOn || key == last_key_ || (graph_exec_ && key == graph_key_);
+        last_key_ = key;
+    }
+    if (graph) {
         if (!graph_exec_ || key != graph_key_) {
             if (graph_exec_) {
                 (void)hipGraphExecDestroy(graph_exec_);
@@ -1590,7 +1598,7 @@ int32_t Engine::msm_resident_noalloc(const void* d_scalars, const uint32_t* coun
     uint8_t* res = ws.hw + z.host;
     {
         std::lock_guard<std::mutex> lk(g_noalloc_mu[dev_]);
-        rc = run_ws(pl, ws, z, true, false, d_scalars, srs_.p, srs_n_, res);
+        rc = run_ws(pl, ws, z, true, kGraphOff, d_scalars, srs_.p, srs_n_, res);
     }
     if (rc != kOk) return rc;
     for (uint32_t i = 0; i < k; i++) memcpy(out96 + (uint64_t)idx[i] * 96, res + (uint64_t)i * 96, 96);

@@ -255,7 +255,14 @@ class Engine {
     uint64_t workspace_bytes() const { return ws_.used; }
     // graph mode: the device launch sequence of a call is captured into a
     // hipGraph and replayed while calls keep the same shape and buffers
-    void set_graph_mode(bool on) { graph_mode_ = on; }
+    // kGraphAuto (default): a call whose shape and buffers repeat the
+    // previous call's is captured once and replayed from then on
+    enum GraphMode : int32_t { kGraphOff = 0, kGraphOn = 1, kGraphAuto = 2 };
+    int32_t set_graph_mode(int32_t mode) {
+        if (mode < kGraphOff || mode > kGraphAuto) return kInvalidInput;
+        graph_mode_ = mode;
+        return kOk;
+    }
     // parts a call's segments are split into for the sort / accumulation
     // overlap (0: by size); a measurement knob, results are the same
     void set_parts(uint32_t p) { parts_req_ = std::min<uint32_t>(p, kMaxParts); }
@@ -266,7 +273,8 @@ class Engine {
     static constexpr uint32_t kDeviceTailMinMsms = 16;
     bool device_tail(uint32_t nmsm, CurveId cv) const {
         return cv != CurveId::Bn254G2 &&
-               (tail_mode_ == kTailDevice || (tail_mode_ == kTailAuto && (graph_mode_ || nmsm >= kDeviceTailMinMsms)));
+               (tail_mode_ == kTailDevice ||
+                (tail_mode_ == kTailAuto && (graph_mode_ == kGraphOn || nmsm >= kDeviceTailMinMsms)));
     }
     int32_t set_tail_mode(int32_t m) {
         if (m < kTailAuto || m > kTailDevice) return kInvalidInput;
@@ -311,7 +319,7 @@ class Engine {
 
   private:
     Engine() = default;
-    int32_t run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, bool graph,
+    int32_t run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bool dev_tail, int32_t graph_mode,
                    const void* d_scalars, const void* d_table, uint64_t table_n, uint8_t* out96);
     int32_t noalloc_setup();  // the device array's address, host capacity
     int32_t ensure_gtab();
@@ -328,7 +336,8 @@ class Engine {
     uint32_t c_override_ = 0;
     uint32_t ctab_[33] = {};
     uint32_t ctab_batch_[33] = {};
-    bool graph_mode_ = false;
+    int32_t graph_mode_ = kGraphAuto;
+    std::vector<uint64_t> last_key_;  // the previous call's shape key (kGraphAuto)
     int32_t tail_mode_ = kTailAuto;
     uint64_t acc_cap_ = 1u << 17;  // resident k_accumulate lanes on this device
     uint64_t acc_cap_g2_ = 1u << 16;  // the same for the G2 accumulation
