@@ -1063,12 +1063,13 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
-    Plan pl;
+    // the table build goes first: the host plans the call while it runs
+    ZK_ALLOC(ptable_, table_bytes(total, cv));
+    launch_build_table(cv, d_points, ptable_.p, total, stream_);
+    Plan& pl = plan_;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
                        lane_cap(cv), cv, parts_req_);
     if (rc != kOk) return rc;
-    ZK_ALLOC(ptable_, table_bytes(total, cv));
-    launch_build_table(cv, d_points, ptable_.p, total, stream_);
     std::vector<uint8_t> res(ns.size() * result_bytes(cv));
     rc = run(pl, d_scalars, ptable_.p, total, res.data());
     if (rc != kOk) return rc;
@@ -1417,7 +1418,7 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     }
     if (ns.empty()) return kOk;
     if (total >= (1ull << 30)) return kInvalidInput;
-    Plan pl;
+    Plan& pl = plan_;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_,
                        CurveId::Bn254, parts_req_, plan_shift(table, shift_stride), kShiftC,
                        shift_floor((uint32_t)ns.size()));

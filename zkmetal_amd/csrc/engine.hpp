@@ -354,6 +354,7 @@ class Engine {
     HostBuf hdesc_, hwin_;
     uint32_t srs_n_ = 0;
     int32_t shift_mode_ = kShiftAuto;
+    Plan plan_;  // the device-input and resident calls' plan, its arrays reused from call to call
     uint64_t srs_shift_ = 0;  // entries between the resident table's window copies (0: none)
     // allocate the resident table for n bases; *stride = the window copies'
     // stride when they fit (built by srs_finish), else 0
