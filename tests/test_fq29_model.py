@@ -624,3 +624,4 @@ def test_repeated_doubling_chain():
     z3 = from_int(Z3)
     bad = (from_int(X3) * pow(z3 * z3, -1, P) % P, from_int(Y3) * pow(z3 * z3 * z3, -1, P) % P)
     assert bad != B.add(pt, pt)
+

@@ -1196,27 +1196,35 @@ __global__ __launch_bounds__(256) void k_br_bits(const SegDesc* __restrict__ seg
 // The reference does its window combine on the host
 // (Sources/zkMetal/MSM/MSMEngine.swift:899-904); on the device it lets a
 // captured graph end with final results.  The doubling chain is inherently
-// serial (2^127 S_top needs 127 doublings), so one lane per segment computes
-// S_w (Horner over its planes), then one lane per MSM runs the Horner over
-// segments and the inversion.
+// serial (2^127 S_top needs 127 doublings), so sixteen lanes per segment
+// compute S_w, then one lane per MSM runs the Horner over segments and the
+// inversion.
 
+// k_tail_windows: sixteen lanes per segment (four segments per wave): lane
+// t < c - 1 doubles plane Q_t t times (a Jacobian chain), lane 15 holds U,
+// then a 4-level shuffle tree sums the sixteen.  The longest chain is c - 2
+// doublings and four additions, against c - 2 doublings and c - 1 additions
+// one after another in a Horner over the planes.
 template <class C>
 __global__ __launch_bounds__(64) void k_tail_windows(const SegDesc* __restrict__ segs,
                                                      const MsmDesc* __restrict__ msms, uint32_t nseg,
                                                      const uint4* __restrict__ bits,
                                                      uint4* __restrict__ wins) {
-    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= nseg) return;
+    const uint32_t t = threadIdx.x & 15u;
+    const uint32_t sidx = (blockIdx.x * 64 + threadIdx.x) >> 4;
+    const bool valid = sidx < nseg;
+    const uint32_t s = valid ? sidx : 0u;
     const uint32_t c = msms[segs[s].msm].c;
     const uint64_t b0 = (uint64_t)s * 16;
-    typename C::Pt acc = C::load(bits, b0 + 1 + (c - 2));  // Q_{c-2}
-#pragma unroll 1
-    for (int t = (int)c - 3; t >= 0; t--) {
-        acc = C::dbl(acc);
-        C::add(acc, C::load(bits, b0 + 1 + t));
+    typename C::Pt v = C::identity();
+    if (valid && t + 1 < c) v = C::dbl_k(C::load(bits, b0 + 1 + t), t);  // 2^t Q_t
+    if (valid && t == 15) v = C::load(bits, b0);                         // U (c - 1 <= 15 planes)
+#pragma unroll
+    for (uint32_t d = 8; d >= 1; d >>= 1) {
+        const typename C::Pt o = C::shfl_down(v, d);
+        if (t < d) C::add(v, o);
     }
-    C::add(acc, C::load(bits, b0));  // U
-    C::store(wins, s, acc);
+    if (valid && t == 0) C::store(wins, s, v);
 }
 
 template <class C>
@@ -1765,12 +1773,12 @@ void launch_tail(CurveId cv, const SegDesc* segs, const MsmDesc* msms, uint32_t 
                  const void* bits, void* wins, void* out, hipStream_t st) {
     if (!nseg || !nmsm) return;
     if (cv == CurveId::Grumpkin) {
-        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fr>>, dim3((nseg + 63) / 64), dim3(64), 0, st, segs,
+        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fr>>, dim3((nseg + 3) / 4), dim3(64), 0, st, segs,
                            msms, nseg, (const uint4*)bits, (uint4*)wins);
         hipLaunchKernelGGL(k_tail_combine<G1Curve<Bn254Fr>>, dim3((nmsm + 63) / 64), dim3(64), 0, st, msms,
                            nmsm, (const uint4*)wins, (uint4*)out);
     } else if (cv == CurveId::Bn254) {
-        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fp>>, dim3((nseg + 63) / 64), dim3(64), 0, st, segs,
+        hipLaunchKernelGGL(k_tail_windows<G1Curve<Bn254Fp>>, dim3((nseg + 3) / 4), dim3(64), 0, st, segs,
                            msms, nseg, (const uint4*)bits, (uint4*)wins);
         hipLaunchKernelGGL(k_tail_combine<G1Curve<Bn254Fp>>, dim3((nmsm + 63) / 64), dim3(64), 0, st, msms,
                            nmsm, (const uint4*)wins, (uint4*)out);
