@@ -552,7 +552,7 @@ WsSizes ws_sizes(const Plan& pl, bool dev_tail) {
     z.rc = pl.groups * ptb;
     // bit-plane points: host form for the host tail, stored form for the device's
     z.bits = nseg * 16 * (dev_tail ? ptb : hpb);
-    z.err = 4 * (1 + 2 * kMaxParts) + 12;  // the decomposition flag, two merge counters per part
+    z.err = 16;  // the decomposition flag, the merge's long and mid run counters
     z.wins = dev_tail ? nseg * ptb : 0;
     z.res = dev_tail ? nmsm * result_bytes(pl.curve) : 0;
     // copied back: the results (device tail) or the bit planes, then err, then
@@ -655,19 +655,11 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     uint32_t* d_err = (uint32_t*)ws.err;
     hipStream_t st = stream_;
     const uint32_t P = pl.nparts;
-    // merge lists of part p: a listed run has more than kRunLimit records, so
-    // each list holds at most nrec / (kRunLimit + 1) runs
-    // each list holds at most nrec / (kRunLimit + 1) runs; the parts' lists
-    // sit side by side in recBk (2 * lanes + 2 words hold them all)
-    uint32_t list_cap[kMaxParts];
-    uint64_t list_off[kMaxParts];
-    uint64_t off = 0;
-    for (uint32_t p = 0; p < P; p++) {
-        list_cap[p] = (uint32_t)std::min<uint64_t>(
-            2 * (pl.parts[p].lane_hi - pl.parts[p].lane_lo) / (kRunLimit + 1) + 1, 1u << 29);
-        list_off[p] = off;
-        off += 2 * list_cap[p];
-    }
+    // the merge lists (long and mid runs): a listed run has more than
+    // kRunLimit records, so each list holds at most nrec / (kRunLimit + 1)
+    // runs; both sit in recBk (2 * lanes + 2 words)
+    const uint64_t nrec = 2 * pl.parts[P - 1].lane_hi;
+    const uint32_t list_cap = (uint32_t)std::min<uint64_t>(nrec / (kRunLimit + 1) + 1, 1u << 29);
     uint8_t* hw = ws.hw;
     // The device part of the call.  Stream A (the engine stream): descriptors,
     // decomposition, then per part the accumulation and the merge, then the
@@ -719,17 +711,16 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
                               d_table, d_part_T + p, pt.lane_lo, pt.lane_hi, ws.buckets, (uint32_t*)ws.recAk,
                               ws.recAp, st);
             ZK_TRY(rec(ev_a1_[p], st));
-            // bucket partials that crossed lane boundaries: short runs by their
-            // first lane, longer runs (heavy buckets) by lane groups.  (On
-            // stream S beside the next part's accumulation the merges measured
-            // 1 ms slower on skewed scalars, whose heavy buckets take the
-            // long-run kernel, and at most 0.1 ms faster elsewhere.)
-            launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk + 2 * pt.lane_lo,
-                              (const uint8_t*)ws.recAp + 2 * pt.lane_lo * ptb, 2 * (pt.lane_hi - pt.lane_lo),
-                              kRunLimit, ws.buckets, d_err + 1 + 2 * p, (uint32_t*)ws.recBk + list_off[p],
-                              list_cap[p], st);
-            ZK_TRY(rec(ev_m1_[p], st));
         }
+        // bucket partials that crossed lane boundaries, every part's at once
+        // after the last accumulation (a part's records and buckets are its
+        // own, and records are sorted by bucket across parts): short runs by
+        // their first lane, longer runs (heavy buckets) by lane groups.  One
+        // merge after the last part instead of one after each keeps the
+        // accumulations back to back.
+        launch_merge_runs(pl.curve, (const uint32_t*)ws.recAk, (const uint8_t*)ws.recAp, nrec, kRunLimit,
+                          ws.buckets, d_err + 1, (uint32_t*)ws.recBk, list_cap, st);
+        ZK_TRY(rec(ev_m1_[0], st));
         // the bucket reduction of all parts at once, after the last merge: as
         // a kernel beside the next part's accumulation it finds no free slots
         // (its waves need as many registers as the accumulation's) and
@@ -744,9 +735,9 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
         }
         ZK_TRY(rec(ev_[kStCopyOut], st));
         ZK_TRY(hipMemcpyAsync(hw, dev_tail ? ws.res : ws.bits, back_bytes, hipMemcpyDeviceToHost, st));
-        // err[0]: decomposition error flag; err[1 + 2p], err[2 + 2p]: long / mid
-        // run counts of part p's merge (above list_cap: listed runs were dropped)
-        ZK_TRY(hipMemcpyAsync(hw + back_bytes, ws.err, 4 * (1 + 2 * P), hipMemcpyDeviceToHost, st));
+        // err[0]: decomposition error flag; err[1], err[2]: long / mid run
+        // counts of the merge (above list_cap: listed runs were dropped)
+        ZK_TRY(hipMemcpyAsync(hw + back_bytes, ws.err, 4 * 3, hipMemcpyDeviceToHost, st));
         ZK_TRY(hipMemcpyAsync(hw + back_bytes + z.err, ws.segvalid, (uint64_t)nseg * 4,
                               hipMemcpyDeviceToHost, st));
         return kOk;
@@ -820,11 +811,10 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     }
     ZK_TRY(hipStreamSynchronize(st));
     ZK_TRY(hipGetLastError());
-    uint32_t err[1 + 2 * kMaxParts];
-    memcpy(err, hw + back_bytes, 4 * (1 + 2 * P));
+    uint32_t err[3];
+    memcpy(err, hw + back_bytes, 4 * 3);
     if (err[0]) return kGpuError;
-    for (uint32_t p = 0; p < P; p++)
-        if (err[1 + 2 * p] > list_cap[p] || err[2 + 2 * p] > list_cap[p]) return kGpuError;
+    if (err[1] > list_cap || err[2] > list_cap) return kGpuError;
 
     // stage times: each stage's kernels summed over the parts (they overlap
     // across streams; a graph replay has only the device total)
@@ -840,8 +830,8 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
         for (uint32_t p = 0; p < P; p++) {
             timing_.stage_ms[kStSort] += span(ev_s0_[p], ev_s1_[p]);
             timing_.stage_ms[kStAccumulate] += span(ev_a0_[p], ev_a1_[p]);
-            timing_.stage_ms[kStMerge] += span(ev_a1_[p], ev_m1_[p]);
         }
+        timing_.stage_ms[kStMerge] = span(ev_a1_[P - 1], ev_m1_[0]);
         timing_.stage_ms[kStReduce] = span(ev_r0_[0], ev_r1_[0]);
         if (dev_tail) timing_.device_tail_ms = span(ev_r1_[0], ev_tail_);
         timing_.stage_ms[kStCopyOut] = span(ev_[kStCopyOut], ev_[kNumStages]);
