@@ -197,7 +197,8 @@ uint32_t zkmsm_default_window_bits(uint64_t n);
  * out[13]: accumulation lanes; out[14]: entries with a nonzero digit (the
  * mixed additions performed); out[15]: device time of the window combine and
  * normalisation when the device ran them (0 when the host did, see
- * zkmsm_set_tail_mode).  Returns the number of values written. */
+ * zkmsm_set_tail_mode); out[16]: pipeline parts of the call.  Returns the
+ * number of values written (at most 17). */
 int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_values);
 
 /* Per-size-class window widths: c_by_log2n[ceil(log2 n)] for 0 <= index < len
@@ -212,6 +213,29 @@ ZkMetalStatus zkmsm_set_window_table(ZkMetalMSMEngine engine, const uint32_t* c_
                                      uint32_t len);
 ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32_t* c_by_log2n,
                                            uint32_t len);
+
+/* Per-size-class launch schedule, indexed by ceil(log2 e) of a call's sorted
+ * entries e (every (MSM, window) digit slot of the call, before zero digits
+ * are dropped) for 0 <= index < len (<= 33 entries):
+ *   parts_by_log2e[k]: pipeline parts the call's segments are split into
+ *     (the sort of part p + 1 runs on a second stream beside the bucket
+ *     accumulation of part p), 1..8;
+ *   reduce_per_by_log2e[k]: buckets each bucket-reduction lane sums before
+ *     the lanes of a row or column meet in a shuffle tree, a power of two
+ *     in 4..64.
+ * 0 keeps the built-in rule for that class; either array may be NULL (all
+ * 0).  batch != 0 sets the table for calls of several MSMs, 0 for calls of
+ * one.  An engine starts from the tables measured on MI355X
+ * (zkmetal_amd/csrc/tuned_gfx950.hpp); they come from the autotuning sweep in
+ * zkmetal_amd/tune.py, and a request server copies them from the engine it
+ * is created on.  zkmsm_set_pipeline_parts, when nonzero, overrides the
+ * parts.  Results do not depend on the schedule.  An out-of-range value
+ * returns ZKMETAL_ERR_INVALID_INPUT and leaves the table unchanged.  (No
+ * counterpart in the reference, whose threadgroup sizes come from
+ * Sources/zkMetal/Tuning.swift:79-246.) */
+ZkMetalStatus zkmsm_set_schedule_table(ZkMetalMSMEngine engine, int32_t batch,
+                                       const uint32_t* parts_by_log2e,
+                                       const uint32_t* reduce_per_by_log2e, uint32_t len);
 
 /* Graph mode.  mode 1 (on): the device launch sequence of a call (all of
  * its MSMs: decomposition, sort, accumulation, merge, reduction, copy-out) is

@@ -1,6 +1,7 @@
 """GPU tests of graph mode (captured launch sequence replayed per call shape)
-and of the window-width autotuner: both change how the work is launched or
-cut into windows, never the result."""
+and of the autotuner (window widths and the per-size-class launch schedule:
+pipeline parts, buckets per reduction lane): all change how the work is
+launched or cut into windows, never the result."""
 import pytest
 
 import zkmetal_amd as zk
@@ -84,14 +85,74 @@ def test_tuner_tables_keep_results(srs):
     before_batch = srs.msm_batch_resident(zk.gen_scalars(SEED + 4, 4 * 1000, 0), [1000] * 4)
     # the sweep runs on its own engine (own seeded table); its tables then
     # go to this one
-    res = tune.tune(None, min_logn=10, max_logn=12, reps=1, spread=1, batch_points=1 << 13)
+    res = tune.tune(None, min_logn=10, max_logn=12, reps=1, spread=1, batch_points=1 << 13,
+                    sched_batch_logn=(13, 14))
     for key in ("plain", "batch"):
         assert set(res[key]) == {10, 11, 12}
         assert all(4 <= c <= 16 for c in res[key].values())
     plain, batch = tune.tables(res)
     assert plain[12] == res["plain"][12] and batch[10] == res["batch"][10]
+    for key in ("plain", "batch"):
+        for cls, (p, g) in res["schedule"][key].items():
+            assert 0 <= p <= 8 and g in (0, 4, 8, 16, 32, 64) and (p or g), (key, cls)
+    assert res["schedule_ms"]["plain"] and res["schedule_ms"]["batch"]
     tune.apply(srs, res)
     assert srs.msm_batch_resident(sc, [3000]) == before
     assert srs.msm_batch_resident(zk.gen_scalars(SEED + 4, 4 * 1000, 0), [1000] * 4) == before_batch
     srs.set_window_table([0] * 33)
     srs.set_window_table([0] * 33, batch=True)
+    srs.set_schedule_table([0] * 33, [0] * 33)
+    srs.set_schedule_table([0] * 33, [0] * 33, batch=True)
+
+
+def _class_of_last_call(eng):
+    return (int(eng.last_timing()["entries"]) - 1).bit_length()
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_schedule_tables_set_parts_and_reduction_tile(srs, batch):
+    counts = [N_SRS, 3000, 1, 20000] if batch else [N_SRS]
+    scal = zk.gen_scalars(SEED + 7, sum(counts), 0)
+    d = zk.DeviceBuffer(scal.nbytes)
+    d.upload(scal)
+    srs.set_graph_mode(srs.GRAPH_OFF)
+    try:
+        srs.set_schedule_table([0] * 33, [0] * 33, batch=batch)
+        want = srs.msm_batch_resident(d, counts)
+        cls = _class_of_last_call(srs)
+        builtin_per = srs.last_timing()["buckets_per_group"]
+        assert builtin_per >= 8
+        for parts, per in ((3, 4), (1, 64), (5, 16), (2, 0), (0, 32)):
+            pt, pr = [0] * 33, [0] * 33
+            pt[cls], pr[cls] = parts, per
+            srs.set_schedule_table(pt, pr, batch=batch)
+            assert srs.msm_batch_resident(d, counts) == want, (parts, per)
+            t = srs.last_timing()
+            if parts:
+                # the plan has at least 3 segments here; with more parts than
+                # segments it makes one part per segment
+                assert 1 <= t["parts"] <= parts, (parts, t["parts"])
+                if parts <= 3:
+                    assert t["parts"] == parts, (parts, t["parts"])
+            if per:
+                assert t["buckets_per_group"] == per
+        # the other table (one MSM vs several) does not apply to this call
+        srs.set_schedule_table([0] * 33, [0] * 33, batch=batch)
+        pt = [0] * 33
+        pt[cls] = 1
+        srs.set_schedule_table(pt, [4] * 33, batch=not batch)
+        assert srs.msm_batch_resident(d, counts) == want
+        assert srs.last_timing()["buckets_per_group"] == builtin_per
+        srs.set_schedule_table([0] * 33, [0] * 33, batch=not batch)
+        # out-of-range values are refused and leave the table as it was
+        for bad_parts, bad_per in ((9, 0), (0, 3), (0, 2), (0, 128), (0, 12)):
+            pt, pr = [0] * 33, [0] * 33
+            pt[cls], pr[cls] = bad_parts, bad_per
+            with pytest.raises(zk.MSMError):
+                srs.set_schedule_table(pt, pr, batch=batch)
+        with pytest.raises(zk.MSMError):
+            srs.set_schedule_table([0] * 33, [0] * 32, batch=batch)
+    finally:
+        srs.set_schedule_table([0] * 33, [0] * 33, batch=batch)
+        srs.set_graph_mode(srs.GRAPH_AUTO)
+        d.free()

@@ -103,6 +103,7 @@ def lib():
         "zkmsm_probe_copy": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
         "zkmsm_set_window_table": (i32, [vp, vp, u32]),
         "zkmsm_set_batch_window_table": (i32, [vp, vp, u32]),
+        "zkmsm_set_schedule_table": (i32, [vp, i32, vp, vp, u32]),
         "zkmsm_set_graph_mode": (i32, [vp, i32]),
         "zkmsm_kzg_commit": (i32, [vp, vp, vp, u32, vp]),
         "zkmsm_srs_load_ptau": (i32, [vp, vp, u64, u32, ctypes.POINTER(u32)]),
@@ -257,7 +258,7 @@ class HostBuffer:
 
 TIMING_KEYS = ["decompose_ms", "sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "copyout_ms",
                "device_ms", "host_tail_ms", "wall_ms", "window_bits", "entries_per_lane",
-               "buckets_per_group", "entries", "lanes", "valid_entries", "device_tail_ms"]
+               "buckets_per_group", "entries", "lanes", "valid_entries", "device_tail_ms", "parts"]
 
 
 class MetalMSM:
@@ -532,6 +533,22 @@ class MetalMSM:
         fn = lib().zkmsm_set_batch_window_table if batch else lib().zkmsm_set_window_table
         _check(fn(self.handle, ctypes.c_void_p(arr.ctypes.data), len(arr)), "set_window_table")
 
+    def set_schedule_table(self, parts=None, reduce_per=None, batch=False):
+        """Launch schedule by size class ceil(log2 entries) of a call:
+        parts[k] pipeline parts (1..8), reduce_per[k] buckets per
+        bucket-reduction lane (a power of two in 4..64); 0 (or None for a
+        whole table) keeps the built-in rule.  batch=True sets the table for
+        calls of several MSMs."""
+        arrs = []
+        for t in (parts, reduce_per):
+            arrs.append(None if t is None else np.ascontiguousarray(np.asarray(t, dtype=np.uint32)))
+        lens = [len(a) for a in arrs if a is not None]
+        if len(set(lens)) > 1:
+            raise MSMError(ERR_INVALID_INPUT, "set_schedule_table: tables of different lengths")
+        ptrs = [None if a is None else ctypes.c_void_p(a.ctypes.data) for a in arrs]
+        _check(lib().zkmsm_set_schedule_table(self.handle, int(bool(batch)), ptrs[0], ptrs[1],
+                                              lens[0] if lens else 0), "set_schedule_table")
+
     GRAPH_OFF, GRAPH_ON, GRAPH_AUTO = 0, 1, 2
 
     def set_graph_mode(self, mode):
@@ -583,8 +600,8 @@ class MetalMSM:
         _check(lib().zkmsm_set_window_bits(self.handle, c), "set_window_bits")
 
     def last_timing(self):
-        v = (ctypes.c_double * 16)()
-        k = lib().zkmsm_last_timing(self.handle, v, 16)
+        v = (ctypes.c_double * 17)()
+        k = lib().zkmsm_last_timing(self.handle, v, 17)
         return {TIMING_KEYS[i]: v[i] for i in range(k)}
 
     def probe_rates(self):

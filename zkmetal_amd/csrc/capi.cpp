@@ -445,7 +445,7 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     if (!e || !out) return 0;
     std::lock_guard<std::mutex> lk(e->mutex());
     const zk::Timing& t = e->last_timing();
-    double v[16];
+    double v[17];
     for (int s = 0; s < zk::kNumStages; s++) v[s] = t.stage_ms[s];
     v[6] = t.device_ms;
     v[7] = t.host_tail_ms;
@@ -457,7 +457,8 @@ int32_t zkmsm_last_timing(ZkMetalMSMEngine engine, double* out, int32_t max_valu
     v[13] = (double)t.lanes;
     v[14] = (double)t.valid_entries;
     v[15] = t.device_tail_ms;
-    int32_t k = max_values < 16 ? max_values : 16;
+    v[16] = t.parts;
+    int32_t k = max_values < 17 ? max_values : 17;
     for (int32_t i = 0; i < k; i++) out[i] = v[i];
     return k;
 }
@@ -585,6 +586,14 @@ ZkMetalStatus zkmsm_set_batch_window_table(ZkMetalMSMEngine engine, const uint32
     if (!e) return ZKMETAL_ERR_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(e->mutex());
     return e->set_window_table(c_by_log2n, len, true);
+}
+
+ZkMetalStatus zkmsm_set_schedule_table(ZkMetalMSMEngine engine, int32_t batch, const uint32_t* parts_by_log2e,
+                                       const uint32_t* reduce_per_by_log2e, uint32_t len) {
+    Engine* e = as_engine(engine);
+    if (!e) return ZKMETAL_ERR_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(e->mutex());
+    return e->set_schedule_table(parts_by_log2e, reduce_per_by_log2e, len, batch != 0);
 }
 
 // ---------------------------------------------------------------- server

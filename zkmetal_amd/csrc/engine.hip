@@ -146,9 +146,23 @@ bool shifted_pays(uint64_t n, uint32_t c, uint32_t c_s) {
 // against 0.61 / 0.65 / 1.71 / 6.05 ms plain at 2^16 / 2^18 / 2^20 / 2^22.
 constexpr uint64_t kShiftSingleMax = 1u << 17;
 
+// ceil(log2 x), at most 32
+static uint32_t size_class(uint64_t x) {
+    uint32_t k = 0;
+    while (k < 32 && (1ull << k) < x) k++;
+    return k;
+}
+
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const uint32_t* cs,
               Plan& pl, uint32_t form, uint64_t lane_cap, CurveId cv, uint32_t nparts,
-              uint64_t shift_stride, uint32_t shift_c, uint32_t shift_floor) {
+              uint64_t shift_stride, uint32_t shift_c, uint32_t shift_floor, const Schedule* sched) {
+    // parts for a call of `entries` entries: the request, else the
+    // schedule's entry for its size class, else by size
+    auto parts_for = [&](uint64_t entries) -> uint32_t {
+        if (nparts) return nparts;
+        if (sched && sched->parts[size_class(entries)]) return sched->parts[size_class(entries)];
+        return default_parts(entries);
+    };
     plan_reset(pl);
     pl.curve = cv;
     const uint32_t eps = entries_per_point(cv);
@@ -160,7 +174,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     if (can_shift && nmsm) {
         uint64_t est = 0;
         for (uint32_t j = 0; j < nmsm; j++) est += (uint64_t)eps * ns[j] * ((128 + shift_c - 1) / shift_c);
-        const uint32_t P = nparts ? nparts : default_parts(est);
+        const uint32_t P = parts_for(est);
         if (nmsm < P) spm = (P + nmsm - 1) / nmsm;
     }
     pl.msms.resize(nmsm);
@@ -217,7 +231,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     if (pl.entries >= (1ull << 31) || pl.buckets >= (1ull << 30)) return kInvalidInput;
     const uint32_t nseg = (uint32_t)pl.segs.size();
     // parts: consecutive segment ranges of about entries / P entries each
-    uint32_t P = nparts ? nparts : default_parts(pl.entries);
+    uint32_t P = parts_for(pl.entries);
     P = std::max<uint32_t>(1, std::min<uint32_t>({P, kMaxParts, nseg}));
     {
         uint32_t s = 0;
@@ -268,6 +282,7 @@ int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm, const u
     // up to 32 so that fewer additions go to tree levels
     pl.G = 8;
     while (pl.G < 32 && (pl.buckets >> 17) >= 2 * pl.G) pl.G *= 2;
+    if (sched && sched->per[size_class(pl.entries)]) pl.G = sched->per[size_class(pl.entries)];
     pl.seg_lane0.resize(nseg + 1);
     pl.seg_grp0.resize(nseg + 1);
     pl.seg_rcw0.resize(nseg + 1);
@@ -342,6 +357,10 @@ int32_t Engine::create(Engine** out) {
     e->acc_cap_g2_ = accumulate_capacity(e->dev_, CurveId::Bn254G2);
     memcpy(e->ctab_, kTunedPlain, sizeof(e->ctab_));
     memcpy(e->ctab_batch_, kTunedBatch, sizeof(e->ctab_batch_));
+    for (int b = 0; b < 2; b++) {
+        memcpy(e->sched_[b].parts, kTunedParts[b], sizeof(e->sched_[b].parts));
+        memcpy(e->sched_[b].per, kTunedPer[b], sizeof(e->sched_[b].per));
+    }
     for (auto& ev : e->ev_) {
         if (hipEventCreate(&ev) != hipSuccess) {
             delete e;
@@ -852,6 +871,7 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     timing_.window_bits_first = nmsm ? pl.msms[0].c : 0;
     timing_.T = pl.T;
     timing_.G = pl.G;
+    timing_.parts = pl.nparts;
     timing_.entries = pl.entries;
     timing_.buckets = pl.buckets;
     timing_.lanes = pl.lanes;
@@ -1003,7 +1023,7 @@ int32_t Engine::msm_batch_host(const uint8_t* points, const uint8_t* scalars,
     }
     Plan pl;
     rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                   lane_cap(cv), cv, parts_req_);
+                   lane_cap(cv), cv, parts_req_, 0, 0, 0, schedule((uint32_t)ns.size()));
     if (rc != kOk) return rc;
     ZK_ALLOC(pts_in_, total * input_point_bytes(cv));
     ZK_ALLOC(ptable_, table_bytes(total, cv));
@@ -1058,7 +1078,7 @@ int32_t Engine::msm_device(const void* d_points, const void* d_scalars, const ui
     launch_build_table(cv, d_points, ptable_.p, total, stream_);
     Plan& pl = plan_;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns, cv).data(), pl, 0,
-                       lane_cap(cv), cv, parts_req_);
+                       lane_cap(cv), cv, parts_req_, 0, 0, 0, schedule((uint32_t)ns.size()));
     if (rc != kOk) return rc;
     std::vector<uint8_t> res(ns.size() * result_bytes(cv));
     rc = run(pl, d_scalars, ptable_.p, total, res.data());
@@ -1411,7 +1431,7 @@ int32_t Engine::msm_table_parts(const void* table, uint32_t table_n,
     Plan& pl = plan_;
     int rc = make_plan(ns.data(), t0.data(), (uint32_t)ns.size(), window_bits_for(ns).data(), pl, form, acc_cap_,
                        CurveId::Bn254, parts_req_, plan_shift(table, shift_stride), kShiftC,
-                       shift_floor((uint32_t)ns.size()));
+                       shift_floor((uint32_t)ns.size()), schedule((uint32_t)ns.size()));
     if (rc != kOk) return rc;
     const void* d_sc = d_scalars;
     if (host_parts) {
@@ -1458,6 +1478,21 @@ int32_t Engine::set_window_table(const uint32_t* c_by_log2n, uint32_t len, bool 
         t[i] = c;
     }
     memcpy(batch ? ctab_batch_ : ctab_, t, sizeof(t));
+    return kOk;
+}
+
+int32_t Engine::set_schedule_table(const uint32_t* parts, const uint32_t* per, uint32_t len, bool batch) {
+    Schedule t{};
+    for (uint32_t i = 0; i <= 32; i++) {
+        const uint32_t p = (parts && i < len) ? parts[i] : 0;
+        const uint32_t g = (per && i < len) ? per[i] : 0;
+        // per: a power of two from 4 (a column of up to 256 buckets then has
+        // at most 64 lanes, one wave's shuffle tree) to 64
+        if (p > kMaxParts || (g != 0 && (g < 4 || g > 64 || (g & (g - 1)) != 0))) return kInvalidInput;
+        t.parts[i] = (uint8_t)p;
+        t.per[i] = (uint8_t)g;
+    }
+    sched_[batch ? 1 : 0] = t;
     return kOk;
 }
 

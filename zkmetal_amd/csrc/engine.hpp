@@ -68,7 +68,7 @@ struct Timing {
     float host_tail_ms;          // window combine + normalisation on the host
     float device_tail_ms;        // the same on the device (0 when the host ran it)
     float wall_ms;               // whole call, host clock
-    uint32_t nmsm, window_bits_first, T, G;
+    uint32_t nmsm, window_bits_first, T, G, parts;
     uint64_t entries, buckets, lanes;
     uint64_t valid_entries;  // sorted entries with a nonzero digit (mixed additions done)
 };
@@ -124,9 +124,20 @@ WsSizes ws_sizes(const Plan& pl, bool dev_tail);
 // window width for an MSM of n points (cost model; overridable)
 uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
 
+// Launch schedule by size class ceil(log2 entries) of a call (its sorted
+// entries before the zero digits are dropped): parts[k] the pipeline parts
+// (1..kMaxParts), per[k] the buckets each bucket-reduction lane sums before
+// the shuffle tree (4..64, a power of two); 0 keeps the built-in rule.
+struct Schedule {
+    uint8_t parts[33];
+    uint8_t per[33];
+};
+
 // build a plan; ns[j] > 0 for all j; table0[j] = first table entry of MSM j;
-// nparts: how many parts to split the segments into (0: by size, see
-// default_parts; at most kMaxParts and the segment count); shift_stride > 0:
+// nparts: how many parts to split the segments into (0: the schedule's entry
+// for the call's size class, else by size, see default_parts; at most
+// kMaxParts and the segment count); sched (nullable): the size-class
+// schedule, which also sets the reduction's buckets per lane; shift_stride > 0:
 // the table also holds window-shifted copies for window width shift_c, copy w
 // at entry w * shift_stride (a multiple of kSortTile), and MSMs run
 // window-shifted at width shift_c when shifted_pays or (shift_floor > 0) they
@@ -134,7 +145,8 @@ uint32_t default_window_bits(uint64_t n, CurveId cv = CurveId::Bn254);
 int make_plan(const uint32_t* ns, const uint64_t* table0, uint32_t nmsm,
               const uint32_t* cs /* nullable */, Plan& pl, uint32_t form = 0,
               uint64_t lane_cap = 1u << 17, CurveId cv = CurveId::Bn254, uint32_t nparts = 0,
-              uint64_t shift_stride = 0, uint32_t shift_c = 0, uint32_t shift_floor = 0);
+              uint64_t shift_stride = 0, uint32_t shift_c = 0, uint32_t shift_floor = 0,
+              const Schedule* sched = nullptr);
 uint32_t default_parts(uint64_t entries);
 bool shifted_pays(uint64_t n, uint32_t c, uint32_t c_s);
 
@@ -235,9 +247,16 @@ class Engine {
     // per-size-class window widths for single-MSM calls (batch = false) and
     // for MSMs inside batched calls (batch = true); 0 = cost-model default
     int32_t set_window_table(const uint32_t* c_by_log2n, uint32_t len, bool batch = false);
+    // per-size-class launch schedule (class ceil(log2 entries) of the call):
+    // pipeline parts (0..kMaxParts) and the bucket reduction's buckets per
+    // lane (0 or a power of two in 4..64); 0 keeps the built-in rule.  batch:
+    // the table for calls of several MSMs, else for calls of one
+    int32_t set_schedule_table(const uint32_t* parts, const uint32_t* per, uint32_t len, bool batch);
+    const Schedule* schedule(uint32_t nmsm) const { return &sched_[nmsm > 1 ? 1 : 0]; }
     void copy_tuning_from(const Engine& o) {
         memcpy(ctab_, o.ctab_, sizeof(ctab_));
         memcpy(ctab_batch_, o.ctab_batch_, sizeof(ctab_batch_));
+        memcpy(sched_, o.sched_, sizeof(sched_));
         c_override_ = o.c_override_;
         shift_mode_ = o.shift_mode_;
     }
@@ -336,6 +355,7 @@ class Engine {
     uint32_t c_override_ = 0;
     uint32_t ctab_[33] = {};
     uint32_t ctab_batch_[33] = {};
+    Schedule sched_[2] = {};  // [0]: calls of one MSM, [1]: of several
     int32_t graph_mode_ = kGraphAuto;
     std::vector<uint64_t> last_key_;  // the previous call's shape key (kGraphAuto)
     int32_t tail_mode_ = kTailAuto;
