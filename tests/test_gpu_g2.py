@@ -326,3 +326,31 @@ def test_skewed_scalars_against_the_c_pippenger_at_2_14(engine):
     got = engine.g2_msm(pts, sc.tobytes())
     want = ref.g2_pippenger(pts, sc.tobytes(), n, 0, 8)
     assert G.jacobian_bytes_to_affine(got) == G.jacobian_bytes_to_affine(want)
+
+
+@pytest.mark.parametrize("parts,per", [(3, 4), (1, 64)])
+def test_schedule_tables_keep_results(bases, parts, per):
+    """Pipeline parts and the bucket-reduction tile change how a G2 call is
+    launched, never its result (closed form, single and batched)."""
+    pts, ks = bases
+    counts = [700, 1, 1100]
+    rnd = random.Random(36 + parts)
+    ss = [rnd.randrange(R) for _ in range(sum(counts))]
+    cat = []
+    for c in counts:
+        cat += pts[:c]
+    eng = zk.MetalMSM()
+    try:
+        eng.set_graph_mode(eng.GRAPH_OFF)
+        for batch in (False, True):
+            eng.set_schedule_table([parts] * 33, [per] * 33, batch=batch)
+        res = eng.g2_msm_batch(pbytes(cat), sc_bytes(ss), counts)
+        assert eng.last_timing()["buckets_per_group"] == per
+        one = eng.g2_msm(pbytes(pts[:counts[2]]), sc_bytes(ss[counts[0] + counts[1]:]))
+        assert one == res[2]
+        off = 0
+        for j, c in enumerate(counts):
+            assert G.jacobian_bytes_to_affine(res[j]) == G.msm_closed_form(ks[:c], ss[off:off + c]), j
+            off += c
+    finally:
+        eng.close()

@@ -265,3 +265,32 @@ def test_device_entry_closed_form_at_2_20_and_batch(engine):
     lo = G.mul(G.G, ref.scalar_dot(sc[:1000].tobytes(), ks[:1000].tobytes(), 1000, curve="grumpkin"))
     assert G.jacobian_bytes_to_affine(rb[0]) == lo
     assert G.add(G.jacobian_bytes_to_affine(rb[0]), G.jacobian_bytes_to_affine(rb[2])) == want
+
+
+@pytest.mark.parametrize("parts,per", [(3, 4), (1, 64), (8, 16)])
+def test_schedule_tables_keep_results(bases, parts, per):
+    """Pipeline parts and the bucket-reduction tile change how a Grumpkin call
+    is launched, never its result (closed form, single and batched)."""
+    pts, ks = bases
+    counts = [3000, 1, 2000]
+    rnd = random.Random(26 + parts)
+    ss = [rnd.randrange(G.ORDER) for _ in range(sum(counts))]
+    cat = []
+    for c in counts:
+        cat += pts[:c]
+    eng = zk.MetalMSM()
+    try:
+        eng.set_graph_mode(eng.GRAPH_OFF)
+        for batch in (False, True):
+            eng.set_schedule_table([parts] * 33, [per] * 33, batch=batch)
+        res = eng.grumpkin_msm_batch(pbytes(cat), sc_bytes(ss), counts)
+        assert eng.last_timing()["buckets_per_group"] == per
+        one = eng.grumpkin_msm(pbytes(pts[:counts[0]]), sc_bytes(ss[:counts[0]]))
+        assert eng.last_timing()["buckets_per_group"] == per
+        assert one == res[0]
+        off = 0
+        for j, c in enumerate(counts):
+            assert G.jacobian_bytes_to_affine(res[j]) == G.msm_closed_form(ks[:c], ss[off:off + c]), j
+            off += c
+    finally:
+        eng.close()
