@@ -579,14 +579,14 @@ def serve(args, eng):
 
 
 def profile(args, eng):
-    """Profiling run for rocprofv3 (kernel traces, PMC counters): one plain
-    copy kernel over --copy-mib MiB (known traffic: that many bytes read and
-    written, the HBM counters' calibration), then --steps calls of the
-    workload with their stage timings, one JSON line each.  --config plain,
-    grumpkin or g2: plain MSMs of 2^logn points, bases and scalars resident;
-    batched: the 64-MSM call over a resident 2^20 table."""
-    copy_ms = eng.probe_copy(args.copy_mib << 20)
-    print(json.dumps({"copy_bytes": args.copy_mib << 20, "copy_ms": round(copy_ms, 4)}), flush=True)
+    """Profiling run for rocprofv3 (kernel traces, PMC counters): the
+    workload's setup and --warmup untimed calls, then one plain copy kernel
+    over --copy-mib MiB (known traffic: that many bytes read and written, the
+    HBM counters' calibration; it also marks where the measured calls start
+    in a trace), then --steps calls of the workload with their stage timings,
+    one JSON line each.  --config plain, grumpkin or g2: plain MSMs of 2^logn
+    points, bases and scalars resident; batched / skewed: the 64-MSM and the
+    32 x 2^18 witness-like calls over a resident 2^20 table."""
     n = 1 << args.logn
     bufs = []
     if args.config in ("batched", "skewed"):
@@ -623,6 +623,10 @@ def profile(args, eng):
 
         def run():
             fn(dp, ds, [n])
+    for _ in range(args.warmup):
+        run()
+    copy_ms = eng.probe_copy(args.copy_mib << 20)
+    print(json.dumps({"copy_bytes": args.copy_mib << 20, "copy_ms": round(copy_ms, 4)}), flush=True)
     for r in range(args.steps):
         run()
         print(json.dumps({"step": r, **{k: round(v, 4) for k, v in eng.last_timing().items()}}), flush=True)
