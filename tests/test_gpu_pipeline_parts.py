@@ -1,7 +1,9 @@
-"""The part pipeline (bucket sort of part p + 1 on one stream while part p
-accumulates, part p's reduction on a third stream) gives the same bytes for
-every number of parts: single MSMs, batches of mixed sizes, a skewed batch,
-Grumpkin and G2, eager and captured as a graph; and the closed form."""
+"""The part pipeline (bucket sort of part p + 1 on a second stream while part
+p accumulates; one merge of every part's crossing partials and one bucket
+reduction after the last accumulation) gives the same bytes for every number
+of parts: single MSMs, batches of mixed sizes, a skewed batch (heavy buckets
+through the long-run merge), Grumpkin and G2, eager and captured as a graph;
+and the closed form."""
 import numpy as np
 import pytest
 

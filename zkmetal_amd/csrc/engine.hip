@@ -681,11 +681,11 @@ int32_t Engine::run_ws(const Plan& pl, const Workspace& ws, const WsSizes& z, bo
     const uint32_t list_cap = (uint32_t)std::min<uint64_t>(nrec / (kRunLimit + 1) + 1, 1u << 29);
     uint8_t* hw = ws.hw;
     // The device part of the call.  Stream A (the engine stream): descriptors,
-    // decomposition, then per part the accumulation and the merge, then the
-    // bucket reduction and the device tail; stream S: the two sort passes and
-    // the lane plan of each part, so that part p + 1 sorts while part p
-    // accumulates (the accumulation leaves registers and LDS for the sort's
-    // workgroups).
+    // decomposition, then the accumulation of each part, one merge of every
+    // part's records, the bucket reduction and the device tail; stream S: the
+    // two sort passes and the lane plan of each part, so that part p + 1 sorts
+    // while part p accumulates (the accumulation leaves registers and LDS for
+    // the sort's workgroups).
     // Timing events are recorded only outside graph capture; the dependency
     // events are part of a captured graph.
     auto enqueue = [&](bool events) -> int32_t {
