@@ -55,3 +55,26 @@ def test_config_matches_the_golden_fixture(engine, idx):
     finally:
         ds.free()
     assert got == expected_point(case)
+
+
+@pytest.mark.parametrize("detect", [True, False])
+def test_drop_in_host_call_at_2_22(engine, detect):
+    """The zkMetal entry point a caller binds (zkmetal_bn254_msm, host
+    buffers, PCIe included) at the headline size: with the bases recognised
+    as the resident SRS (no base upload), and with the recognition off (bases
+    uploaded and the table built for the call), against the golden result."""
+    fx = load()
+    case = next(c for c in fx["cases"] if c["curve"] == "bn254" and c["n"] == 1 << 22)
+    n = case["n"]
+    print(f"zkmetal_bn254_msm n={n} srs_detect={detect}", flush=True)
+    engine.srs_generate(fx["seed_bases"], n)
+    bases = engine.srs_download(n)
+    sc = zk.gen_scalars(fx["seed_scalars"], n, 0)
+    engine.set_srs_detect(detect)
+    try:
+        hits0 = engine.srs_detect_hits()
+        got = result_affine(engine.msm(bases, sc))
+        assert (engine.srs_detect_hits() > hits0) == detect
+    finally:
+        engine.set_srs_detect(True)
+    assert got == expected_point(case)
