@@ -80,6 +80,12 @@ def test_null_engine_is_invalid_input():
     assert L.zkmsm_bn254_g2_msm_batch_device(None, buf, buf, buf, 1, buf) == zk.ERR_INVALID_INPUT
     assert L.zkmsm_bn254_g2_points_generate(None, 1, 1, buf) == zk.ERR_INVALID_INPUT
     assert L.zkmsm_test_g2(None, 0, buf, buf, buf, 1) == zk.ERR_INVALID_INPUT
+    # and so do the tuning tables
+    tab = (ctypes.c_uint32 * 33)()
+    assert L.zkmsm_set_window_table(None, tab, 33) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_set_batch_window_table(None, tab, 33) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_set_schedule_table(None, 0, tab, tab, 33) == zk.ERR_INVALID_INPUT
+    assert L.zkmsm_set_schedule_table(None, 1, None, None, 0) == zk.ERR_INVALID_INPUT
     L.zkmetal_msm_engine_destroy(None)
 
 
