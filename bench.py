@@ -177,6 +177,14 @@ def plain(args, eng, peak):
         "msm_algorithmic_field_muls": algorithmic_field_muls(n),
         "measured_rates": {k: float("%.4g" % v) for k, v in args.rates.items()},
     }
+    # the same call repeated under the library's default graph mode (auto:
+    # the second call of a shape captures, later ones replay one hipGraph);
+    # the headline above is eager, whose per-stage HIP events time the
+    # accumulation for the roofline
+    eng.set_graph_mode(eng.GRAPH_AUTO)
+    ms_rep, _ = timed_steps(lambda: eng.msm_batch_device(dpts, dsc, [n]), args.steps, max(2, args.warmup))
+    eng.set_graph_mode(eng.GRAPH_OFF)
+    extra["ms_graph_replay"] = round(ms_rep, 4)
     # PCIe-inclusive host-buffer calls (zkmetal_bn254_msm): bases that are
     # the resident SRS (detected, no base upload) and the same call with the
     # detection off (bases uploaded, table rebuilt); not the headline value
