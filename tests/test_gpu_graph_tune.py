@@ -156,3 +156,35 @@ def test_schedule_tables_set_parts_and_reduction_tile(srs, batch):
         srs.set_schedule_table([0] * 33, [0] * 33, batch=batch)
         srs.set_graph_mode(srs.GRAPH_AUTO)
         d.free()
+
+
+def test_extreme_schedule_at_2_20_matches_the_closed_form():
+    """Eight parts and 64 buckets per reduction lane on one 2^20-point MSM
+    (device bases, table built per call) give the closed form."""
+    from helpers import result_affine
+    from oracle import bn254 as B
+    from oracle import ref
+
+    n = 1 << 20
+    e = zk.MetalMSM()
+    try:
+        e.set_graph_mode(e.GRAPH_OFF)
+        e.srs_generate(SEED + 11, n)
+        bases = e.srs_download(n)
+        sc = zk.gen_scalars(SEED + 12, n, 0)
+        dp, ds = zk.DeviceBuffer(bases.nbytes), zk.DeviceBuffer(sc.nbytes)
+        dp.upload(bases)
+        ds.upload(sc)
+        try:
+            for parts, per in ((8, 64), (1, 4)):
+                e.set_schedule_table([parts] * 33, [per] * 33)
+                got = result_affine(e.msm_batch_device(dp, ds, [n])[0])
+                t = e.last_timing()
+                assert t["buckets_per_group"] == per and t["parts"] == parts
+                ks = zk.gen_scalars(SEED + 11, n, 0).tobytes()
+                assert got == B.mul(B.G, ref.scalar_dot(sc.tobytes(), ks, n)), (parts, per)
+        finally:
+            dp.free()
+            ds.free()
+    finally:
+        e.close()
